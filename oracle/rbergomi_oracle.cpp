@@ -1,0 +1,542 @@
+/*
+ * rbergomi_oracle.cpp -- CPU restatement of the reference rough-Bergomi pricing path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker and the timed CPU baseline).  The product library
+ * under rbergomi_amd/ never links or calls this file.  See rbergomi_oracle.h for the list of
+ * checks that pin each function.
+ *
+ * Reference semantics restated here (paths relative to the reference repository root):
+ *   rBergLib/RBergomi.cpp:157-232, 381-491   single-thread class (the semantic oracle)
+ *   rBergLib/rBergomiMT.cpp:63-172, 301-610  OpenMP drivers and numerics
+ *   rBergLib/BlackScholes.cpp:16-92          Phi, BS prices, implied vol
+ *   rBergLib/ParamTot.h:20-50                index algebra and memoisation triggers
+ *   rBergLib/cinterface.cpp:49-131           C ABI layouts
+ */
+#include "rbergomi_oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <limits>
+#include <random>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+typedef std::vector<double> Vec;
+
+/* ParamTot: (H, eta, rho) zipped into par_size triples, (T, K) zipped into op_size pairs,
+ * I = h*op_size + t; a trigger fires when the value differs from the one at I-1 by more
+ * than 1e-10.  rBergLib/ParamTot.h:20-50, rBergLib/ParamTot.cpp:18-29 */
+struct Params {
+    const double *H, *eta, *rho, *T, *K;
+    long par_size, op_size;
+    double eps = 1e-10;
+    long size() const { return par_size * op_size; }
+    double h(long I) const { return H[I / op_size]; }
+    double e(long I) const { return eta[I / op_size]; }
+    double r(long I) const { return rho[I / op_size]; }
+    double t(long I) const { return T[I % op_size]; }
+    double k(long I) const { return K[I % op_size]; }
+    bool HTrig(long I) const { return I == 0 || std::fabs(h(I) - h(I - 1)) > eps; }
+    bool etaTrig(long I) const { return I == 0 || std::fabs(e(I) - e(I - 1)) > eps; }
+    bool rhoTrig(long I) const { return I == 0 || std::fabs(r(I) - r(I - 1)) > eps; }
+    bool TTrig(long I) const { return I == 0 || std::fabs(t(I) - t(I - 1)) > eps; }
+};
+
+/* In-place iterative radix-2 FFT (Cooley-Tukey, bit reversal then butterflies); sign -1
+ * forward, +1 backward (unnormalised).  n must be a power of two. */
+void fft_pow2(std::complex<double>* a, int n, int sign) {
+    for (int i = 1, j = 0; i < n; ++i) {
+        int bit = n >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j) std::swap(a[i], a[j]);
+    }
+    for (int len = 2; len <= n; len <<= 1) {
+        const double ang = sign * 2.0 * M_PI / len;
+        const std::complex<double> wl(std::cos(ang), std::sin(ang));
+        for (int i = 0; i < n; i += len) {
+            std::complex<double> w(1.0, 0.0);
+            for (int j = 0; j < len / 2; ++j) {
+                std::complex<double> u = a[i + j], v = a[i + j + len / 2] * w;
+                a[i + j] = u + v;
+                a[i + j + len / 2] = u - v;
+                w *= wl;
+            }
+        }
+    }
+}
+
+/* FFT workspace of one thread: the reference keeps three complex arrays and plans per
+ * thread (fftData, rBergLib/rBergomiMT.cpp:34-61); here the length is the power of two
+ * >= 2N-1 instead of 2N-1 (a linear convolution is exact for any length >= 2N-1). */
+struct FFTWork {
+    int n = 0;
+    std::vector<std::complex<double>> x, y;
+    explicit FFTWork(int N) {
+        n = 1;
+        while (n < 2 * N - 1) n <<= 1;
+        x.resize(n);
+        y.resize(n);
+    }
+    /* three transforms per call, as updateWtilde does (:93-102) */
+    void convolve(const double* a, const double* b, int N, double* z) {
+        for (int i = 0; i < n; ++i) {
+            x[i] = i < N ? std::complex<double>(a[i], 0.0) : 0.0;
+            y[i] = i < N ? std::complex<double>(b[i], 0.0) : 0.0;
+        }
+        fft_pow2(x.data(), n, -1);
+        fft_pow2(y.data(), n, -1);
+        for (int i = 0; i < n; ++i) x[i] *= y[i];
+        fft_pow2(x.data(), n, +1);
+        for (int i = 0; i < N; ++i) z[i] = x[i].real() / n;
+    }
+};
+
+/* updateWtilde, rBergLib/rBergomiMT.cpp:83-109: W1hat from the 2x2 first-cell Cholesky,
+ * Y2 = Gamma * W1 (truncated), Wtilde = sqrt(2H)*N^-H*(Y2 + W1hat). */
+void wtilde_impl(const double* W1, const double* W1perp, const double* gamma, double H, int N,
+                 FFTWork* fft, double* Wt) {
+    const double s2H = std::sqrt(2.0 * H);
+    const double rhoH = s2H / (H + 0.5);
+    const double a = rhoH / s2H, b = std::sqrt(1.0 - rhoH * rhoH) / s2H;
+    Vec Y2(N);
+    if (fft)
+        fft->convolve(W1, gamma, N, Y2.data());
+    else
+        orc_convolve_direct(W1, gamma, N, Y2.data());
+    const double sc = s2H * std::pow(1.0 / N, H);
+    for (int i = 0; i < N; ++i) Wt[i] = sc * Y2[i] + sc * (a * W1[i] + b * W1perp[i]);
+}
+
+/* scaleWtilde + updateV, rBergLib/rBergomiMT.cpp:163-167 and :73-80 */
+void v_impl(const double* Wt, double xi, double H, double eta, double T, int N, int comp,
+            double* v) {
+    const double dt = T / N;
+    const double TH = std::pow(T, H);
+    v[0] = xi;
+    for (int i = 1; i < N; ++i) {
+        const double tc = comp == ORC_COMP_EXACT ? i * dt : (i - 1) * dt;
+        v[i] = xi * std::exp(eta * (TH * Wt[i - 1]) - 0.5 * eta * eta * std::pow(tc, 2 * H));
+    }
+}
+
+/* updateZ + scaleZ + updateS, rBergLib/rBergomiMT.cpp:111-136 */
+double s_impl(const double* v, const double* W1, const double* Wperp, double rho, double dt,
+              int N) {
+    const double sdt = std::sqrt(dt), c = std::sqrt(1.0 - rho * rho);
+    double X = 0.0;
+    for (int i = 0; i < N; ++i) {
+        const double Z = rho * W1[i] + c * Wperp[i];
+        X += std::sqrt(v[i]) * (sdt * Z) - 0.5 * v[i] * dt;
+    }
+    return std::exp(X);
+}
+
+/* One path through the options loop with the memoisation triggers of
+ * ComputePayoffRTsamples_ST (rBergLib/rBergomiMT.cpp:576-606).  Writes the payoff (and the
+ * two integrals, if ints != nullptr) of every option I. */
+struct PathState {
+    Vec Wt, v;
+    double dt = 0.0, sdt = 0.0;
+    explicit PathState(int N) : Wt(N), v(N) {}
+};
+
+void rt_path(const Params& par, double xi, int N, const double* W1, const double* W1perp,
+             int comp, FFTWork* fft, const std::vector<Vec>& gammas, PathState& st,
+             double* payoff, double* ints) {
+    for (long I = 0; I < par.size(); ++I) {
+        bool update = par.HTrig(I);
+        if (update) wtilde_impl(W1, W1perp, gammas[I / par.op_size].data(), par.h(I), N, fft, st.Wt.data());
+        update = update || par.TTrig(I);
+        if (update) {
+            st.dt = par.t(I) / N;
+            st.sdt = std::sqrt(st.dt);
+        }
+        update = update || par.etaTrig(I);
+        if (update) v_impl(st.Wt.data(), xi, par.h(I), par.e(I), par.t(I), N, comp, st.v.data());
+        const double Iv = orc_int_vdt(st.v.data(), N, st.dt);
+        const double Is = orc_int_rootvdw(st.v.data(), W1, N, st.sdt);
+        payoff[I] = orc_rt_payoff(Iv, Is, par.r(I), par.k(I));
+        if (ints) {
+            ints[2 * I] = Iv;
+            ints[2 * I + 1] = Is;
+        }
+    }
+}
+
+std::vector<Vec> gamma_tables(const Params& par, int N) {
+    std::vector<Vec> g(par.par_size, Vec(N));
+    for (long h = 0; h < par.par_size; ++h) orc_gamma(par.H[h], N, g[h].data());
+    return g;
+}
+
+double posPart(double x) { return x > 0 ? x : 0; }
+
+/* Black-Scholes price as a function of sigma for the root solver */
+double bs_otm(double S0, double K, double tau, double sigma, bool call) {
+    return call ? orc_bs_call(S0, K, tau, sigma) : orc_bs_put(S0, K, tau, sigma);
+}
+
+/* 64-bit finaliser of splitmix64 (Steele, Lea, Flood, OOPSLA'14) */
+uint64_t splitmix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+}  // namespace
+
+extern "C" {
+
+void orc_gamma(double H, int N, double* gamma) {
+    const double alpha = H - 0.5;
+    if (N <= 0) return;
+    gamma[0] = 0.0;
+    for (int i = 1; i < N; ++i)
+        gamma[i] = (std::pow(i + 1.0, alpha + 1.0) - std::pow(double(i), alpha + 1.0)) / (alpha + 1.0);
+}
+
+void orc_convolve_direct(const double* x, const double* y, int n, double* z) {
+    for (int i = 0; i < n; ++i) {
+        double s = 0.0;
+        for (int k = 0; k <= i; ++k) s += x[k] * y[i - k];
+        z[i] = s;
+    }
+}
+
+void orc_convolve_fft(const double* x, const double* y, int n, double* z) {
+    FFTWork w(n);
+    w.convolve(x, y, n, z);
+}
+
+void orc_wtilde(const double* W1, const double* W1perp, double H, int N, int conv_mode,
+                double* Wtilde) {
+    Vec g(N);
+    orc_gamma(H, N, g.data());
+    if (conv_mode == ORC_CONV_FFT) {
+        FFTWork w(N);
+        wtilde_impl(W1, W1perp, g.data(), H, N, &w, Wtilde);
+    } else {
+        wtilde_impl(W1, W1perp, g.data(), H, N, nullptr, Wtilde);
+    }
+}
+
+void orc_update_v(const double* Wtilde, double xi, double H, double eta, double T, int N,
+                  int comp_mode, double* v) {
+    v_impl(Wtilde, xi, H, eta, T, N, comp_mode, v);
+}
+
+double orc_int_vdt(const double* v, int N, double dt) {
+    double s = 0.0;
+    for (int i = 0; i < N; ++i) s += v[i];
+    return dt * s;
+}
+
+double orc_int_rootvdw(const double* v, const double* W1, int N, double sdt) {
+    double s = 0.0;
+    for (int i = 0; i < N; ++i) s += std::sqrt(v[i]) * sdt * W1[i];
+    return s;
+}
+
+double orc_phi(double x) { return 0.5 * std::erfc(-x * M_SQRT1_2); }
+
+double orc_bs_call(double S0, double K, double tau, double sigma) {
+    const double d1 = (std::log(S0 / K) + 0.5 * sigma * sigma * tau) / (sigma * std::sqrt(tau));
+    const double d2 = d1 - sigma * std::sqrt(tau);
+    return orc_phi(d1) * S0 - orc_phi(d2) * K;
+}
+
+double orc_bs_put(double S0, double K, double tau, double sigma) {
+    const double d1 = (std::log(S0 / K) + 0.5 * sigma * sigma * tau) / (sigma * std::sqrt(tau));
+    const double d2 = d1 - sigma * std::sqrt(tau);
+    return -orc_phi(-d1) * S0 + orc_phi(-d2) * K;
+}
+
+double orc_iv_call(double price, double S0, double K, double tau) {
+    /* S0 > K: solve on the out-of-the-money put through put-call parity
+     * (rBergLib/BlackScholes.cpp:36-40, r = 0) */
+    bool call = true;
+    double target = price;
+    if (S0 > K) {
+        call = false;
+        target = price - S0 + K;
+    }
+    const double upper = call ? S0 : K; /* sup over sigma of the OTM price */
+    if (!(target > 0.0) || !(target < upper) || !(tau > 0.0)) return std::nan("");
+    double lo = 0.0, hi = 1.0;
+    while (bs_otm(S0, K, tau, hi, call) < target) {
+        lo = hi;
+        hi *= 2.0;
+        if (hi > 1e6) return std::nan("");
+    }
+    double s = 0.5 * (lo + hi);
+    for (int it = 0; it < 200; ++it) {
+        const double f = bs_otm(S0, K, tau, s, call) - target;
+        if (f > 0) hi = s; else lo = s;
+        const double d1 = (std::log(S0 / K) + 0.5 * s * s * tau) / (s * std::sqrt(tau));
+        const double vega = S0 * std::exp(-0.5 * d1 * d1) / std::sqrt(2.0 * M_PI) * std::sqrt(tau);
+        double sn = vega > 0 ? s - f / vega : 0.5 * (lo + hi);
+        if (!(sn > lo && sn < hi)) sn = 0.5 * (lo + hi);
+        const double ds = std::fabs(sn - s);
+        s = sn;
+        if (ds <= 1e-14 * s || hi - lo <= 1e-15 * s) break;
+    }
+    return s;
+}
+
+double orc_rt_payoff(double Ivdt, double IsvdW, double rho, double K) {
+    const double vol = std::sqrt((1.0 - rho * rho) * Ivdt);
+    const double spot = std::exp(-0.5 * rho * rho * Ivdt + rho * IsvdW);
+    return orc_bs_call(spot, K, 1.0, vol);
+}
+
+void orc_payoff_rt_samples(double xi, const double* H, const double* eta, const double* rho,
+                           int par_size, const double* T, const double* K, int op_size,
+                           int N, long M, const double* Z, int comp_mode, double* payoff) {
+    Params par{H, eta, rho, T, K, par_size, op_size};
+    const std::vector<Vec> g = gamma_tables(par, N);
+    const long out = par.size();
+#pragma omp parallel
+    {
+        PathState st(N);
+#pragma omp for schedule(static)
+        for (long m = 0; m < M; ++m)
+            rt_path(par, xi, N, Z + 2 * m * N, Z + (2 * m + 1) * N, comp_mode, nullptr, g, st,
+                    payoff + m * out, nullptr);
+    }
+}
+
+void orc_integrals_rt_samples(double xi, const double* H, const double* eta, const double* rho,
+                              int par_size, const double* T, const double* K, int op_size,
+                              int N, long M, const double* Z, int comp_mode, double* out) {
+    Params par{H, eta, rho, T, K, par_size, op_size};
+    const std::vector<Vec> g = gamma_tables(par, N);
+    const long os = par.size();
+#pragma omp parallel
+    {
+        PathState st(N);
+        Vec pay(os);
+#pragma omp for schedule(static)
+        for (long m = 0; m < M; ++m)
+            rt_path(par, xi, N, Z + 2 * m * N, Z + (2 * m + 1) * N, comp_mode, nullptr, g, st,
+                    pay.data(), out + 2 * m * os);
+    }
+}
+
+void orc_payoff_mc_samples(double xi, const double* H, const double* eta, const double* rho,
+                           int par_size, const double* T, const double* K, int op_size,
+                           int N, long M, const double* Z, int comp_mode, double* payoff) {
+    Params par{H, eta, rho, T, K, par_size, op_size};
+    const std::vector<Vec> g = gamma_tables(par, N);
+    const long out = par.size();
+#pragma omp parallel
+    {
+        PathState st(N);
+        double S = 0.0;
+#pragma omp for schedule(static)
+        for (long m = 0; m < M; ++m) {
+            const double* W1 = Z + 3 * m * N;
+            const double* W1p = W1 + N;
+            const double* Wp = W1 + 2 * N;
+            /* rBergLib/rBergomiMT.cpp:238-267 */
+            for (long I = 0; I < out; ++I) {
+                bool update = par.HTrig(I);
+                if (update) wtilde_impl(W1, W1p, g[I / op_size].data(), par.h(I), N, nullptr, st.Wt.data());
+                update = update || par.TTrig(I);
+                if (update) {
+                    st.dt = par.t(I) / N;
+                    st.sdt = std::sqrt(st.dt);
+                }
+                update = update || par.etaTrig(I);
+                if (update) v_impl(st.Wt.data(), xi, par.h(I), par.e(I), par.t(I), N, comp_mode, st.v.data());
+                update = update || par.rhoTrig(I);
+                if (update) S = s_impl(st.v.data(), W1, Wp, par.r(I), st.dt, N);
+                payoff[m * out + I] = posPart(S - par.k(I));
+            }
+        }
+    }
+}
+
+void orc_price_stat(const double* payoff, long M, int out_size, double* price, double* stat) {
+    std::vector<double> s(out_size, 0.0), s2(out_size, 0.0);
+    for (long m = 0; m < M; ++m)
+        for (int I = 0; I < out_size; ++I) {
+            const double x = payoff[m * out_size + I];
+            s[I] += x;
+            s2[I] += x * x;
+        }
+    for (int I = 0; I < out_size; ++I) {
+        price[I] = s[I] / double(M);
+        const double var = s2[I] / double(M) - price[I] * price[I];
+        stat[I] = std::sqrt(var) / std::sqrt(double(M));
+    }
+}
+
+/* Shared driver of orc_compute_rt / orc_compute_mc (rBergLib/rBergomiMT.cpp:174-289 and
+ * :301-414; single thread rBergLib/RBergomi.cpp:218-370). */
+static void compute_seeded(bool rt, double xi, const double* H, const double* eta,
+                           const double* rho, int par_size, const double* T, const double* K,
+                           int op_size, int N, long M, const uint64_t* seed, int seed_size,
+                           int num_threads, int conv_mode, double* price, double* iv,
+                           double* stat) {
+    Params par{H, eta, rho, T, K, par_size, op_size};
+    const std::vector<Vec> g = gamma_tables(par, N);
+    const long out = par.size();
+    if (num_threads < 1) num_threads = 1;
+    /* seeding: single thread = RBergomiST::setGen (seed_seq over the whole seed array,
+     * generate seed_size words, take the first); several threads = RNG::RNG (generate one
+     * word per thread). */
+    std::vector<uint64_t> seeds(std::max(num_threads, 1));
+    {
+        std::seed_seq seeder(seed, seed + seed_size);
+        if (num_threads == 1) {
+            std::vector<uint64_t> tmp(seed, seed + seed_size);
+            seeder.generate(tmp.begin(), tmp.end());
+            seeds[0] = tmp.empty() ? 0 : tmp[0];
+        } else {
+            seeder.generate(seeds.begin(), seeds.end());
+        }
+    }
+    std::vector<double> sum(out, 0.0), sum2(out, 0.0);
+    /* thread-private partial sums, combined in thread order after the loop instead of in
+     * the arrival order of the reference's critical section (:395-401), so that a fixed
+     * thread count gives bit-identical results */
+    std::vector<Vec> part(num_threads), part2(num_threads);
+#ifdef _OPENMP
+    omp_set_dynamic(0);
+#endif
+#pragma omp parallel num_threads(num_threads)
+    {
+        int tid = 0;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+#endif
+        std::mt19937_64 gen(seeds[tid]);
+        std::normal_distribution<double> dist(0.0, 1.0);
+        Vec W1(N), W1perp(N), Wperp(N), pay(out);
+        Vec ps(out, 0.0), ps2(out, 0.0);
+        PathState st(N);
+        FFTWork fft(N);
+        FFTWork* f = conv_mode == ORC_CONV_FFT ? &fft : nullptr;
+        double S = 0.0;
+#pragma omp for schedule(static)
+        for (long m = 0; m < M; ++m) {
+            for (auto& x : W1) x = dist(gen);
+            for (auto& x : W1perp) x = dist(gen);
+            if (rt) {
+                rt_path(par, xi, N, W1.data(), W1perp.data(), ORC_COMP_REF, f, g, st, pay.data(), nullptr);
+            } else {
+                for (auto& x : Wperp) x = dist(gen);
+                for (long I = 0; I < out; ++I) {
+                    bool update = par.HTrig(I);
+                    if (update) wtilde_impl(W1.data(), W1perp.data(), g[I / op_size].data(), par.h(I), N, f, st.Wt.data());
+                    update = update || par.TTrig(I);
+                    if (update) {
+                        st.dt = par.t(I) / N;
+                        st.sdt = std::sqrt(st.dt);
+                    }
+                    update = update || par.etaTrig(I);
+                    if (update) v_impl(st.Wt.data(), xi, par.h(I), par.e(I), par.t(I), N, ORC_COMP_REF, st.v.data());
+                    update = update || par.rhoTrig(I);
+                    if (update) S = s_impl(st.v.data(), W1.data(), Wperp.data(), par.r(I), st.dt, N);
+                    pay[I] = posPart(S - par.k(I));
+                }
+            }
+            for (long I = 0; I < out; ++I) {
+                ps[I] += pay[I];
+                ps2[I] += pay[I] * pay[I];
+            }
+        }
+        part[tid] = ps;
+        part2[tid] = ps2;
+    }
+    for (int t = 0; t < num_threads; ++t)
+        for (long I = 0; I < out && !part[t].empty(); ++I) {
+            sum[I] += part[t][I];
+            sum2[I] += part2[t][I];
+        }
+    for (long I = 0; I < out; ++I) {
+        price[I] = sum[I] / double(M);
+        const double var = sum2[I] / double(M) - price[I] * price[I];
+        stat[I] = std::sqrt(var) / std::sqrt(double(M));
+        if (iv) iv[I] = orc_iv_call(price[I], 1.0, par.k(I), par.t(I));
+    }
+}
+
+void orc_compute_rt(double xi, const double* H, const double* eta, const double* rho,
+                    int par_size, const double* T, const double* K, int op_size, int N, long M,
+                    const uint64_t* seed, int seed_size, int num_threads, int conv_mode,
+                    double* price, double* iv, double* stat) {
+    compute_seeded(true, xi, H, eta, rho, par_size, T, K, op_size, N, M, seed, seed_size,
+                   num_threads, conv_mode, price, iv, stat);
+}
+
+void orc_compute_mc(double xi, const double* H, const double* eta, const double* rho,
+                    int par_size, const double* T, const double* K, int op_size, int N, long M,
+                    const uint64_t* seed, int seed_size, int num_threads, int conv_mode,
+                    double* price, double* iv, double* stat) {
+    compute_seeded(false, xi, H, eta, rho, par_size, T, K, op_size, N, M, seed, seed_size,
+                   num_threads, conv_mode, price, iv, stat);
+}
+
+/* Philox4x32 round function with the published multipliers and Weyl key increments
+ * (Salmon et al., "Parallel random numbers: as easy as 1, 2, 3", SC'11, section 4.3). */
+void orc_philox4x32_10(const uint32_t* ctr, const uint32_t* key, uint32_t* out) {
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = uint64_t(0xD2511F53u) * c0;
+        const uint64_t p1 = uint64_t(0xCD9E8D57u) * c2;
+        const uint32_t n0 = uint32_t(p1 >> 32) ^ c1 ^ k0;
+        const uint32_t n1 = uint32_t(p1);
+        const uint32_t n2 = uint32_t(p0 >> 32) ^ c3 ^ k1;
+        const uint32_t n3 = uint32_t(p0);
+        c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+void orc_philox_key(const uint64_t* seed, int seed_size, uint32_t* key) {
+    uint64_t h = splitmix64(0x243F6A8885A308D3ULL ^ uint64_t(seed_size));
+    for (int i = 0; i < seed_size; ++i) h = splitmix64(h ^ seed[i]);
+    key[0] = uint32_t(h);
+    key[1] = uint32_t(h >> 32);
+}
+
+void orc_normals_philox(const uint64_t* seed, int seed_size, long p0, long np, int nstreams,
+                        int N, double* out) {
+    uint32_t key[2];
+    orc_philox_key(seed, seed_size, key);
+    const double two32i = 1.0 / 4294967296.0;
+#pragma omp parallel for schedule(static)
+    for (long pp = 0; pp < np; ++pp) {
+        const uint64_t p = uint64_t(p0 + pp);
+        for (int s = 0; s < nstreams; ++s) {
+            double* z = out + (pp * nstreams + s) * long(N);
+            for (int q = 0; q < (N + 3) / 4; ++q) {
+                const uint32_t ctr[4] = {uint32_t(q), uint32_t(p), uint32_t(p >> 32), uint32_t(s)};
+                uint32_t x[4];
+                orc_philox4x32_10(ctr, key, x);
+                double zz[4];
+                for (int j = 0; j < 2; ++j) {
+                    const double u0 = (double(x[2 * j]) + 0.5) * two32i;
+                    const double u1 = (double(x[2 * j + 1]) + 0.5) * two32i;
+                    const double r = std::sqrt(-2.0 * std::log(u0));
+                    zz[2 * j] = r * std::cos(2.0 * M_PI * u1);
+                    zz[2 * j + 1] = r * std::sin(2.0 * M_PI * u1);
+                }
+                for (int j = 0; j < 4 && 4 * q + j < N; ++j) z[4 * q + j] = zz[j];
+            }
+        }
+    }
+}
+
+}  // extern "C"

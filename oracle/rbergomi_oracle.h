@@ -1,0 +1,143 @@
+/*
+ * rbergomi_oracle.h -- CPU restatement of the rough-Bergomi Romano-Touzi pricing path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP product path
+ * (rbergomi_amd/csrc) and the timed CPU baseline of bench.py.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.  The product
+ * library never links, loads or calls it.
+ *
+ * Every function restates the reference algorithm in plain FP64 C++ and cites the
+ * reference file:line it follows (paths relative to the reference repository root).
+ *
+ * Parity pins (what checks each function against something other than itself):
+ *   orc_gamma            closed form, telescoping sum, H = 1/2 => Gamma_k = 1        (tests/test_oracle_math.py)
+ *   orc_convolve_*       brute force in exact rationals, the reference's own known answer
+ *                        x={1,2.4,3.9,4.8}, y={9,5.4,2.9,7.8} -> {9,27,50.96,79.02}
+ *                        (rBergomi/text/tests.cpp:333-361), FFT == direct              (test_oracle_math.py)
+ *   orc_wtilde           impulse responses (unit W1 / W1perp vectors) against the hybrid-scheme
+ *                        weights written independently in the test; variance closed form (test_oracle_math.py)
+ *   orc_payoff_rt_samples  the reference's own 200 single-thread per-path payoffs
+ *                        rBergomi/testing/debug8.txt~ (tests/golden/debug8_payoffs.json);
+ *                        eta = rho = 0 closed form; eta = 0 closed form; K -> 0 gives S' (test_oracle_golden.py)
+ *   orc_compute_rt       reference known prices testRBergLib/tests.cpp:16-28 within 0.005 + stat;
+ *                        E[S'] = 1 martingale identity within 4.5 SE                 (test_oracle_golden.py)
+ *   orc_compute_mc       same known prices (the reference tests both methods against them)
+ *   orc_iv_call          round trip IV(BS(sigma)) = sigma; put-call parity branch     (test_oracle_math.py)
+ *   orc_philox4x32_10    Random123 known-answer vectors                                (test_oracle_math.py)
+ *   orc_normals_philox   distribution moments; equals Box-Muller on orc_philox4x32_10 outputs
+ *
+ * Parity unpinned: the exact roundoff of FFTW (the reference convolution) and the iterate
+ * sequence of Boost's bracket_and_solve_root in IV_call (rBergLib/BlackScholes.cpp:36-63).
+ * Neither library exists in this environment; the direct-sum definition and the IV
+ * round trip pin the mathematics instead.
+ */
+#pragma once
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* compensator of the variance process: 0 = reference form 0.5*eta^2*((i-1)*dt)^(2H)
+ * (rBergLib/rBergomiMT.cpp:77-79), 1 = exact form 0.5*eta^2*(i*dt)^(2H)
+ * (clean_Rfiles/hybridScheme.R:110-113). */
+enum { ORC_COMP_REF = 0, ORC_COMP_EXACT = 1 };
+/* convolution used inside orc_wtilde: 0 = direct sum, 1 = own radix-2 FFT */
+enum { ORC_CONV_DIRECT = 0, ORC_CONV_FFT = 1 };
+
+/* Gamma_0 = 0, Gamma_k = ((k+1)^(a+1) - k^(a+1))/(a+1), a = H - 1/2.
+ * rBergLib/rBergomiMT.cpp:143-149 */
+void orc_gamma(double H, int N, double* gamma);
+
+/* truncated linear convolution z_i = sum_{k=0}^{i} x_k y_{i-k}, i < n.  This is what the
+ * length-(2N-1) FFT product in updateWtilde returns in its first N entries
+ * (rBergLib/rBergomiMT.cpp:91-105). */
+void orc_convolve_direct(const double* x, const double* y, int n, double* z);
+void orc_convolve_fft(const double* x, const double* y, int n, double* z);
+
+/* Volterra process on the unit interval, value at t=(i+1)/N.
+ * rBergLib/rBergomiMT.cpp:83-109 (single thread: rBergLib/RBergomi.cpp:397-423) */
+void orc_wtilde(const double* W1, const double* W1perp, double H, int N, int conv_mode,
+                double* Wtilde);
+
+/* v_0 = xi, v_i = xi*exp(eta*T^H*Wtilde_{i-1} - compensator_i), dt = T/N.
+ * scaleWtilde rBergLib/rBergomiMT.cpp:163-167, updateV :73-80 */
+void orc_update_v(const double* Wtilde, double xi, double H, double eta, double T, int N,
+                  int comp_mode, double* v);
+
+/* intVdt / intRootVdW, rBergLib/rBergomiMT.cpp:427-436 */
+double orc_int_vdt(const double* v, int N, double dt);
+double orc_int_rootvdw(const double* v, const double* W1, int N, double sdt);
+
+/* Phi and Black-Scholes, rBergLib/BlackScholes.cpp:16-33 */
+double orc_phi(double x);
+double orc_bs_call(double S0, double K, double tau, double sigma);
+double orc_bs_put(double S0, double K, double tau, double sigma);
+
+/* implied vol of a call (S0 > K handled through the put, as rBergLib/BlackScholes.cpp:36-42).
+ * Safeguarded Newton to |dsigma| <= 1e-14*sigma instead of Boost's 2^-10 bracket.
+ * Returns NaN when the price is outside the no-arbitrage range. */
+double orc_iv_call(double price, double S0, double K, double tau);
+
+/* Romano-Touzi conditional payoff of one path for one option, given the path integrals.
+ * rBergLib/rBergomiMT.cpp:383-388 */
+double orc_rt_payoff(double Ivdt, double IsvdW, double rho, double K);
+
+/* Per-path Romano-Touzi payoffs from user Gaussians.
+ * Z layout (rBergLib/cinterface.cpp:113-119): path m uses W1 = Z[2mN, 2mN+N),
+ * W1perp = Z[2mN+N, 2mN+2N).  payoff[m*out_size + I], I = h*op_size + t
+ * (rBergLib/ParamTot.h:32-33).  Loop and trigger structure of
+ * ComputePayoffRTsamples_ST, rBergLib/rBergomiMT.cpp:528-610. */
+void orc_payoff_rt_samples(double xi, const double* H, const double* eta, const double* rho,
+                           int par_size, const double* T, const double* K, int op_size,
+                           int N, long M, const double* Z, int comp_mode, double* payoff);
+
+/* Per-path integrals (Ivdt, IsvdW) for option I from the same Z layout: out[2*(m*out_size+I)+{0,1}] */
+void orc_integrals_rt_samples(double xi, const double* H, const double* eta, const double* rho,
+                              int par_size, const double* T, const double* K, int op_size,
+                              int N, long M, const double* Z, int comp_mode, double* out);
+
+/* Per-path plain Monte Carlo payoffs (S_T - K)^+ with three Gaussian blocks per path:
+ * W1 = Z[3mN, 3mN+N), W1perp = Z[3mN+N, 3mN+2N), Wperp = Z[3mN+2N, 3mN+3N).
+ * ComputePriceMT loop body, rBergLib/rBergomiMT.cpp:231-267 (updateZ :134-136,
+ * scaleZ :129-132, updateS :111-116). */
+void orc_payoff_mc_samples(double xi, const double* H, const double* eta, const double* rho,
+                           int par_size, const double* T, const double* K, int op_size,
+                           int N, long M, const double* Z, int comp_mode, double* payoff);
+
+/* price/stat from a payoff matrix [M][out_size]: mean and sqrt(E[X^2]-mean^2)/sqrt(M),
+ * rBergLib/rBergomiMT.cpp:404-409 */
+void orc_price_stat(const double* payoff, long M, int out_size, double* price, double* stat);
+
+/* Reference-structured seeded pricers (the timed CPU baseline).  Loop structure of
+ * ComputePriceRTMT / ComputeIVRTMT (rBergLib/rBergomiMT.cpp:301-425): mt19937_64 +
+ * std::normal_distribution per thread seeded through std::seed_seq (RNG::RNG :63-71;
+ * single thread: RBergomiST::setGen rBergLib/RBergomi.cpp:381-389), per-path FFT
+ * convolution, the options loop with the memoisation triggers, thread-private sums and a
+ * critical-section reduction.  The W1perp data race of the reference (:348-349) is fixed:
+ * W1perp is private per thread.  conv_mode selects FFT (reference structure) or direct.
+ * iv may be NULL. */
+void orc_compute_rt(double xi, const double* H, const double* eta, const double* rho,
+                    int par_size, const double* T, const double* K, int op_size, int N, long M,
+                    const uint64_t* seed, int seed_size, int num_threads, int conv_mode,
+                    double* price, double* iv, double* stat);
+/* plain Monte Carlo sibling, ComputePriceMT / ComputeIVMT rBergLib/rBergomiMT.cpp:174-299 */
+void orc_compute_mc(double xi, const double* H, const double* eta, const double* rho,
+                    int par_size, const double* T, const double* K, int op_size, int N, long M,
+                    const uint64_t* seed, int seed_size, int num_threads, int conv_mode,
+                    double* price, double* iv, double* stat);
+
+/* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11), the counter-based generator of the
+ * product path.  ctr[4], key[2] -> out[4]. */
+void orc_philox4x32_10(const uint32_t* ctr, const uint32_t* key, uint32_t* out);
+/* 64-bit Philox key from the seed array (splitmix64 fold), as the product path derives it */
+void orc_philox_key(const uint64_t* seed, int seed_size, uint32_t* key);
+/* Standard normals of the product path: normal number j of stream s of path p is
+ * Box-Muller on Philox(ctr = {j/4, p_lo, p_hi, s}); z[p][s][j] written to
+ * out[(p*nstreams + s)*N + j] for p in [p0, p0+np). */
+void orc_normals_philox(const uint64_t* seed, int seed_size, long p0, long np, int nstreams,
+                        int N, double* out);
+
+#ifdef __cplusplus
+}
+#endif
