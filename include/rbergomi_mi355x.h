@@ -1,0 +1,151 @@
+/*
+ * rbergomi_mi355x.h -- C ABI of librbergomi_mi355x.so, the MI355X (gfx950) rough-Bergomi pricer.
+ *
+ * Part 1 is the drop-in boundary: the three extern "C" entry points of the reference's
+ * rBergLib C interface with identical signatures, argument meaning and error behaviour.
+ * Part 2 adds status-returning entry points (v2), device-resident plans and probes used by
+ * the parity tests and the benchmark.
+ *
+ * Conventions shared with the reference (rBergLib/cinterface.h:17-60):
+ *   (H, eta, rho) are zipped arrays of length par_size, (T, K) zipped arrays of length
+ *   op_size; every model triple is priced for every option pair; output index
+ *   I = h*op_size + t, out_size = par_size*op_size; S0 = 1, r = 0, constant forward variance xi.
+ */
+#ifndef RBERGOMI_MI355X_H
+#define RBERGOMI_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ======================================================================================
+ * Part 1 -- reference C ABI (drop-in)
+ * ====================================================================================== */
+
+/* Plain Monte Carlo prices and implied vols (log-Euler spot, payoff (S_T-K)^+).
+ * Replaces computeIV_tensor, rBergLib/cinterface.h:63-66 (implementation
+ * rBergLib/cinterface.cpp:10-46).  Like the reference it writes the rho values into the
+ * eta_tot buffer and the eta values into rho_tot (the swapped Result2Array call at
+ * rBergLib/cinterface.cpp:45); rb_price_v2 does not swap.
+ * out_size != par_size*op_size: message on stderr and exit(17); num_threads < 1: exit(21).
+ * num_threads >= 1 is accepted as a hint only: the results do not depend on it.
+ * A device failure prints a message and exits with status 40 (the reference ABI has no
+ * error channel). */
+void computeIV_tensor(double xi, double* H, double* eta, double* rho, int par_size,
+        double* T, double* K, int op_size, int N, long M, uint64_t* seed, int seed_size,
+        int num_threads, double* price, double* iv, double* stat, double* H_tot, double* eta_tot,
+        double* rho_tot, double* T_tot, double* K_tot, int out_size);
+
+/* Romano-Touzi conditioned prices and implied vols: the hot path.
+ * Replaces computeIVRT_tensor, rBergLib/cinterface.h:68-71 (rBergLib/cinterface.cpp:49-85).
+ * Same error behaviour as computeIV_tensor; the echo arrays are not swapped. */
+void computeIVRT_tensor(double xi, double* H, double* eta, double* rho, int par_size,
+        double* T, double* K, int op_size, int N, long M, uint64_t* seed, int seed_size,
+        int num_threads, double* price, double* iv, double* stat, double* H_tot, double* eta_tot,
+        double* rho_tot, double* T_tot, double* K_tot, int out_size);
+
+/* Romano-Touzi per-path payoffs from user Gaussians.
+ * Replaces computeIVRT_tensor_quadrature, rBergLib/cinterface.h:73-75
+ * (rBergLib/cinterface.cpp:102-131).  Z holds 2*N*M doubles: path m uses
+ * W1 = Z[2mN, 2mN+N) and W1perp = Z[2mN+N, 2mN+2N).  payoff holds M*out_size doubles,
+ * payoff[m*out_size + I].  No size check (as in the reference). */
+void computeIVRT_tensor_quadrature(double xi, double* H, double* eta, double* rho, int par_size,
+        double* T, double* K, int op_size, int N, long M, double* Z,
+        int num_threads, double* payoff);
+
+/* ======================================================================================
+ * Part 2 -- status-returning API (no counterpart in the reference)
+ * ====================================================================================== */
+
+enum rb_status {
+    RB_OK = 0,
+    RB_ERR_ARG = 1,     /* invalid argument values                    */
+    RB_ERR_SIZE = 2,    /* out_size != par_size*op_size               */
+    RB_ERR_DEVICE = 3,  /* HIP runtime or kernel failure              */
+    RB_ERR_NOMEM = 4,   /* device allocation failed                   */
+    RB_ERR_LIMIT = 5    /* size beyond what this build supports       */
+};
+enum rb_precision {
+    RB_PREC_FP64 = 0,        /* every stage in FP64 (the parity path)               */
+    RB_PREC_FP32 = 1,        /* Toeplitz GEMM on FP32 MFMA                          */
+    RB_PREC_SPLIT_FP16 = 2,  /* Toeplitz GEMM as FP16 hi+lo products, FP32 accumulate */
+    RB_PREC_SPLIT_BF16 = 3   /* Toeplitz GEMM as BF16 hi+lo products, FP32 accumulate */
+};
+enum rb_compensator {
+    RB_COMP_REF = 0,   /* 0.5 eta^2 ((i-1) dt)^(2H), rBergLib/rBergomiMT.cpp:77-79  */
+    RB_COMP_EXACT = 1  /* 0.5 eta^2 (i dt)^(2H), clean_Rfiles/hybridScheme.R:110-113 */
+};
+enum rb_method { RB_METHOD_RT = 0, RB_METHOD_MC = 1 };
+
+typedef struct rb_config {
+    int precision;    /* rb_precision   */
+    int compensator;  /* rb_compensator */
+    int method;       /* rb_method      */
+    int reserved;
+} rb_config;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* rb_last_error(void);
+
+/* Prices, implied vols and standard errors; returns an rb_status.  cfg may be NULL
+ * (FP64, reference compensator, Romano-Touzi).  iv may be NULL. */
+int rb_price_v2(double xi, const double* H, const double* eta, const double* rho, int par_size,
+                const double* T, const double* K, int op_size, int N, long M,
+                const uint64_t* seed, int seed_size, const rb_config* cfg,
+                double* price, double* iv, double* stat, int out_size);
+
+/* Per-path payoffs from user Gaussians (quadrature layout of computeIVRT_tensor_quadrature:
+ * 2 blocks of N per path for RB_METHOD_RT, 3 blocks W1, W1perp, Wperp for RB_METHOD_MC).
+ * price/stat (may be NULL) receive the sample mean and standard error. */
+int rb_payoff_samples_v2(double xi, const double* H, const double* eta, const double* rho,
+                         int par_size, const double* T, const double* K, int op_size, int N,
+                         long M, const double* Z, const rb_config* cfg, double* payoff,
+                         double* price, double* stat);
+
+/* Device-resident plan: validated, tables uploaded and work buffers allocated once;
+ * rb_plan_run launches the whole pricing pipeline on the plan's stream (stream = NULL
+ * creates one; otherwise a hipStream_t the caller owns) and waits for it. */
+typedef struct rb_plan rb_plan;
+int rb_plan_create(double xi, const double* H, const double* eta, const double* rho,
+                   int par_size, const double* T, const double* K, int op_size, int N, long M,
+                   const uint64_t* seed, int seed_size, const rb_config* cfg, void* stream,
+                   rb_plan** out);
+int rb_plan_run(rb_plan* plan);
+int rb_plan_results(rb_plan* plan, double* price, double* iv, double* stat);
+/* Kernel timing of the following runs with HIP events on the plan's stream (enable = 1). */
+int rb_plan_set_profiling(rb_plan* plan, int enable);
+/* Per-kernel totals of the last run: names (NUL-terminated, 32 bytes each), total ms and
+ * launch counts for up to max entries; *count receives the number written. */
+int rb_plan_kernel_times(rb_plan* plan, char* names, double* total_ms, int* launches, int max,
+                         int* count);
+/* Work sizes of the plan: units (distinct H), groups, options, paths per chunk. */
+int rb_plan_info(rb_plan* plan, int* nunits, int* ngroups, int* noptions, long* chunk);
+void rb_plan_destroy(rb_plan* plan);
+
+/* Host-only planning (no device call): group index of every option I and the unit of every
+ * group, as the executor deduplicates them.  group_of_option has out_size entries,
+ * unit_of_group at least out_size entries. */
+int rb_layout(const double* H, const double* eta, const double* rho, int par_size,
+              const double* T, const double* K, int op_size, int N, int method,
+              int* nunits, int* ngroups, int* group_of_option, int* unit_of_group);
+
+/* ---- probes used by the parity tests (device work, FP64) ---- */
+/* Philox4x32-10 on the device: ctr[4n], key[2] -> out[4n] */
+int rb_probe_philox(const uint32_t* ctr, const uint32_t* key, long n, uint32_t* out);
+/* Device Gaussians of paths [p0, p0+np): out[((p-p0)*nstreams + s)*N + j] */
+int rb_probe_normals(const uint64_t* seed, int seed_size, long p0, long np, int nstreams, int N,
+                     double* out);
+/* Volterra process of every path of Z (2 blocks of N per path): Wtilde[m*N + i] */
+int rb_probe_wtilde(const double* Z, long M, int N, double H, double* Wtilde);
+/* Per-path integrals (Ivdt, IsvdW) of option I from Z: out[2*(m*out_size + I) + {0,1}] */
+int rb_probe_integrals(double xi, const double* H, const double* eta, const double* rho,
+                       int par_size, const double* T, const double* K, int op_size, int N,
+                       long M, const double* Z, const rb_config* cfg, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RBERGOMI_MI355X_H */

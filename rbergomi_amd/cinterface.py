@@ -1,0 +1,188 @@
+"""Python mirror of the reference's pricing interface, on top of the HIP library.
+
+Names and argument meaning follow the reference's R binding of the C ABI
+(rBergomiR/RBergomi/src/rbergomi.cpp:16-77, exported as computeIV / computeIV_RT) and the
+C interface itself (rBergLib/cinterface.h:62-76).  Every call goes through the C ABI of
+librbergomi_mi355x.so; there is no CPU fallback.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import RbConfig, check, dp, u64p
+
+
+def _arr(x):
+    a = np.ascontiguousarray(np.atleast_1d(np.asarray(x, dtype=np.float64)))
+    return a, a.ctypes.data_as(dp)
+
+
+def _seed(seed):
+    s = np.ascontiguousarray(np.atleast_1d(np.asarray(seed, dtype=np.uint64)))
+    return s, s.ctypes.data_as(u64p)
+
+
+def _legacy(fn, xi, H, eta, rho, T, K, N, M, seed, num_threads):
+    H, pH = _arr(H)
+    eta, pe = _arr(eta)
+    rho, pr = _arr(rho)
+    T, pT = _arr(T)
+    K, pK = _arr(K)
+    s, ps = _seed(seed)
+    par_size, op_size = len(H), len(T)
+    out_size = par_size * op_size
+    outs = {k: np.zeros(out_size) for k in ("price", "iv", "stat", "H", "eta", "rho", "T", "K")}
+    p = {k: v.ctypes.data_as(dp) for k, v in outs.items()}
+    fn(float(xi), pH, pe, pr, par_size, pT, pK, op_size, int(N), int(M), ps, len(s),
+       int(num_threads), p["price"], p["iv"], p["stat"], p["H"], p["eta"], p["rho"], p["T"], p["K"],
+       out_size)
+    # DataFrame columns of rbergomi.cpp:40-44
+    res = {"price": outs["price"], "iv": outs["iv"], "H": outs["H"], "eta": outs["eta"],
+           "rho": outs["rho"], "T": outs["T"], "K": outs["K"], "xi": float(xi),
+           "stat": outs["stat"], "N": int(N), "M": int(M), "num.threads": int(num_threads)}
+    return res
+
+
+def computeIV(xi, H, eta, rho, T, K, N, M, seed, num_threads=1):
+    """Plain Monte Carlo through computeIV_tensor (rbergomi.cpp:17-45).  As in the reference,
+    the returned 'eta' column holds rho and the 'rho' column holds eta."""
+    return _legacy(_lib.load().computeIV_tensor, xi, H, eta, rho, T, K, N, M, seed, num_threads)
+
+
+def computeIV_RT(xi, H, eta, rho, T, K, N, M, seed, num_threads=1):
+    """Romano-Touzi prices and implied vols through computeIVRT_tensor (rbergomi.cpp:48-77)."""
+    return _legacy(_lib.load().computeIVRT_tensor, xi, H, eta, rho, T, K, N, M, seed, num_threads)
+
+
+def computeIVRT_tensor_quadrature(xi, H, eta, rho, T, K, N, Z, num_threads=1):
+    """Per-path RT payoffs from user Gaussians Z (2*N*M values, path-major: W1 block then
+    W1perp block) -> payoff [M, out_size]  (rBergLib/cinterface.cpp:102-131)."""
+    H, pH = _arr(H)
+    eta, pe = _arr(eta)
+    rho, pr = _arr(rho)
+    T, pT = _arr(T)
+    K, pK = _arr(K)
+    Z, pZ = _arr(np.asarray(Z, dtype=np.float64).ravel())
+    M = Z.size // (2 * N)
+    out = np.zeros((M, len(H) * len(T)))
+    _lib.load().computeIVRT_tensor_quadrature(float(xi), pH, pe, pr, len(H), pT, pK, len(T), int(N),
+                                              int(M), pZ, int(num_threads), out.ctypes.data_as(dp))
+    return out
+
+
+def _cfg(precision=_lib.PREC_FP64, compensator=_lib.COMP_REF, method=_lib.METHOD_RT):
+    return RbConfig(precision, compensator, method, 0)
+
+
+def price_v2(xi, H, eta, rho, T, K, N, M, seed, precision=_lib.PREC_FP64,
+             compensator=_lib.COMP_REF, method=_lib.METHOD_RT):
+    """Status-checked pricing (rb_price_v2) -> (price, iv, stat); raises RbError on failure."""
+    H, pH = _arr(H)
+    eta, pe = _arr(eta)
+    rho, pr = _arr(rho)
+    T, pT = _arr(T)
+    K, pK = _arr(K)
+    s, ps = _seed(seed)
+    out = len(H) * len(T)
+    price, iv, stat = np.zeros(out), np.zeros(out), np.zeros(out)
+    cfg = _cfg(precision, compensator, method)
+    check(_lib.load().rb_price_v2(float(xi), pH, pe, pr, len(H), pT, pK, len(T), int(N), int(M), ps,
+                                  len(s), ctypes.byref(cfg), price.ctypes.data_as(dp),
+                                  iv.ctypes.data_as(dp), stat.ctypes.data_as(dp), out))
+    return price, iv, stat
+
+
+def payoff_samples_v2(xi, H, eta, rho, T, K, N, Z, method=_lib.METHOD_RT,
+                      compensator=_lib.COMP_REF):
+    """Per-path payoffs from user Gaussians -> (payoff [M, out], price, stat)"""
+    H, pH = _arr(H)
+    eta, pe = _arr(eta)
+    rho, pr = _arr(rho)
+    T, pT = _arr(T)
+    K, pK = _arr(K)
+    Z, pZ = _arr(np.asarray(Z, dtype=np.float64).ravel())
+    nst = 2 if method == _lib.METHOD_RT else 3
+    M = Z.size // (nst * N)
+    out = len(H) * len(T)
+    payoff = np.zeros((M, out))
+    price, stat = np.zeros(out), np.zeros(out)
+    cfg = _cfg(_lib.PREC_FP64, compensator, method)
+    check(_lib.load().rb_payoff_samples_v2(float(xi), pH, pe, pr, len(H), pT, pK, len(T), int(N),
+                                           int(M), pZ, ctypes.byref(cfg), payoff.ctypes.data_as(dp),
+                                           price.ctypes.data_as(dp), stat.ctypes.data_as(dp)))
+    return payoff, price, stat
+
+
+class Plan:
+    """Device-resident pricing plan (rb_plan_*): build once, run many times."""
+
+    def __init__(self, xi, H, eta, rho, T, K, N, M, seed, precision=_lib.PREC_FP64,
+                 compensator=_lib.COMP_REF, method=_lib.METHOD_RT, profile=False):
+        self._keep = [_arr(H), _arr(eta), _arr(rho), _arr(T), _arr(K), _seed(seed)]
+        (H, pH), (eta, pe), (rho, pr), (T, pT), (K, pK), (s, ps) = self._keep
+        self.out_size = len(H) * len(T)
+        self.cfg = _cfg(precision, compensator, method)
+        self.ptr = ctypes.c_void_p()
+        L = _lib.load()
+        check(L.rb_plan_create(float(xi), pH, pe, pr, len(H), pT, pK, len(T), int(N), int(M), ps,
+                               len(s), ctypes.byref(self.cfg), None, ctypes.byref(self.ptr)))
+        if profile:
+            check(L.rb_plan_set_profiling(self.ptr, 1))
+
+    def run(self):
+        check(_lib.load().rb_plan_run(self.ptr))
+
+    def results(self):
+        n = self.out_size
+        price, iv, stat = np.zeros(n), np.zeros(n), np.zeros(n)
+        check(_lib.load().rb_plan_results(self.ptr, price.ctypes.data_as(dp), iv.ctypes.data_as(dp),
+                                          stat.ctypes.data_as(dp)))
+        return price, iv, stat
+
+    def kernel_times(self):
+        """{kernel name: (total ms, launches)} of the last run (profiling enabled)"""
+        names = ctypes.create_string_buffer(32 * 16)
+        ms = (ctypes.c_double * 16)()
+        ln = (ctypes.c_int * 16)()
+        cnt = ctypes.c_int()
+        check(_lib.load().rb_plan_kernel_times(self.ptr, names, ms, ln, 16, ctypes.byref(cnt)))
+        raw = names.raw
+        return {raw[32 * i:32 * i + 32].split(b"\0")[0].decode(): (ms[i], ln[i])
+                for i in range(cnt.value)}
+
+    def info(self):
+        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        ch = ctypes.c_long()
+        check(_lib.load().rb_plan_info(self.ptr, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c),
+                                       ctypes.byref(ch)))
+        return {"units": a.value, "groups": b.value, "options": c.value, "chunk": ch.value}
+
+    def close(self):
+        if self.ptr:
+            _lib.load().rb_plan_destroy(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def layout(H, eta, rho, T, K, N, method=_lib.METHOD_RT):
+    """Host-only planning: (nunits, ngroups, group_of_option, unit_of_group)"""
+    H, pH = _arr(H)
+    eta, pe = _arr(eta)
+    rho, pr = _arr(rho)
+    T, pT = _arr(T)
+    K, pK = _arr(K)
+    out = len(H) * len(T)
+    nu, ng = ctypes.c_int(), ctypes.c_int()
+    gofo = np.zeros(out, dtype=np.int32)
+    uofg = np.zeros(out, dtype=np.int32)
+    ip = ctypes.POINTER(ctypes.c_int)
+    check(_lib.load().rb_layout(pH, pe, pr, len(H), pT, pK, len(T), int(N), int(method),
+                                ctypes.byref(nu), ctypes.byref(ng), gofo.ctypes.data_as(ip),
+                                uofg.ctypes.data_as(ip)))
+    return nu.value, ng.value, gofo, uofg[:ng.value]

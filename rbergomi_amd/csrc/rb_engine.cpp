@@ -1,0 +1,369 @@
+// rb_engine.cpp -- host planner and executor.
+//
+// Replaces the driver layer of the reference: ComputePriceRTMT / ComputePriceMT
+// (rBergLib/rBergomiMT.cpp:174-414), the single-thread class RBergomiST
+// (rBergLib/RBergomi.cpp:157-370) and the ParamTot index algebra (rBergLib/ParamTot.h:20-50).
+// Instead of walking the flat option index per path and recomputing on "triggers", the
+// planner deduplicates the work once (units, groups, options; rb_types.h) and launches each
+// stage over all paths of a chunk.
+#include "rb_engine.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <tuple>
+
+#include "rb_kernels.h"
+
+namespace rb {
+
+static thread_local std::string g_last_error;
+void set_last_error(const std::string& s) { g_last_error = s; }
+const char* last_error() { return g_last_error.c_str(); }
+
+const char* kernel_name(int id) {
+    static const char* names[K_COUNT] = {"k_normals_f64", "k_volterra_f64", "k_scan_f64",
+                                         "k_payoff_f64",  "k_reduce_f64",   "k_iv_f64"};
+    return id >= 0 && id < K_COUNT ? names[id] : "?";
+}
+
+int validate(const Request& r, std::string& err) {
+    if (r.H.empty() || r.T.empty()) {
+        err = "empty parameter or option arrays";
+        return ST_ERR_ARG;
+    }
+    if (r.eta.size() != r.H.size() || r.rho.size() != r.H.size() || r.K.size() != r.T.size()) {
+        err = "parameter arrays must be zipped: |H| = |eta| = |rho| and |T| = |K|";
+        return ST_ERR_ARG;
+    }
+    if (r.H.size() * r.T.size() > 65535) {
+        err = "par_size*op_size above the supported maximum of 65535 outputs per call";
+        return ST_ERR_LIMIT;
+    }
+    if (r.N < 1 || r.M < 1) {
+        err = "N and M must be positive";
+        return ST_ERR_ARG;
+    }
+    if (r.N > 8192) {
+        err = "N above the supported maximum of 8192 steps";
+        return ST_ERR_LIMIT;
+    }
+    for (double h : r.H)
+        if (!(h > 0.0 && h < 1.0)) {
+            err = "H must lie in (0, 1)";
+            return ST_ERR_ARG;
+        }
+    for (double t : r.T)
+        if (!(t > 0.0)) {
+            err = "maturities must be positive";
+            return ST_ERR_ARG;
+        }
+    for (double k : r.K)
+        if (!(k > 0.0)) {
+            err = "strikes must be positive";
+            return ST_ERR_ARG;
+        }
+    for (double x : r.rho)
+        if (!(std::fabs(x) <= 1.0)) {
+            err = "rho must lie in [-1, 1]";
+            return ST_ERR_ARG;
+        }
+    if (!(r.xi >= 0.0)) {
+        err = "xi must be non-negative";
+        return ST_ERR_ARG;
+    }
+    return ST_OK;
+}
+
+Layout make_layout(const Request& r) {
+    Layout L;
+    const int N = r.N;
+    L.N = N;
+    const long P = long(r.H.size()), O = long(r.T.size());
+    // units: distinct H (getGamma per distinct H, rBergLib/rBergomiMT.cpp:318-324)
+    std::map<double, int> unit_of;
+    std::vector<int> unit_of_par(P);
+    for (long h = 0; h < P; ++h) {
+        auto it = unit_of.find(r.H[h]);
+        if (it == unit_of.end()) {
+            const int u = int(L.units.size());
+            unit_of[r.H[h]] = u;
+            const double H = r.H[h];
+            const double s2H = std::sqrt(2.0 * H);
+            const double rhoH = s2H / (H + 0.5);
+            Unit un;
+            un.H = H;
+            un.a = rhoH / s2H;
+            un.b = std::sqrt(1.0 - rhoH * rhoH) / s2H;
+            un.sc = s2H * std::pow(1.0 / N, H);
+            L.units.push_back(un);
+            // Gamma_0 = 0, Gamma_k = ((k+1)^(a+1) - k^(a+1))/(a+1)  (getGamma :143-149)
+            const double alpha = H - 0.5;
+            L.gamma.push_back(0.0);
+            for (int k = 1; k < N; ++k)
+                L.gamma.push_back((std::pow(k + 1.0, alpha + 1.0) - std::pow(double(k), alpha + 1.0)) /
+                                  (alpha + 1.0));
+            unit_of_par[h] = u;
+        } else {
+            unit_of_par[h] = it->second;
+        }
+    }
+    // groups: distinct (H, eta, T) for RT; (H, eta, T, rho) for plain MC
+    std::map<std::tuple<int, double, double, double>, int> group_of;
+    for (long h = 0; h < P; ++h) {
+        for (long t = 0; t < O; ++t) {
+            const int u = unit_of_par[h];
+            const double rho_key = r.method == METHOD_MC ? r.rho[h] : 0.0;
+            const auto key = std::make_tuple(u, r.eta[h], r.T[t], rho_key);
+            auto it = group_of.find(key);
+            int g;
+            if (it == group_of.end()) {
+                g = int(L.groups.size());
+                group_of[key] = g;
+                const double H = r.H[h], T = r.T[t], eta = r.eta[h];
+                Group G;
+                G.unit = u;
+                G.pad = 0;
+                G.xi = r.xi;
+                G.eta = eta;
+                G.TH = std::pow(T, H);        // scaleWtilde :163-167
+                G.dt = T / N;
+                G.sdt = std::sqrt(G.dt);
+                G.rho = r.rho[h];
+                G.crho = std::sqrt(1.0 - r.rho[h] * r.rho[h]);
+                L.groups.push_back(G);
+                // compensator of updateV (:73-80): 0.5 eta^2 ((i-1) dt)^(2H), or (i dt) in exact mode
+                L.comp.push_back(0.0);
+                for (int i = 1; i < N; ++i) {
+                    const double tc = r.comp == COMP_EXACT ? i * G.dt : (i - 1) * G.dt;
+                    L.comp.push_back(0.5 * eta * eta * std::pow(tc, 2 * H));
+                }
+            } else {
+                g = it->second;
+            }
+            Option o;
+            o.group = g;
+            o.pad = 0;
+            o.rho = r.rho[h];
+            o.K = r.K[t];
+            o.lnK = std::log(r.K[t]);
+            o.T = r.T[t];
+            L.options.push_back(o);
+        }
+    }
+    return L;
+}
+
+// splitmix64 finaliser (Steele, Lea, Flood, OOPSLA'14) folded over the seed words
+static uint64_t splitmix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+PhiloxKey philox_key(const uint64_t* seed, int n) {
+    uint64_t h = splitmix64(0x243F6A8885A308D3ULL ^ uint64_t(n));
+    for (int i = 0; i < n; ++i) h = splitmix64(h ^ seed[i]);
+    PhiloxKey k;
+    k.k0 = uint32_t(h);
+    k.k1 = uint32_t(h >> 32);
+    return k;
+}
+
+#define RB_TRY(expr)                                                                 \
+    do {                                                                             \
+        hipError_t e_ = (expr);                                                      \
+        if (e_ != hipSuccess) {                                                      \
+            err = std::string(#expr) + ": " + hipGetErrorString(e_);                 \
+            return e_ == hipErrorOutOfMemory ? ST_ERR_NOMEM : ST_ERR_DEVICE;         \
+        }                                                                            \
+    } while (0)
+
+template <class T>
+static hipError_t dalloc(T** p, size_t n) {
+    return hipMalloc(reinterpret_cast<void**>(p), std::max<size_t>(n, 1) * sizeof(T));
+}
+
+template <class T>
+static hipError_t upload(T** p, const std::vector<T>& v, hipStream_t st) {
+    hipError_t e = dalloc(p, v.size());
+    if (e != hipSuccess || v.empty()) return e;
+    return hipMemcpyAsync(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st);
+}
+
+Plan::~Plan() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    void* bufs[] = {d_units, d_gamma, d_groups, d_comp, d_opts,  d_Z,  d_Wt,
+                    d_ints,  d_partial, d_price, d_stat, d_iv, d_payoff};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
+    if (own_stream && stream) (void)hipStreamDestroy(stream);
+}
+
+int Plan::init(const Request& r, hipStream_t st, bool quadrature, std::string& err) {
+    int rc = validate(r, err);
+    if (rc) return rc;
+    req = r;
+    lay = make_layout(r);
+    int ndev = 0;
+    RB_TRY(hipGetDeviceCount(&ndev));
+    if (ndev < 1) {
+        err = "no HIP device visible";
+        return ST_ERR_DEVICE;
+    }
+    RB_TRY(hipGetDevice(&device));
+    {
+        hipDeviceProp_t prop;
+        RB_TRY(hipGetDeviceProperties(&prop, device));
+        if (volterra_f64_lds_bytes(r.N) > prop.sharedMemPerBlock) {
+            err = "N = " + std::to_string(r.N) + " needs more LDS than one workgroup can hold";
+            return ST_ERR_LIMIT;
+        }
+    }
+    if (st) {
+        stream = st;
+    } else {
+        RB_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        own_stream = true;
+    }
+    const int N = r.N;
+    nstreams = r.method == METHOD_MC ? 3 : 2;
+    const int nopt = int(lay.options.size());
+    const long nunits = long(lay.units.size()), ngroups = long(lay.groups.size());
+    // chunk the paths so that the staged buffers stay within ~6 GiB
+    const double per_path = 8.0 * N * (nstreams + nunits) + 16.0 * ngroups +
+                            (quadrature ? 8.0 * nopt : 0.0);
+    long ch = long(6.0e9 / per_path);
+    // RBERGOMI_MAX_CHUNK caps the paths per chunk (used to test chunk boundaries)
+    if (const char* env = std::getenv("RBERGOMI_MAX_CHUNK")) {
+        const long cap = std::atol(env);
+        if (cap > 0) ch = std::min(ch, cap);
+    }
+    ch = std::max<long>(256, (ch / 256) * 256);
+    const long Mr = ((r.M + 255) / 256) * 256;
+    chunk = std::min(ch, Mr);
+    nblk = (r.M + 255) / 256;
+    RB_TRY(upload(&d_units, lay.units, stream));
+    RB_TRY(upload(&d_gamma, lay.gamma, stream));
+    RB_TRY(upload(&d_groups, lay.groups, stream));
+    RB_TRY(upload(&d_comp, lay.comp, stream));
+    RB_TRY(upload(&d_opts, lay.options, stream));
+    RB_TRY(dalloc(&d_Z, size_t(chunk) * nstreams * N));
+    RB_TRY(dalloc(&d_Wt, size_t(nunits) * chunk * N));
+    RB_TRY(dalloc(&d_ints, size_t(ngroups) * chunk * 2));
+    RB_TRY(dalloc(&d_partial, size_t(nopt) * nblk * 2));
+    RB_TRY(dalloc(&d_price, nopt));
+    RB_TRY(dalloc(&d_stat, nopt));
+    RB_TRY(dalloc(&d_iv, nopt));
+    if (quadrature) RB_TRY(dalloc(&d_payoff, size_t(chunk) * nopt));
+    RB_TRY(hipStreamSynchronize(stream));
+    return ST_OK;
+}
+
+void Plan::begin_timing() {
+    pending.clear();
+    ev_used = 0;
+    times.assign(K_COUNT, KernelTime());
+    for (int i = 0; i < K_COUNT; ++i) times[i].name = kernel_name(i);
+}
+
+int Plan::timed(int id) {
+    if (!profile) return -1;
+    while (ev_pool.size() < ev_used + 2) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return -1;
+        ev_pool.push_back(e);
+    }
+    hipEvent_t a = ev_pool[ev_used], b = ev_pool[ev_used + 1];
+    ev_used += 2;
+    pending.push_back({id, {a, b}});
+    (void)hipEventRecord(a, stream);
+    return int(pending.size()) - 1;
+}
+
+void Plan::mark(int slot, hipEvent_t, hipEvent_t) {
+    if (slot < 0) return;
+    (void)hipEventRecord(pending[slot].second.second, stream);
+}
+
+void Plan::end_timing() {
+    if (!profile) return;
+    for (auto& p : pending) {
+        float ms = 0.f;
+        if (hipEventSynchronize(p.second.second) == hipSuccess &&
+            hipEventElapsedTime(&ms, p.second.first, p.second.second) == hipSuccess) {
+            times[p.first].total_ms += ms;
+            times[p.first].launches += 1;
+        }
+    }
+    pending.clear();
+}
+
+#define RB_LAUNCH(id, call)                  \
+    do {                                     \
+        const int slot_ = timed(id);         \
+        RB_TRY(call);                        \
+        mark(slot_, nullptr, nullptr);       \
+    } while (0)
+
+int Plan::run(std::string& err) {
+    const int N = req.N;
+    const int nopt = int(lay.options.size());
+    const PhiloxKey key = philox_key(req.seed.data(), int(req.seed.size()));
+    begin_timing();
+    for (long p0 = 0; p0 < req.M; p0 += chunk) {
+        const long np = std::min(chunk, req.M - p0);
+        RB_LAUNCH(K_NORMALS, launch_normals_f64(key, p0, np, nstreams, N, d_Z, stream));
+        RB_LAUNCH(K_VOLTERRA, launch_volterra_f64(d_Z, long(nstreams) * N, np, N, int(lay.units.size()),
+                                                  d_units, d_gamma, d_Wt, stream));
+        RB_LAUNCH(K_SCAN, launch_scan_f64(req.method, d_Z, long(nstreams) * N, d_Wt, np, N, d_groups,
+                                          int(lay.groups.size()), d_comp, d_ints, stream));
+        RB_LAUNCH(K_PAYOFF, launch_payoff_f64(d_ints, np, p0, d_opts, nopt, req.method, p0 / 256, nblk,
+                                              d_partial, nullptr, stream));
+    }
+    RB_LAUNCH(K_REDUCE, launch_reduce_f64(d_partial, nblk, req.M, nopt, d_price, d_stat, stream));
+    RB_LAUNCH(K_IV, launch_iv_f64(d_price, d_opts, nopt, d_iv, stream));
+    RB_TRY(hipStreamSynchronize(stream));
+    end_timing();
+    return ST_OK;
+}
+
+int Plan::run_samples(const double* Z_host, double* payoff_host, std::string& err) {
+    const int N = req.N;
+    const int nopt = int(lay.options.size());
+    begin_timing();
+    for (long p0 = 0; p0 < req.M; p0 += chunk) {
+        const long np = std::min(chunk, req.M - p0);
+        RB_TRY(hipMemcpyAsync(d_Z, Z_host + size_t(p0) * nstreams * N,
+                              size_t(np) * nstreams * N * sizeof(double), hipMemcpyHostToDevice, stream));
+        RB_LAUNCH(K_VOLTERRA, launch_volterra_f64(d_Z, long(nstreams) * N, np, N, int(lay.units.size()),
+                                                  d_units, d_gamma, d_Wt, stream));
+        RB_LAUNCH(K_SCAN, launch_scan_f64(req.method, d_Z, long(nstreams) * N, d_Wt, np, N, d_groups,
+                                          int(lay.groups.size()), d_comp, d_ints, stream));
+        RB_LAUNCH(K_PAYOFF, launch_payoff_f64(d_ints, np, 0, d_opts, nopt, req.method, p0 / 256, nblk,
+                                              d_partial, d_payoff, stream));
+        RB_TRY(hipMemcpyAsync(payoff_host + size_t(p0) * nopt, d_payoff, size_t(np) * nopt * sizeof(double),
+                              hipMemcpyDeviceToHost, stream));
+        RB_TRY(hipStreamSynchronize(stream));
+    }
+    RB_LAUNCH(K_REDUCE, launch_reduce_f64(d_partial, nblk, req.M, nopt, d_price, d_stat, stream));
+    RB_LAUNCH(K_IV, launch_iv_f64(d_price, d_opts, nopt, d_iv, stream));
+    RB_TRY(hipStreamSynchronize(stream));
+    end_timing();
+    return ST_OK;
+}
+
+int Plan::results(double* price, double* iv, double* stat, std::string& err) {
+    const size_t n = lay.options.size() * sizeof(double);
+    if (price) RB_TRY(hipMemcpyAsync(price, d_price, n, hipMemcpyDeviceToHost, stream));
+    if (iv) RB_TRY(hipMemcpyAsync(iv, d_iv, n, hipMemcpyDeviceToHost, stream));
+    if (stat) RB_TRY(hipMemcpyAsync(stat, d_stat, n, hipMemcpyDeviceToHost, stream));
+    RB_TRY(hipStreamSynchronize(stream));
+    return ST_OK;
+}
+
+}  // namespace rb

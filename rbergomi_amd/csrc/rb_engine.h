@@ -1,0 +1,112 @@
+// rb_engine.h -- host planner and executor of the rough-Bergomi pricing path.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "rb_device.h"
+#include "rb_types.h"
+
+namespace rb {
+
+enum Precision { PREC_FP64 = 0, PREC_FP32 = 1, PREC_SPLIT_FP16 = 2, PREC_SPLIT_BF16 = 3 };
+enum Compensator { COMP_REF = 0, COMP_EXACT = 1 };
+enum Status {
+    ST_OK = 0,
+    ST_ERR_ARG = 1,
+    ST_ERR_SIZE = 2,
+    ST_ERR_DEVICE = 3,
+    ST_ERR_NOMEM = 4,
+    ST_ERR_LIMIT = 5,
+};
+
+// One pricing request in the reference's tensor form: (H, eta, rho) zipped over par_size,
+// (T, K) zipped over op_size, outputs I = h*op_size + t (rBergLib/ParamTot.h:32-33).
+struct Request {
+    double xi = 0.0;
+    std::vector<double> H, eta, rho, T, K;
+    int N = 0;
+    long M = 0;
+    std::vector<uint64_t> seed;
+    int method = METHOD_RT;
+    int precision = PREC_FP64;
+    int comp = COMP_REF;
+    int out_size() const { return int(H.size() * T.size()); }
+};
+
+// Deduplicated work tables (see rb_types.h)
+struct Layout {
+    int N = 0;
+    std::vector<Unit> units;
+    std::vector<double> gamma;  // [units][N]
+    std::vector<Group> groups;
+    std::vector<double> comp;   // [groups][N]: c_i = eta^2/2 ((i-1) dt)^(2H), c_0 = 0
+    std::vector<Option> options;
+};
+
+int validate(const Request& r, std::string& err);
+Layout make_layout(const Request& r);
+PhiloxKey philox_key(const uint64_t* seed, int n);
+
+// Per-kernel timing accumulated over the launches of one run (HIP events on the run's stream).
+struct KernelTime {
+    std::string name;
+    double total_ms = 0.0;
+    int launches = 0;
+};
+
+// Device-resident plan: tables uploaded once, chunk buffers allocated once, run many times.
+struct Plan {
+    Request req;
+    Layout lay;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    bool profile = false;
+    long chunk = 0;          // paths per chunk
+    long nblk = 0;           // 256-path blocks over all M
+    int nstreams = 2;        // Gaussian blocks per path: W1, W1perp (+ Wperp for plain MC)
+    // device tables
+    Unit* d_units = nullptr;
+    double* d_gamma = nullptr;
+    Group* d_groups = nullptr;
+    double* d_comp = nullptr;
+    Option* d_opts = nullptr;
+    // device work buffers
+    double* d_Z = nullptr;        // [chunk][nstreams][N]
+    double* d_Wt = nullptr;       // [units][chunk][N]
+    double* d_ints = nullptr;     // [groups][chunk][2]
+    double* d_partial = nullptr;  // [options][nblk][2]
+    double* d_price = nullptr;
+    double* d_stat = nullptr;
+    double* d_iv = nullptr;
+    double* d_payoff = nullptr;   // [chunk][options], quadrature mode only
+    std::vector<KernelTime> times;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
+
+    ~Plan();
+    int init(const Request& r, hipStream_t st, bool quadrature, std::string& err);
+    // seeded run: Philox Gaussians on the device
+    int run(std::string& err);
+    // user Gaussians (host buffer, quadrature layout), per-path payoffs back to the host
+    int run_samples(const double* Z_host, double* payoff_host, std::string& err);
+    int results(double* price, double* iv, double* stat, std::string& err);
+    void begin_timing();
+    void end_timing();
+    int timed(int id);
+    void mark(int id, hipEvent_t a, hipEvent_t b);
+};
+
+// Kernel ids for timing
+enum KernelId { K_NORMALS = 0, K_VOLTERRA, K_SCAN, K_PAYOFF, K_REDUCE, K_IV, K_COUNT };
+const char* kernel_name(int id);
+
+// thread-local error channel of the C API
+void set_last_error(const std::string& s);
+const char* last_error();
+
+}  // namespace rb

@@ -1,0 +1,30 @@
+// rb_kernels.h -- host-side launchers of the HIP kernels (defined in the .hip files).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+#include "rb_device.h"
+#include "rb_types.h"
+
+namespace rb {
+
+// staged FP64 path (rb_staged.hip)
+hipError_t launch_normals_f64(PhiloxKey key, long p0, long np, int nstreams, int N, double* out,
+                              hipStream_t st);
+size_t volterra_f64_lds_bytes(int N);
+hipError_t launch_volterra_f64(const double* Z, long zstride, long np, int N, int nunits,
+                               const Unit* units, const double* gamma, double* Wt,
+                               hipStream_t st);
+hipError_t launch_scan_f64(int method, const double* Z, long zstride, const double* Wt, long np,
+                           int N, const Group* groups, int ngroups, const double* comp,
+                           double* out, hipStream_t st);
+hipError_t launch_payoff_f64(const double* ints, long np, long p0, const Option* opts,
+                             int out_size, int method, long blk0, long nblk_total,
+                             double* partial, double* payoff, hipStream_t st);
+hipError_t launch_reduce_f64(const double* partial, long nblk, long M, int out_size,
+                             double* price, double* stat, hipStream_t st);
+hipError_t launch_iv_f64(const double* price, const Option* opts, int n, double* iv,
+                         hipStream_t st);
+hipError_t launch_philox_probe(const void* ctr, PhiloxKey key, long n, void* out, hipStream_t st);
+
+}  // namespace rb

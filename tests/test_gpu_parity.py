@@ -1,0 +1,236 @@
+"""GPU parity of the HIP path (through the C ABI) against the CPU oracle.
+
+Tolerances (FP64 path): per-path Wtilde and integrals <= 1e-10 relative (floor 1e-10
+absolute), payoffs <= 1e-10 relative + 1e-13 absolute, prices <= 1e-6 absolute (observed
+~1e-15).  Independent random streams: within 4.5 combined standard errors.
+"""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def rb():
+    import rbergomi_amd
+    rbergomi_amd.load()
+    return rbergomi_amd
+
+
+def relerr(a, b, floor=1.0):
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), floor))) if np.size(a) else 0.0
+
+
+def test_philox_device_matches_known_answers(rb, orc):
+    import ctypes
+    L = rb.load()
+    rng = np.random.default_rng(1)
+    ctr = np.concatenate([np.array([[0, 0, 0, 0], [0xFFFFFFFF] * 4,
+                                    [0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344]], dtype=np.uint32),
+                          rng.integers(0, 2 ** 32, (61, 4), dtype=np.uint64).astype(np.uint32)])
+    keys = [[0, 0], [0xFFFFFFFF] * 2, [0xA4093822, 0x299F31D0]]
+    for key in keys:
+        k = np.array(key, dtype=np.uint32)
+        out = np.zeros_like(ctr)
+        u = ctypes.POINTER(ctypes.c_uint32)
+        rc = L.rb_probe_philox(ctr.ctypes.data_as(u), k.ctypes.data_as(u), len(ctr), out.ctypes.data_as(u))
+        assert rc == 0
+        for i in range(len(ctr)):
+            assert list(out[i]) == list(orc.philox(ctr[i], k))
+
+
+def test_device_normals_match_oracle(rb, orc):
+    import ctypes
+    L = rb.load()
+    seed = np.array([5, 6, 7], dtype=np.uint64)
+    for (p0, np_, ns, N) in [(0, 300, 2, 64), (123456789012, 17, 3, 13)]:
+        say("normals", p0, np_, ns, N)
+        out = np.zeros((np_, ns, N))
+        rc = L.rb_probe_normals(seed.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), 3, p0, np_, ns, N,
+                                out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        assert rc == 0
+        ref = orc.normals_philox(seed, p0, np_, ns, N)
+        assert np.max(np.abs(out - ref)) < 1e-13
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 5, 64, 200, 256, 511, 512, 1024, 2048, 4096])
+@pytest.mark.parametrize("H", [0.02, 0.07, 0.45])
+def test_wtilde_matches_oracle(rb, orc, N, H):
+    import ctypes
+    say("wtilde N=%d H=%g" % (N, H))
+    L = rb.load()
+    M = 7 if N <= 1024 else 3
+    rng = np.random.default_rng(N)
+    Z = rng.standard_normal((M, 2, N))
+    out = np.zeros((M, N))
+    dp = ctypes.POINTER(ctypes.c_double)
+    rc = L.rb_probe_wtilde(Z.ctypes.data_as(dp), M, N, H, out.ctypes.data_as(dp))
+    assert rc == 0, L.rb_last_error()
+    for m in range(M):
+        ref = orc.wtilde(Z[m, 0], Z[m, 1], H)
+        assert relerr(out[m], ref) < 1e-10
+
+
+def _opts():
+    # two triples sharing H, three maturities, strikes across the smile
+    H = [0.07, 0.1, 0.07]
+    eta = [1.9, 1.5, 2.5]
+    rho = [-0.9, -0.7, -0.5]
+    T = [0.1, 0.5, 0.5, 2.0, 1.0]
+    K = [0.9, 0.8, 1.25, 1.0, 1.6]
+    return H, eta, rho, T, K
+
+
+@pytest.mark.parametrize("N,M", [(1, 50), (7, 300), (64, 1000), (256, 513)])
+def test_integrals_match_oracle(rb, orc, N, M):
+    import ctypes
+    say("integrals N=%d M=%d" % (N, M))
+    L = rb.load()
+    H, eta, rho, T, K = _opts()
+    xi = 0.05
+    rng = np.random.default_rng(M)
+    Z = rng.standard_normal((M, 2, N))
+    a = lambda x: np.ascontiguousarray(x, dtype=np.float64)
+    dp = ctypes.POINTER(ctypes.c_double)
+    arrs = [a(x) for x in (H, eta, rho, T, K)]
+    out = np.zeros((M, 15, 2))
+    rc = L.rb_probe_integrals(xi, arrs[0].ctypes.data_as(dp), arrs[1].ctypes.data_as(dp),
+                              arrs[2].ctypes.data_as(dp), 3, arrs[3].ctypes.data_as(dp),
+                              arrs[4].ctypes.data_as(dp), 5, N, M, Z.ctypes.data_as(dp), None,
+                              out.ctypes.data_as(dp))
+    assert rc == 0, L.rb_last_error()
+    ref = orc.integrals_rt_samples(xi, H, eta, rho, T, K, N, Z)
+    assert relerr(out, ref, floor=1e-3) < 1e-10
+
+
+@pytest.mark.parametrize("N,M", [(16, 256), (64, 1000), (100, 257), (512, 300)])
+def test_quadrature_payoffs_match_oracle(rb, orc, N, M):
+    say("quadrature N=%d M=%d" % (N, M))
+    H, eta, rho, T, K = _opts()
+    xi = 0.055225
+    rng = np.random.default_rng(N + M)
+    Z = rng.standard_normal((M, 2, N))
+    gpu = rb.computeIVRT_tensor_quadrature(xi, H, eta, rho, T, K, N, Z)
+    ref = orc.payoff_rt_samples(xi, H, eta, rho, T, K, N, Z)
+    assert gpu.shape == ref.shape
+    assert np.all(np.abs(gpu - ref) <= 1e-10 * np.abs(ref) + 1e-13)
+    p_gpu = gpu.mean(axis=0)
+    p_ref, _ = orc.price_stat(ref)
+    assert np.max(np.abs(p_gpu - p_ref)) < 1e-6
+
+
+def test_quadrature_reproduces_reference_debug8(rb):
+    with open(os.path.join(HERE, "golden", "debug8_payoffs.json")) as f:
+        fx = json.load(f)
+    p = fx["params"]
+    N = p["N"]
+    Z = np.zeros((200, 2, N))
+    for m in range(200):
+        Z[m, 0] = Z[m, 1] = np.cos(np.arange(N, dtype=np.float64) * m)
+    pay = rb.computeIVRT_tensor_quadrature(p["xi"], p["H"], p["eta"], p["rho"], p["T"], p["K"], N, Z)[:, 0]
+    g = np.array(fx["payoff"])
+    assert np.all(np.abs(pay - g) <= 5.01e-6 * np.abs(g) + 1e-300)
+
+
+def test_plain_mc_payoffs_match_oracle(rb, orc):
+    H, eta, rho, T, K = _opts()
+    N, M, xi = 48, 700, 0.04
+    Z = np.random.default_rng(11).standard_normal((M, 3, N))
+    payoff, price, stat = rb.payoff_samples_v2(xi, H, eta, rho, T, K, N, Z, method=rb.METHOD_MC)
+    ref = orc.payoff_mc_samples(xi, H, eta, rho, T, K, N, Z)
+    assert np.all(np.abs(payoff - ref) <= 1e-10 * np.abs(ref) + 1e-13)
+    pr, st = orc.price_stat(ref)
+    assert np.max(np.abs(price - pr)) < 1e-12 and np.max(np.abs(stat - st)) < 1e-12
+
+
+@pytest.mark.parametrize("method", ["rt", "mc"])
+def test_seeded_prices_match_oracle_on_same_gaussians(rb, orc, method):
+    H, eta, rho, T, K = _opts()
+    N, M, xi = 32, 3000, 0.045
+    seed = [42, 7]
+    m = rb.METHOD_RT if method == "rt" else rb.METHOD_MC
+    price, iv, stat = rb.price_v2(xi, H, eta, rho, T, K, N, M, seed, method=m)
+    Z = orc.normals_philox(seed, 0, M, 2 if method == "rt" else 3, N)
+    if method == "rt":
+        ref = orc.payoff_rt_samples(xi, H, eta, rho, T, K, N, Z)
+    else:
+        ref = orc.payoff_mc_samples(xi, H, eta, rho, T, K, N, Z)
+    pr, st = orc.price_stat(ref)
+    assert np.max(np.abs(price - pr)) < 1e-12
+    assert np.max(np.abs(stat - st)) < 1e-12
+    for I in range(len(price)):
+        want = orc.iv_call(pr[I], 1.0, K[I % 5], T[I % 5])
+        if np.isnan(want):
+            assert np.isnan(iv[I])
+        else:
+            assert abs(iv[I] - want) < 1e-9
+
+
+def test_chunking_and_reruns_are_bit_identical(rb, monkeypatch):
+    H, eta, rho, T, K = _opts()
+    args = (0.04, H, eta, rho, T, K, 40, 2000, [3, 1, 4])
+    a = rb.price_v2(*args)
+    b = rb.price_v2(*args)
+    monkeypatch.setenv("RBERGOMI_MAX_CHUNK", "256")
+    c = rb.price_v2(*args)
+    for x, y, z in zip(a, b, c):
+        assert np.array_equal(x, y, equal_nan=True) and np.array_equal(x, z, equal_nan=True)
+
+
+REF_ARGS = dict(xi=0.07, H=[0.07, 0.1], eta=[2.2, 2.4], rho=[-0.9, -0.85], T=[0.5, 1.0], K=[0.8, 1.0])
+REF_PRICE = [0.2149403, 0.08461882, 0.2157838, 0.07758564]
+
+
+@pytest.mark.parametrize("method", ["rt", "mc"])
+def test_reference_known_prices_and_independent_oracle(rb, orc, method):
+    # testRBergLib/tests.cpp:16-35: N = 256, M = 10*256^2, tolerance 0.005 + stat
+    a = REF_ARGS
+    N, M = 256, 10 * 256 * 256
+    fn = rb.computeIV_RT if method == "rt" else rb.computeIV
+    res = fn(a["xi"], a["H"], a["eta"], a["rho"], a["T"], a["K"], N, M, [123, 452, 567, 248], 8)
+    assert np.all(np.abs(res["price"] - REF_PRICE) < 0.005 + res["stat"])
+    # independent streams (oracle: mt19937_64 + std::normal_distribution) agree within 4.5 SE
+    ofn = orc.compute_rt if method == "rt" else orc.compute_mc
+    p2, _, s2 = ofn(a["xi"], a["H"], a["eta"], a["rho"], a["T"], a["K"], N, M // 8,
+                    [123, 452, 567, 248], num_threads=8)
+    assert np.all(np.abs(res["price"] - p2) < 4.5 * np.sqrt(res["stat"] ** 2 + s2 ** 2))
+    # echo arrays; computeIV_tensor swaps eta and rho like rBergLib/cinterface.cpp:45
+    assert list(res["H"]) == [0.07, 0.07, 0.1, 0.1] and list(res["T"]) == [0.5, 1.0, 0.5, 1.0]
+    if method == "rt":
+        assert list(res["eta"]) == [2.2, 2.2, 2.4, 2.4] and list(res["rho"]) == [-0.9, -0.9, -0.85, -0.85]
+    else:
+        assert list(res["eta"]) == [-0.9, -0.9, -0.85, -0.85] and list(res["rho"]) == [2.2, 2.2, 2.4, 2.4]
+
+
+def test_martingale_and_bs_limits_seeded(rb):
+    # K -> 0: E[S'] = 1;  eta = rho = 0: every path pays C_BS(1, K, 1, sqrt(xi T)) exactly
+    price, _, stat = rb.price_v2(0.06, [0.1], [2.0], [-0.9], [1.0], [1e-250], 64, 200000, [9])
+    assert abs(price[0] - 1.0) < 4.5 * stat[0]
+    from scipy import stats
+    T, K, xi = [0.5, 2.0], [1.0, 1.2], 0.05
+    price, iv, stat = rb.price_v2(xi, [0.1], [0.0], [0.0], T, K, 16, 1000, [1])
+    for j in range(2):
+        s = np.sqrt(xi * T[j])
+        d1 = (np.log(1 / K[j]) + 0.5 * s * s) / s
+        want = stats.norm.cdf(d1) - K[j] * stats.norm.cdf(d1 - s)
+        assert abs(price[j] - want) < 1e-14
+        assert abs(iv[j] - np.sqrt(xi)) < 1e-10
+
+
+def test_cpp_interface_symbols_callable_via_legacy_equivalence(rb):
+    # single/multi "thread" hints do not change results (counter-based Gaussians)
+    a = REF_ARGS
+    r1 = rb.computeIV_RT(a["xi"], a["H"], a["eta"], a["rho"], a["T"], a["K"], 32, 5000, [1, 2], 1)
+    r8 = rb.computeIV_RT(a["xi"], a["H"], a["eta"], a["rho"], a["T"], a["K"], 32, 5000, [1, 2], 8)
+    assert np.array_equal(r1["price"], r8["price"])
