@@ -69,6 +69,8 @@ def load():
     L.rb_probe_philox.argtypes = [u32p, u32p, l, u32p]
     L.rb_probe_normals.argtypes = [u64p, i, l, l, i, i, dp]
     L.rb_probe_wtilde.argtypes = [dp, l, i, d, dp]
+    fp = ctypes.POINTER(ctypes.c_float)
+    L.rb_probe_mfma_layout.argtypes = [fp, fp, i, fp]
     L.rb_probe_integrals.argtypes = [d, dp, dp, dp, i, dp, dp, i, i, l, dp, cfgp, dp]
     _lib = L
     return _lib

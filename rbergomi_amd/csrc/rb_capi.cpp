@@ -317,6 +317,23 @@ done:
     return rc;
 }
 
+int rb_probe_mfma_layout(const float* A, const float* B, int bf16, float* out) {
+    int rc = RB_OK;
+    float *dA = nullptr, *dB = nullptr, *dO = nullptr;
+    PROBE_TRY(hipMalloc(&dA, 512 * sizeof(float)));
+    PROBE_TRY(hipMalloc(&dB, 512 * sizeof(float)));
+    PROBE_TRY(hipMalloc(&dO, 1024 * sizeof(float)));
+    PROBE_TRY(hipMemcpy(dA, A, 512 * sizeof(float), hipMemcpyHostToDevice));
+    PROBE_TRY(hipMemcpy(dB, B, 512 * sizeof(float), hipMemcpyHostToDevice));
+    PROBE_TRY(launch_probe_mfma_layout(dA, dB, dO, bf16, nullptr));
+    PROBE_TRY(hipMemcpy(out, dO, 1024 * sizeof(float), hipMemcpyDeviceToHost));
+done:
+    if (dA) (void)hipFree(dA);
+    if (dB) (void)hipFree(dB);
+    if (dO) (void)hipFree(dO);
+    return rc;
+}
+
 int rb_probe_normals(const uint64_t* seed, int seed_size, long p0, long np, int nstreams, int N,
                      double* out) {
     int rc = RB_OK;

@@ -25,7 +25,8 @@ const char* last_error() { return g_last_error.c_str(); }
 
 const char* kernel_name(int id) {
     static const char* names[K_COUNT] = {"k_normals_f64", "k_volterra_f64", "k_scan_f64",
-                                         "k_payoff_f64",  "k_reduce_f64",   "k_iv_f64"};
+                                         "k_payoff_f64",  "k_reduce_f64",   "k_iv_f64",
+                                         "k_fused"};
     return id >= 0 && id < K_COUNT ? names[id] : "?";
 }
 
@@ -196,8 +197,8 @@ static hipError_t upload(T** p, const std::vector<T>& v, hipStream_t st) {
 
 Plan::~Plan() {
     if (stream) (void)hipStreamSynchronize(stream);
-    void* bufs[] = {d_units, d_gamma, d_groups, d_comp, d_opts,  d_Z,  d_Wt,
-                    d_ints,  d_partial, d_price, d_stat, d_iv, d_payoff};
+    void* bufs[] = {d_units, d_gamma, d_groups, d_comp, d_opts,  d_Z,  d_Wt,   d_ints, d_partial,
+                    d_price, d_stat,  d_iv,     d_payoff, d_fu, d_fg, d_fpg, d_fo, d_frag, d_ptab};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
@@ -229,6 +230,19 @@ int Plan::init(const Request& r, hipStream_t st, bool quadrature, std::string& e
     } else {
         RB_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         own_stream = true;
+    }
+    if (r.precision == PREC_SPLIT_FP16 || r.precision == PREC_SPLIT_BF16) {
+        if (quadrature) {
+            err = "user-supplied Gaussians are priced on the FP64 path only";
+            return ST_ERR_ARG;
+        }
+        fused = true;
+        fused_mode = r.precision == PREC_SPLIT_FP16 ? 0 : 1;
+        return init_fused(err);
+    }
+    if (r.precision != PREC_FP64) {
+        err = "precision not available in this build";
+        return ST_ERR_ARG;
     }
     const int N = r.N;
     nstreams = r.method == METHOD_MC ? 3 : 2;
@@ -311,6 +325,7 @@ void Plan::end_timing() {
     } while (0)
 
 int Plan::run(std::string& err) {
+    if (fused) return run_fused(err);
     const int N = req.N;
     const int nopt = int(lay.options.size());
     const PhiloxKey key = philox_key(req.seed.data(), int(req.seed.size()));

@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "rb_device.h"
+#include "rb_fused.h"
 #include "rb_types.h"
 
 namespace rb {
@@ -83,6 +84,18 @@ struct Plan {
     double* d_stat = nullptr;
     double* d_iv = nullptr;
     double* d_payoff = nullptr;   // [chunk][options], quadrature mode only
+    // fused mixed-precision path
+    bool fused = false;
+    int fused_mode = 0;           // 0 = FP16 split, 1 = BF16 split
+    int fused_upw = 1;
+    int fused_grid = 0;
+    FusedArgs fa{};
+    FusedUnit* d_fu = nullptr;
+    FusedGroup* d_fg = nullptr;
+    FusedOption* d_fo = nullptr;
+    FusedPG* d_fpg = nullptr;
+    uint4* d_frag = nullptr;
+    float* d_ptab = nullptr;
     std::vector<KernelTime> times;
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
@@ -95,6 +108,8 @@ struct Plan {
     // user Gaussians (host buffer, quadrature layout), per-path payoffs back to the host
     int run_samples(const double* Z_host, double* payoff_host, std::string& err);
     int results(double* price, double* iv, double* stat, std::string& err);
+    int init_fused(std::string& err);
+    int run_fused(std::string& err);
     void begin_timing();
     void end_timing();
     int timed(int id);
@@ -102,7 +117,7 @@ struct Plan {
 };
 
 // Kernel ids for timing
-enum KernelId { K_NORMALS = 0, K_VOLTERRA, K_SCAN, K_PAYOFF, K_REDUCE, K_IV, K_COUNT };
+enum KernelId { K_NORMALS = 0, K_VOLTERRA, K_SCAN, K_PAYOFF, K_REDUCE, K_IV, K_FUSED, K_COUNT };
 const char* kernel_name(int id);
 
 // thread-local error channel of the C API

@@ -26,5 +26,8 @@ hipError_t launch_reduce_f64(const double* partial, long nblk, long M, int out_s
 hipError_t launch_iv_f64(const double* price, const Option* opts, int n, double* iv,
                          hipStream_t st);
 hipError_t launch_philox_probe(const void* ctr, PhiloxKey key, long n, void* out, hipStream_t st);
+// MFMA layout probe (rb_probe.hip)
+hipError_t launch_probe_mfma_layout(const float* A, const float* B, float* out, int bf16,
+                                    hipStream_t st);
 
 }  // namespace rb

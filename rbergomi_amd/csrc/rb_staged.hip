@@ -127,8 +127,8 @@ k_scan_rt_f64(const double* __restrict__ Z, long zstride, const double* __restri
     const long wid = (long(blockIdx.x) * blockDim.x + threadIdx.x) / 64;
     const int lane = threadIdx.x & 63;
     if (wid >= np * ngroups) return;
-    const int gi = int(wid / np);
-    const long p = wid % np;
+    const int gi = int(wid % ngroups);  // the groups of one path are adjacent waves
+    const long p = wid / ngroups;
     const Group G = groups[gi];
     const double* wt = Wt + (long(G.unit) * np + p) * N;
     const double* W1 = Z + p * zstride;
@@ -159,8 +159,8 @@ k_scan_mc_f64(const double* __restrict__ Z, long zstride, const double* __restri
     const long wid = (long(blockIdx.x) * blockDim.x + threadIdx.x) / 64;
     const int lane = threadIdx.x & 63;
     if (wid >= np * ngroups) return;
-    const int gi = int(wid / np);
-    const long p = wid % np;
+    const int gi = int(wid % ngroups);  // the groups of one path are adjacent waves
+    const long p = wid / ngroups;
     const Group G = groups[gi];
     const double* wt = Wt + (long(G.unit) * np + p) * N;
     const double* W1 = Z + p * zstride;

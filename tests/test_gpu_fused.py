@@ -1,0 +1,97 @@
+"""GPU checks of the fused split-precision kernel.
+
+- the MFMA operand pairing and output-row layout it relies on (exact small-integer product);
+- prices against the FP64 path on identical Philox counters: <= 1e-4 absolute (the stated
+  mixed-precision budget); a layout or indexing mistake shows up as O(1e-2) errors;
+- bit-identical reruns (deterministic reduction order).
+"""
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def rb():
+    import rbergomi_amd
+    rbergomi_amd.load()
+    return rbergomi_amd
+
+
+@pytest.mark.parametrize("bf16", [0, 1])
+def test_mfma_layout(rb, bf16):
+    rng = np.random.default_rng(3 + bf16)
+    A = rng.integers(-2, 3, (32, 16)).astype(np.float32)
+    B = rng.integers(-4, 5, (16, 32)).astype(np.float32)
+    out = np.zeros(1024, dtype=np.float32)
+    fp = ctypes.POINTER(ctypes.c_float)
+    assert rb.load().rb_probe_mfma_layout(A.ctypes.data_as(fp), B.ctypes.data_as(fp), bf16,
+                                          out.ctypes.data_as(fp)) == 0
+    D = A.astype(np.float64) @ B.astype(np.float64)
+    for lane in range(64):
+        for r in range(16):
+            row = 8 * (r // 4) + 4 * (lane // 32) + (r % 4)
+            assert out[lane * 16 + r] == D[row, lane % 32], (lane, r)
+
+
+def opts_config2_like():
+    mats = [0.1, 0.25, 0.5, 1.0, 2.0]
+    ks = np.linspace(-0.4, 0.2, 41)
+    return np.repeat(mats, 41), np.exp(np.tile(ks, 5))
+
+
+CASES = [
+    # (N, M, H, eta, rho, T, K)
+    (512, 4096, [0.1], [1.5], [-0.7], None, None),
+    (256, 3000, [0.07, 0.1, 0.07], [1.9, 1.5, 2.5], [-0.9, -0.7, -0.5], [0.1, 0.5, 0.5, 2.0, 1.0],
+     [0.9, 0.8, 1.25, 1.0, 1.6]),
+    (64, 2000, [0.07], [1.9], [-0.9], [1.0] * 21, list(np.linspace(0.8, 1.2, 21))),
+    (32, 1500, [0.3], [0.8], [-0.3], [0.05, 2.0], [1.0, 0.7]),
+    (96, 999, [0.02, 0.45], [4.0, 0.5], [-0.99, -0.1], [0.2, 1.0], [0.9, 1.1]),
+    (200, 777, [0.07], [2.2], [-0.9], [0.5], [0.8]),
+    (1000, 600, [0.05], [2.3], [-0.9], [1.0, 1.0], [0.7, 1.3]),
+    (1024, 512, [0.02], [2.3], [-0.9], [1.0], [1.0]),
+]
+
+
+@pytest.mark.parametrize("prec", ["fp16", "bf16"])
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_fused_matches_fp64_on_same_counters(rb, case, prec):
+    N, M, H, eta, rho, T, K = CASES[case]
+    if T is None:
+        T, K = opts_config2_like()
+    xi = 0.04
+    seed = [11, case]
+    say("fused %s case %d N=%d M=%d" % (prec, case, N, M))
+    p64, iv64, s64 = rb.price_v2(xi, H, eta, rho, T, K, N, M, seed, precision=rb.PREC_FP64)
+    code = rb.PREC_SPLIT_FP16 if prec == "fp16" else rb.PREC_SPLIT_BF16
+    pm, ivm, sm = rb.price_v2(xi, H, eta, rho, T, K, N, M, seed, precision=code)
+    err = np.max(np.abs(pm - p64))
+    say("   max |price - fp64| = %.3e   max |stat - fp64| = %.3e" % (err, np.max(np.abs(sm - s64))))
+    assert err <= 1e-4
+    assert np.max(np.abs(sm - s64)) <= 1e-4
+
+
+def test_fused_reruns_are_bit_identical(rb):
+    T, K = opts_config2_like()
+    a = rb.price_v2(0.04, [0.1], [1.5], [-0.7], T, K, 512, 5000, [1], precision=rb.PREC_SPLIT_FP16)
+    b = rb.price_v2(0.04, [0.1], [1.5], [-0.7], T, K, 512, 5000, [1], precision=rb.PREC_SPLIT_FP16)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y, equal_nan=True)
+
+
+def test_fused_limits_are_reported(rb):
+    with pytest.raises(rb.RbError) as e:
+        rb.price_v2(0.04, [0.1], [1.5], [-0.7], [1.0], [1.0], 2048, 100, [1], precision=rb.PREC_SPLIT_FP16)
+    assert e.value.code == 5
+    with pytest.raises(rb.RbError) as e:
+        rb.price_v2(0.04, [0.1], [1.5], [-0.7], [1.0], [1.0], 64, 100, [1], precision=rb.PREC_SPLIT_FP16,
+                    method=rb.METHOD_MC)
+    assert e.value.code == 1
