@@ -142,6 +142,10 @@ int rb_probe_wtilde(const double* Z, long M, int N, double H, double* Wtilde);
 /* One 32x32x16 MFMA (bf16 = 0: FP16 operands, 1: BF16) of A[32][16] by B[16][32] built with
  * the operand layout of the fused kernel; out[lane*16 + r] = accumulator register r of lane. */
 int rb_probe_mfma_layout(const float* A, const float* B, int bf16, float* out);
+/* The split-precision kernel's per-path payoff for given (ln S', sigma, K) -> out[i], and its
+ * Mills-ratio approximation at z[i] -> mills[i] (n values). */
+int rb_probe_payoff(const double* lnS, const double* sig, const double* K, int n, double* out,
+                    const float* z, float* mills);
 /* Per-path integrals (Ivdt, IsvdW) of option I from Z: out[2*(m*out_size + I) + {0,1}] */
 int rb_probe_integrals(double xi, const double* H, const double* eta, const double* rho,
                        int par_size, const double* T, const double* K, int op_size, int N,

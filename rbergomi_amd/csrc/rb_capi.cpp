@@ -334,6 +334,31 @@ done:
     return rc;
 }
 
+int rb_probe_payoff(const double* lnS, const double* sig, const double* K, int n, double* out,
+                    const float* z, float* mills) {
+    int rc = RB_OK;
+    double *d0 = nullptr, *d1 = nullptr, *d2 = nullptr, *d3 = nullptr;
+    float *f0 = nullptr, *f1 = nullptr;
+    const size_t b = size_t(n) * sizeof(double), fb = size_t(n) * sizeof(float);
+    PROBE_TRY(hipMalloc(&d0, b));
+    PROBE_TRY(hipMalloc(&d1, b));
+    PROBE_TRY(hipMalloc(&d2, b));
+    PROBE_TRY(hipMalloc(&d3, b));
+    PROBE_TRY(hipMalloc(&f0, fb));
+    PROBE_TRY(hipMalloc(&f1, fb));
+    PROBE_TRY(hipMemcpy(d0, lnS, b, hipMemcpyHostToDevice));
+    PROBE_TRY(hipMemcpy(d1, sig, b, hipMemcpyHostToDevice));
+    PROBE_TRY(hipMemcpy(d2, K, b, hipMemcpyHostToDevice));
+    PROBE_TRY(hipMemcpy(f0, z, fb, hipMemcpyHostToDevice));
+    PROBE_TRY(launch_probe_payoff(d0, d1, d2, n, d3, f1, f0, nullptr));
+    PROBE_TRY(hipMemcpy(out, d3, b, hipMemcpyDeviceToHost));
+    PROBE_TRY(hipMemcpy(mills, f1, fb, hipMemcpyDeviceToHost));
+done:
+    for (void* p : {(void*)d0, (void*)d1, (void*)d2, (void*)d3, (void*)f0, (void*)f1})
+        if (p) (void)hipFree(p);
+    return rc;
+}
+
 int rb_probe_normals(const uint64_t* seed, int seed_size, long p0, long np, int nstreams, int N,
                      double* out) {
     int rc = RB_OK;

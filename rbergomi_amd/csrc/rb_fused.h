@@ -59,6 +59,7 @@ struct PathPG {
 
 struct FusedArgs {
     int N, Np, NK, PB, TP;       // steps, padded steps (x32), 16-step k-blocks, path blocks, paths per tile
+    int log2TP;                  // log2(TP)
     int nunits, gmax, pgmax, out_size, gchunk, acc_in_lds;
     long M, ntiles, nitems;
     int tiles_per_item;
@@ -74,6 +75,43 @@ struct FusedArgs {
     int wave_nu[4];              // row blocks of each wave, listed in decreasing order
     int wave_rb[4][FUSED_MAX_UPW];
 };
+
+// Mills ratio M(z) = (1 - Phi(z)) / phi(z) for z >= 0, as M = t P(t), t = 1/(1 + z/3), with P a
+// degree-10 polynomial fitted here (weighted least squares at Chebyshev nodes in t) to
+// sqrt(pi/2) erfcx(z/sqrt 2); relative error <= 3.3e-7 in FP32 over z >= 0 (tests compare it
+// with scipy.special.erfcx).  M(z) -> 1/z as z -> infinity; M(inf) = 0.
+__device__ __forceinline__ float mills_f32(float z) {
+    const float c[11] = {3.333333433e-01f, 3.333306611e-01f, 2.964155674e-01f, 2.201747149e-01f, 1.412241310e-01f, -6.350896508e-02f, 2.205852419e-01f, -5.401814580e-01f, 4.626366198e-01f, -1.762671471e-01f, 2.557139099e-02f};
+    const float t = 1.0f / fmaf(z, 0.333333333f, 1.0f);
+    float acc = c[10];
+#pragma unroll
+    for (int i = 9; i >= 0; --i) acc = fmaf(acc, t, c[i]);
+    return acc * t;
+}
+
+// conditioned Black-Scholes payoff of one path for one strike (tau = 1, S0 = S').
+// With E = S' phi(d1) = K phi(d2):  S' Phi(d1) = d1 >= 0 ? S' - E M(d1) : E M(-d1), and the
+// same for K Phi(d2), so one exponential serves both normal CDFs.
+__device__ __forceinline__ double payoff_mixed(const PathPG& q, float lnKf, float Kf, double lnK,
+                                               double K) {
+    const float d1 = (q.lnSf - lnKf) * q.invs + 0.5f * q.sigf;
+    if (d1 < -8.0f) {
+        // deep out of the money (Phi(d1) < 7e-16): the normal CDF in FP64
+        const double e1 = (q.lnS - lnK) / q.sig + 0.5 * q.sig;
+        return exp(q.lnS) * phi_f64(e1) - K * phi_f64(e1 - q.sig);
+    }
+    const float d2 = d1 - q.sigf;
+    const float E = __expf(fmaf(-0.5f * d1, d1, q.lnSf)) * 0.398942280f;
+    const float a1 = mills_f32(fabsf(d1)), a2 = mills_f32(fabsf(d2));
+    float x;
+    if (d2 >= 0.f)
+        x = (q.S - Kf) + E * (a2 - a1);
+    else if (d1 >= 0.f)
+        x = q.S - E * (a1 + a2);
+    else
+        x = E * (a1 - a2);
+    return double(x);
+}
 
 size_t fused_lds_bytes(const FusedArgs& a);
 // mode: 0 = FP16 hi+lo split, 1 = BF16 hi+lo split

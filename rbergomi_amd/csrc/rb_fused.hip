@@ -17,8 +17,8 @@
 //      intRootVdW, :73-80, 163-167, 427-436): FP32 terms summed per 16-row block, FP64 beyond.
 //   4. Per (path, group, rho): ln S', sigma, 1/sigma, S' once; per strike the conditioned
 //      Black-Scholes payoff S' Phi(d1) - K Phi(d2) (:383-391) in FP32, with the FP64 normal
-//      CDF for deep out-of-the-money lanes (d1 < -5).  Sums over paths in FP64: per tile over
-//      the 32 lanes of a half-wave, then per option in the register of its owner lane.
+//      CDF for deep out-of-the-money lanes (d1 < -8).  Each thread owns one option and sums
+//      its payoffs over the paths in FP64 registers (no cross-lane reduction).
 #include <hip/hip_runtime.h>
 
 #include "rb_fused.h"
@@ -59,21 +59,6 @@ __device__ __forceinline__ double half_sum_f64(double v) {
     return v;
 }
 
-__device__ __forceinline__ float phi_f32(float x) { return 0.5f * erfcf(-x * 0.70710678118654752f); }
-
-// conditioned Black-Scholes payoff of one path for one strike (tau = 1, S0 = S')
-__device__ __forceinline__ double payoff_mixed(const PathPG& q, float lnKf, float Kf, double lnK,
-                                               double K) {
-    const float d1 = (q.lnSf - lnKf) * q.invs + 0.5f * q.sigf;
-    if (d1 < -5.0f) {
-        // deep out of the money: the normal CDF in FP64
-        const double e1 = (q.lnS - lnK) / q.sig + 0.5 * q.sig;
-        return exp(q.lnS) * phi_f64(e1) - K * phi_f64(e1 - q.sig);
-    }
-    const float d2 = d1 - q.sigf;
-    return double(q.S * phi_f32(d1) - Kf * phi_f32(d2));
-}
-
 template <int MODE, int UPW>
 __global__ void __launch_bounds__(256, 2) k_fused(FusedArgs A) {
     typedef SplitT<MODE> S;
@@ -83,7 +68,6 @@ __global__ void __launch_bounds__(256, 2) k_fused(FusedArgs A) {
     extern __shared__ uint4 lds4[];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int h = lane >> 5, col = lane & 31;
-    const int hw = 2 * wave + h;
     const int NK = A.NK;
     uint4* Bf = lds4;                                                  // [PB][NK][2][64]
     double* wpart = reinterpret_cast<double*>(Bf + A.PB * NK * 128);  // [4][gchunk][32][2]
@@ -91,7 +75,7 @@ __global__ void __launch_bounds__(256, 2) k_fused(FusedArgs A) {
     double* accopt = reinterpret_cast<double*>(pgd + A.TP * A.pgmax);        // [out][2] if acc_in_lds
     const int pbw = A.wave_pb[wave];
     const int nuw = A.wave_nu[wave];
-    double a1 = 0.0, a2 = 0.0;  // sums of fused option 8*col + hw (register accumulation)
+    double a1 = 0.0, a2 = 0.0;  // sums of fused option tid (register accumulation)
 
     for (long item = blockIdx.x; item < A.nitems; item += gridDim.x) {
         if (A.acc_in_lds)
@@ -102,27 +86,31 @@ __global__ void __launch_bounds__(256, 2) k_fused(FusedArgs A) {
             const long p0 = tile * A.TP;
             // ---- 1. Gaussians W1 -> B fragments (hi, lo); zero past N and past M
             const int nq = A.Np >> 2;
-            for (int e = tid; e < A.TP * nq; e += 256) {
-                const int pl = e / nq, q = e - pl * nq;
+            {
+                const int pl = tid & (A.TP - 1);   // TP is 32, 64 or 128
+                const int pb = pl >> 5;
                 const long p = p0 + pl;
-                float z[4] = {0.f, 0.f, 0.f, 0.f};
-                if (p < A.M && 4 * q < A.N) {
-                    normals4_f32(A.key, uint64_t(p), 0u, uint32_t(q), z);
+                const bool live = p < A.M;
+                for (int q = tid >> A.log2TP; q < nq; q += 256 >> A.log2TP) {
+                    float z[4] = {0.f, 0.f, 0.f, 0.f};
+                    if (live && 4 * q < A.N) {
+                        normals4_f32(A.key, uint64_t(p), 0u, uint32_t(q), z);
 #pragma unroll
-                    for (int j = 0; j < 4; ++j)
-                        if (4 * q + j >= A.N) z[j] = 0.f;
-                }
-                V4 hi, lo;
+                        for (int j = 0; j < 4; ++j)
+                            if (4 * q + j >= A.N) z[j] = 0.f;
+                    }
+                    V4 hi, lo;
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    hi[j] = S::cvt(z[j]);
-                    lo[j] = S::cvt(z[j] - float(hi[j]));
+                    for (int j = 0; j < 4; ++j) {
+                        hi[j] = S::cvt(z[j]);
+                        lo[j] = S::cvt(z[j] - float(hi[j]));
+                    }
+                    const int ks = q >> 2, hh = (q & 3) >> 1, j0 = (q & 1);
+                    const int ln = 32 * hh + (pl & 31);
+                    uint4* base = Bf + ((pb * NK + ks) * 2) * 64;
+                    reinterpret_cast<uint2*>(base + ln)[j0] = __builtin_bit_cast(uint2, hi);
+                    reinterpret_cast<uint2*>(base + 64 + ln)[j0] = __builtin_bit_cast(uint2, lo);
                 }
-                const int ks = q >> 2, hh = (q & 3) >> 1, j0 = (q & 1);
-                const int ln = 32 * hh + (pl & 31), pb = pl >> 5;
-                uint4* base = Bf + ((pb * NK + ks) * 2) * 64;
-                reinterpret_cast<uint2*>(base + ln)[j0] = __builtin_bit_cast(uint2, hi);
-                reinterpret_cast<uint2*>(base + 64 + ln)[j0] = __builtin_bit_cast(uint2, lo);
             }
             __syncthreads();
 
@@ -248,7 +236,7 @@ __global__ void __launch_bounds__(256, 2) k_fused(FusedArgs A) {
                         }
                         __syncthreads();
                         for (int e = tid; e < A.TP * cn; e += 256) {
-                            const int pl = e % A.TP, gg = e / A.TP;
+                            const int pl = e & (A.TP - 1), gg = e >> A.log2TP;
                             const int pb = pl >> 5, c = pl & 31;
                             const FusedGroup G = A.groups[g0 + c0 + gg];
                             const uint4* B0 = Bf + ((pb * NK) * 2) * 64 + c;  // k = 0
@@ -276,29 +264,25 @@ __global__ void __launch_bounds__(256, 2) k_fused(FusedArgs A) {
                         __syncthreads();
                     }
                 }
-                // ---- 4. payoffs; half-wave hw owns the fused options oi = hw (mod 8)
-                const int ostart = U.o_begin + (((hw - U.o_begin) % 8) + 8) % 8;
-                for (int oi = ostart; oi < U.o_end; oi += 8) {
+                // ---- 4. payoffs: thread t owns fused option t (t + 256 r in LDS mode) and loops
+                //         over the tile's paths; no cross-lane reduction
+                const int ofirst = A.acc_in_lds ? U.o_begin + tid : (tid >= U.o_begin ? tid : U.o_end);
+                const int ostep = A.acc_in_lds ? 256 : (1 << 30);
+                for (int oi = ofirst; oi < U.o_end; oi += ostep) {
                     const FusedOption O = A.opts[oi];
                     const int pgl = O.pg - U.pg_begin;
                     const float lnKf = float(O.lnK), Kf = float(O.K);
                     double s1 = 0.0, s2 = 0.0;
-                    for (int pb = 0; pb < A.PB; ++pb) {
-                        const int pl = 32 * pb + col;
-                        if (p0 + pl < A.M) {
-                            const double x = payoff_mixed(pgd[pl * A.pgmax + pgl], lnKf, Kf, O.lnK, O.K);
-                            s1 += x;
-                            s2 += x * x;
-                        }
+                    const int np = A.M - p0 < A.TP ? int(A.M - p0) : A.TP;
+                    for (int pl = 0; pl < np; ++pl) {
+                        const double x = payoff_mixed(pgd[pl * A.pgmax + pgl], lnKf, Kf, O.lnK, O.K);
+                        s1 += x;
+                        s2 += x * x;
                     }
-                    s1 = half_sum_f64(s1);
-                    s2 = half_sum_f64(s2);
                     if (A.acc_in_lds) {
-                        if (col == 0) {
-                            accopt[2 * oi + 0] += s1;
-                            accopt[2 * oi + 1] += s2;
-                        }
-                    } else if (col == (oi >> 3)) {
+                        accopt[2 * oi + 0] += s1;
+                        accopt[2 * oi + 1] += s2;
+                    } else {
                         a1 += s1;
                         a2 += s2;
                     }
@@ -315,7 +299,7 @@ __global__ void __launch_bounds__(256, 2) k_fused(FusedArgs A) {
             }
             __syncthreads();
         } else {
-            const int oi = 8 * col + hw;
+            const int oi = tid;
             if (oi < A.out_size) {
                 const int I = A.opts[oi].out;
                 A.partial[(long(I) * A.nitems + item) * 2 + 0] = a1;

@@ -77,6 +77,7 @@ int Plan::init_fused(std::string& err) {
     const int NB = Np / 32;
     a.PB = NB >= 3 ? 1 : (NB == 2 ? 2 : 4);
     a.TP = 32 * a.PB;
+    a.log2TP = a.PB == 1 ? 5 : (a.PB == 2 ? 6 : 7);
     // ---- waves: output row blocks (cost rb+1), all row blocks of a wave in one path block
     std::vector<std::vector<int>> wrb(4);
     for (int w = 0; w < 4; ++w) a.wave_pb[w] = -1;

@@ -30,4 +30,7 @@ hipError_t launch_philox_probe(const void* ctr, PhiloxKey key, long n, void* out
 hipError_t launch_probe_mfma_layout(const float* A, const float* B, float* out, int bf16,
                                     hipStream_t st);
 
+hipError_t launch_probe_payoff(const double* lnS, const double* sig, const double* K, int n,
+                               double* out, float* mills_out, const float* z, hipStream_t st);
+
 }  // namespace rb

@@ -1,6 +1,7 @@
 // rb_probe.hip -- device probe of the MFMA operand/output layout.
 #include <hip/hip_runtime.h>
 
+#include "rb_fused.h"
 #include "rb_kernels.h"
 #include "rb_mfma.h"
 
@@ -36,6 +37,30 @@ __global__ void k_probe_mfma_layout(const float* __restrict__ A, const float* __
 hipError_t launch_probe_mfma_layout(const float* A, const float* B, float* out, int bf16,
                                     hipStream_t st) {
     hipLaunchKernelGGL(k_probe_mfma_layout, dim3(1), dim3(64), 0, st, A, B, out, bf16);
+    return hipGetLastError();
+}
+
+// Mixed-precision payoff of the fused kernel for given (ln S', sigma, K): out[i]
+__global__ void k_probe_payoff(const double* __restrict__ lnS, const double* __restrict__ sig,
+                               const double* __restrict__ K, int n, double* __restrict__ out,
+                               float* __restrict__ mills_out, const float* __restrict__ z) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    PathPG q;
+    q.lnS = lnS[i];
+    q.sig = sig[i];
+    q.S = float(exp(q.lnS));
+    q.invs = float(1.0 / q.sig);
+    q.lnSf = float(q.lnS);
+    q.sigf = float(q.sig);
+    out[i] = payoff_mixed(q, float(log(K[i])), float(K[i]), log(K[i]), K[i]);
+    mills_out[i] = mills_f32(z[i]);
+}
+
+hipError_t launch_probe_payoff(const double* lnS, const double* sig, const double* K, int n,
+                               double* out, float* mills_out, const float* z, hipStream_t st) {
+    hipLaunchKernelGGL(k_probe_payoff, dim3((n + 255) / 256), dim3(256), 0, st, lnS, sig, K, n, out,
+                       mills_out, z);
     return hipGetLastError();
 }
 

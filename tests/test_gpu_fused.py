@@ -95,3 +95,35 @@ def test_fused_limits_are_reported(rb):
         rb.price_v2(0.04, [0.1], [1.5], [-0.7], [1.0], [1.0], 64, 100, [1], precision=rb.PREC_SPLIT_FP16,
                     method=rb.METHOD_MC)
     assert e.value.code == 1
+
+
+def test_mixed_payoff_and_mills_ratio(rb):
+    """The split-precision payoff (FP32 with one shared Gaussian factor and a fitted Mills
+    ratio, FP64 normal CDF below d1 = -5) against the FP64 Black-Scholes formula, and the
+    Mills-ratio fit against scipy.special.erfcx."""
+    from scipy.special import erfcx
+    from scipy.stats import norm
+    rng = np.random.default_rng(7)
+    n = 20000
+    lnS = rng.normal(-0.02, 0.3, n)
+    sig = np.exp(rng.uniform(np.log(0.005), np.log(1.5), n))
+    K = np.exp(rng.uniform(-0.6, 0.4, n))
+    z = np.concatenate([np.linspace(0, 12, n // 2), np.exp(rng.uniform(-8, 8, n - n // 2))]).astype(np.float32)
+    out = np.zeros(n)
+    mills = np.zeros(n, dtype=np.float32)
+    fp = ctypes.POINTER(ctypes.c_float)
+    dp = ctypes.POINTER(ctypes.c_double)
+    assert rb.load().rb_probe_payoff(lnS.ctypes.data_as(dp), sig.ctypes.data_as(dp), K.ctypes.data_as(dp), n,
+                                     out.ctypes.data_as(dp), z.ctypes.data_as(fp), mills.ctypes.data_as(fp)) == 0
+    zz = z.astype(np.float64)
+    ref_m = erfcx(zz / np.sqrt(2)) * np.sqrt(np.pi / 2)
+    assert np.max(np.abs(mills / ref_m - 1)) < 5e-7
+    S = np.exp(lnS)
+    d1 = (lnS - np.log(K)) / sig + 0.5 * sig
+    ref = S * norm.cdf(d1) - K * norm.cdf(d1 - sig)
+    err = np.abs(out - ref)
+    # FP32 inputs: |d(ln S)| ~ 6e-8 |ln S| propagates through delta ~ S Phi(d1)
+    tol = 2e-6 * np.maximum(S, K) * np.maximum(1e-3, norm.pdf(d1) / sig) + 1e-7 * np.maximum(S, K)
+    assert np.all(err <= tol + 1e-300), np.max(err / tol)
+    deep = d1 < -8
+    assert deep.any() and np.all(np.abs(out[deep] - ref[deep]) <= 1e-8 * ref[deep] + 1e-300)
