@@ -64,11 +64,14 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--precision", default="fp64", choices=sorted(PREC_CODE))
+    ap.add_argument("--precision", default="fp16", choices=sorted(PREC_CODE))
     ap.add_argument("--paths", type=int, default=2 ** 22)
     ap.add_argument("--cpu-sample-paths", type=int, default=2 ** 18)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-check", action="store_true", help="skip the FP64 comparison run")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="also run the streaming copy probe over this many MiB (traffic calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -134,6 +137,24 @@ def main():
         "roofline": roof,
         "check": {"price_atm_T1": float(price[3 * 41 + 27]), "stat_atm_T1": float(stat[3 * 41 + 27])},
     }
+    if not args.no_check and args.precision != "fp64":
+        # same Philox counters through the FP64 path: the price error of the mixed precision
+        ref = rb.Plan(w["xi"], w["H"], w["eta"], w["rho"], w["T"], w["K"], w["N"], w["M"],
+                      [20240601, 7], precision=PREC_CODE["fp64"])
+        ref.run()
+        p64, iv64, s64 = ref.results()
+        ref.close()
+        out["accuracy"] = {"max_abs_price_err_vs_fp64": float(np.max(np.abs(price - p64))),
+                           "max_abs_stat_err_vs_fp64": float(np.max(np.abs(stat - s64))),
+                           "max_price_stat_ratio": float(np.max(np.abs(price - p64) / s64)),
+                           "reference": "staged FP64 path, same Philox counters"}
+    if args.copy_probe:
+        import ctypes
+        ms_copy = ctypes.c_double()
+        nbytes = args.copy_probe << 20
+        assert rb.load().rb_probe_copy(nbytes, 5, ctypes.byref(ms_copy)) == 0
+        out["copy_probe"] = {"bytes": nbytes, "avg_ms": ms_copy.value,
+                             "GB_per_s": 2 * nbytes / (ms_copy.value * 1e-3) / 1e9}
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
         out["cpu_baseline"] = cpu_baseline(args.cpu_sample_paths, threads)

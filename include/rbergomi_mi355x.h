@@ -142,6 +142,9 @@ int rb_probe_wtilde(const double* Z, long M, int N, double H, double* Wtilde);
 /* One 32x32x16 MFMA (bf16 = 0: FP16 operands, 1: BF16) of A[32][16] by B[16][32] built with
  * the operand layout of the fused kernel; out[lane*16 + r] = accumulator register r of lane. */
 int rb_probe_mfma_layout(const float* A, const float* B, int bf16, float* out);
+/* Device-to-device copy of `bytes` (multiple of 16) by a plain streaming kernel, reps times;
+ * *avg_ms receives the mean kernel time (HIP events).  Used to calibrate HBM traffic counters. */
+int rb_probe_copy(long bytes, int reps, double* avg_ms);
 /* The split-precision kernel's per-path payoff for given (ln S', sigma, K) -> out[i], and its
  * Mills-ratio approximation at z[i] -> mills[i] (n values). */
 int rb_probe_payoff(const double* lnS, const double* sig, const double* K, int n, double* out,

@@ -72,6 +72,7 @@ def load():
     fp = ctypes.POINTER(ctypes.c_float)
     L.rb_probe_mfma_layout.argtypes = [fp, fp, i, fp]
     L.rb_probe_payoff.argtypes = [dp, dp, dp, i, dp, fp, fp]
+    L.rb_probe_copy.argtypes = [l, i, dp]
     L.rb_probe_integrals.argtypes = [d, dp, dp, dp, i, dp, dp, i, i, l, dp, cfgp, dp]
     _lib = L
     return _lib

@@ -334,6 +334,32 @@ done:
     return rc;
 }
 
+int rb_probe_copy(long bytes, int reps, double* avg_ms) {
+    int rc = RB_OK;
+    void *a = nullptr, *b = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    float ms = 0.f;
+    if (bytes < 16 || bytes % 16 || reps < 1) return fail(RB_ERR_ARG, "bytes must be a positive multiple of 16");
+    PROBE_TRY(hipMalloc(&a, size_t(bytes)));
+    PROBE_TRY(hipMalloc(&b, size_t(bytes)));
+    PROBE_TRY(hipMemset(a, 1, size_t(bytes)));
+    PROBE_TRY(hipEventCreate(&e0));
+    PROBE_TRY(hipEventCreate(&e1));
+    PROBE_TRY(launch_probe_copy(a, b, bytes, nullptr));
+    PROBE_TRY(hipEventRecord(e0, nullptr));
+    for (int r = 0; r < reps; ++r) PROBE_TRY(launch_probe_copy(a, b, bytes, nullptr));
+    PROBE_TRY(hipEventRecord(e1, nullptr));
+    PROBE_TRY(hipEventSynchronize(e1));
+    PROBE_TRY(hipEventElapsedTime(&ms, e0, e1));
+    if (avg_ms) *avg_ms = double(ms) / reps;
+done:
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    return rc;
+}
+
 int rb_probe_payoff(const double* lnS, const double* sig, const double* K, int n, double* out,
                     const float* z, float* mills) {
     int rc = RB_OK;

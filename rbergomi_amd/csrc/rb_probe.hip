@@ -64,4 +64,18 @@ hipError_t launch_probe_payoff(const double* lnS, const double* sig, const doubl
     return hipGetLastError();
 }
 
+// Plain streaming copy (16-byte vectors): the known-size reference kernel for the HBM
+// traffic counters, and a measure of the achievable copy bandwidth.
+__global__ void k_probe_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, long n16) {
+    for (long i = blockIdx.x * long(blockDim.x) + threadIdx.x; i < n16; i += long(gridDim.x) * blockDim.x)
+        dst[i] = src[i];
+}
+
+hipError_t launch_probe_copy(const void* src, void* dst, long bytes, hipStream_t st) {
+    const long n16 = bytes / 16;
+    hipLaunchKernelGGL(k_probe_copy, dim3(4096), dim3(256), 0, st, static_cast<const uint4*>(src),
+                       static_cast<uint4*>(dst), n16);
+    return hipGetLastError();
+}
+
 }  // namespace rb
