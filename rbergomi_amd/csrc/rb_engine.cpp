@@ -197,8 +197,9 @@ static hipError_t upload(T** p, const std::vector<T>& v, hipStream_t st) {
 
 Plan::~Plan() {
     if (stream) (void)hipStreamSynchronize(stream);
-    void* bufs[] = {d_units, d_gamma, d_groups, d_comp, d_opts,  d_Z,  d_Wt,   d_ints, d_partial,
-                    d_price, d_stat,  d_iv,     d_payoff, d_fu, d_fg, d_fpg, d_fo, d_frag, d_ptab};
+    fused_batch.reset();
+    void* bufs[] = {d_units, d_gamma, d_groups, d_comp, d_opts, d_Z,   d_Wt,
+                    d_ints,  d_partial, d_price, d_stat, d_iv, d_payoff};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
@@ -373,6 +374,12 @@ int Plan::run_samples(const double* Z_host, double* payoff_host, std::string& er
 }
 
 int Plan::results(double* price, double* iv, double* stat, std::string& err) {
+    if (fused) {
+        const int rc = fused_batch->results(0, price, iv, stat, stream, err);
+        if (rc) return rc;
+        RB_TRY(hipStreamSynchronize(stream));
+        return ST_OK;
+    }
     const size_t n = lay.options.size() * sizeof(double);
     if (price) RB_TRY(hipMemcpyAsync(price, d_price, n, hipMemcpyDeviceToHost, stream));
     if (iv) RB_TRY(hipMemcpyAsync(iv, d_iv, n, hipMemcpyDeviceToHost, stream));

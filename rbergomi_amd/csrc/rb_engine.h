@@ -3,11 +3,13 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <memory>
 #include <string>
 #include <vector>
 
 #include "rb_device.h"
 #include "rb_fused.h"
+#include "rb_fused_host.h"
 #include "rb_types.h"
 
 namespace rb {
@@ -84,18 +86,11 @@ struct Plan {
     double* d_stat = nullptr;
     double* d_iv = nullptr;
     double* d_payoff = nullptr;   // [chunk][options], quadrature mode only
-    // fused mixed-precision path
+    // fused mixed-precision path (a batch of one request)
     bool fused = false;
     int fused_mode = 0;           // 0 = FP16 split, 1 = BF16 split
-    int fused_upw = 1;
-    int fused_grid = 0;
-    FusedArgs fa{};
-    FusedUnit* d_fu = nullptr;
-    FusedGroup* d_fg = nullptr;
-    FusedOption* d_fo = nullptr;
-    FusedPG* d_fpg = nullptr;
-    uint4* d_frag = nullptr;
-    float* d_ptab = nullptr;
+    std::unique_ptr<FusedHostReq> fused_req;
+    std::unique_ptr<FusedBatch> fused_batch;
     std::vector<KernelTime> times;
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;

@@ -57,12 +57,14 @@ struct PathPG {
     float sigf;
 };
 
-struct FusedArgs {
+// One request of a batch: its tables, sizes and the split of its output row blocks over the
+// four waves of a workgroup.
+struct FusedReq {
     int N, Np, NK, PB, TP;       // steps, padded steps (x32), 16-step k-blocks, path blocks, paths per tile
     int log2TP;                  // log2(TP)
-    int nunits, gmax, pgmax, out_size, gchunk, acc_in_lds;
-    long M, ntiles, nitems;
+    int nunits, pgmax, out_size, gchunk, acc_in_lds;
     int tiles_per_item;
+    long M, ntiles, nitems;
     PhiloxKey key;
     const FusedUnit* units;
     const FusedGroup* groups;
@@ -74,6 +76,20 @@ struct FusedArgs {
     int wave_pb[4];
     int wave_nu[4];              // row blocks of each wave, listed in decreasing order
     int wave_rb[4][FUSED_MAX_UPW];
+};
+
+// A launch: the requests, the work items (one item = tiles_per_item tiles of one request, handed
+// out through an atomic counter) and the LDS layout sized for the largest request.
+struct FusedBatchArgs {
+    const FusedReq* reqs;
+    const int* item_req;         // request of each work item
+    const int* item_local;       // item index within its request
+    long total_items;
+    int* counter;                // next work item (zeroed before the launch)
+    int bf_u4;                   // LDS: B fragments, in 16-byte units
+    int wpart_d;                 // LDS: cross-wave partials, in doubles
+    int pgd_n;                   // LDS: per-(path, payoff group) records
+    int acc_d;                   // LDS: option sums (largest request in LDS mode), in doubles
 };
 
 // Mills ratio M(z) = (1 - Phi(z)) / phi(z) for z >= 0, as M = t P(t), t = 1/(1 + z/3), with P a
@@ -113,8 +129,10 @@ __device__ __forceinline__ double payoff_mixed(const PathPG& q, float lnKf, floa
     return double(x);
 }
 
-size_t fused_lds_bytes(const FusedArgs& a);
+size_t fused_req_lds_bytes(const FusedReq& a);
+size_t fused_lds_bytes(const FusedBatchArgs& b);
 // mode: 0 = FP16 hi+lo split, 1 = BF16 hi+lo split
-hipError_t launch_fused(int mode, int upw, const FusedArgs& a, int grid, hipStream_t st);
+hipError_t launch_fused(int mode, int upw, const FusedBatchArgs& b, int grid, hipStream_t st);
+hipError_t launch_zero_int(int* p, int n, hipStream_t st);
 
 }  // namespace rb

@@ -60,24 +60,33 @@ __device__ __forceinline__ double half_sum_f64(double v) {
 }
 
 template <int MODE, int UPW>
-__global__ void __launch_bounds__(256, 2) k_fused(FusedArgs A) {
+__global__ void __launch_bounds__(256, 2) k_fused(FusedBatchArgs B) {
     typedef SplitT<MODE> S;
     typedef typename S::T T;
     typedef typename S::V V;
     typedef typename S::V4 V4;
     extern __shared__ uint4 lds4[];
+    __shared__ int s_item;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int h = lane >> 5, col = lane & 31;
-    const int NK = A.NK;
-    uint4* Bf = lds4;                                                  // [PB][NK][2][64]
-    double* wpart = reinterpret_cast<double*>(Bf + A.PB * NK * 128);  // [4][gchunk][32][2]
-    PathPG* pgd = reinterpret_cast<PathPG*>(wpart + 4 * A.gchunk * 32 * 2);  // [TP][pgmax]
-    double* accopt = reinterpret_cast<double*>(pgd + A.TP * A.pgmax);        // [out][2] if acc_in_lds
-    const int pbw = A.wave_pb[wave];
-    const int nuw = A.wave_nu[wave];
-    double a1 = 0.0, a2 = 0.0;  // sums of fused option tid (register accumulation)
+    uint4* Bf = lds4;                                                   // [PB][NK][2][64]
+    double* wpart = reinterpret_cast<double*>(lds4 + B.bf_u4);          // [4][gchunk][32][2]
+    PathPG* pgd = reinterpret_cast<PathPG*>(wpart + B.wpart_d);         // [TP][pgmax]
+    double* accopt = reinterpret_cast<double*>(pgd + B.pgd_n);          // [out][2] in LDS mode
 
-    for (long item = blockIdx.x; item < A.nitems; item += gridDim.x) {
+    for (;;) {
+        // ---- next work item (dynamic schedule; the item's sums do not depend on who runs it)
+        if (tid == 0) s_item = atomicAdd(B.counter, 1);
+        __syncthreads();
+        const long gi = s_item;
+        __syncthreads();
+        if (gi >= B.total_items) break;
+        const FusedReq& A = B.reqs[B.item_req[gi]];
+        const long item = B.item_local[gi];
+        const int NK = A.NK;
+        const int pbw = A.wave_pb[wave];
+        const int nuw = A.wave_nu[wave];
+        double a1 = 0.0, a2 = 0.0;  // sums of option tid (register accumulation)
         if (A.acc_in_lds)
             for (int o = tid; o < 2 * A.out_size; o += 256) accopt[o] = 0.0;
         const long t_begin = item * A.tiles_per_item;
@@ -297,26 +306,37 @@ __global__ void __launch_bounds__(256, 2) k_fused(FusedArgs A) {
                 A.partial[(long(I) * A.nitems + item) * 2 + 0] = accopt[2 * o + 0];
                 A.partial[(long(I) * A.nitems + item) * 2 + 1] = accopt[2 * o + 1];
             }
-            __syncthreads();
-        } else {
-            const int oi = tid;
-            if (oi < A.out_size) {
-                const int I = A.opts[oi].out;
-                A.partial[(long(I) * A.nitems + item) * 2 + 0] = a1;
-                A.partial[(long(I) * A.nitems + item) * 2 + 1] = a2;
-            }
-            a1 = a2 = 0.0;
+        } else if (tid < A.out_size) {
+            const int I = A.opts[tid].out;
+            A.partial[(long(I) * A.nitems + item) * 2 + 0] = a1;
+            A.partial[(long(I) * A.nitems + item) * 2 + 1] = a2;
         }
+        __syncthreads();
     }
 }
 
-size_t fused_lds_bytes(const FusedArgs& a) {
+size_t fused_req_lds_bytes(const FusedReq& a) {
     return size_t(a.PB) * a.NK * 128 * 16 + size_t(4) * a.gchunk * 32 * 2 * 8 +
            size_t(a.TP) * a.pgmax * sizeof(PathPG) + (a.acc_in_lds ? size_t(a.out_size) * 2 * 8 : 0);
 }
 
+size_t fused_lds_bytes(const FusedBatchArgs& b) {
+    return size_t(b.bf_u4) * 16 + size_t(b.wpart_d) * 8 + size_t(b.pgd_n) * sizeof(PathPG) +
+           size_t(b.acc_d) * 8;
+}
+
+__global__ void k_zero_int(int* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+hipError_t launch_zero_int(int* p, int n, hipStream_t st) {
+    hipLaunchKernelGGL(k_zero_int, dim3((n + 255) / 256), dim3(256), 0, st, p, n);
+    return hipGetLastError();
+}
+
 template <int MODE, int UPW>
-static hipError_t launch_t(const FusedArgs& a, int grid, hipStream_t st) {
+static hipError_t launch_t(const FusedBatchArgs& a, int grid, hipStream_t st) {
     const size_t lds = fused_lds_bytes(a);
     if (lds > 65536) {
         const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fused<MODE, UPW>),
@@ -328,7 +348,7 @@ static hipError_t launch_t(const FusedArgs& a, int grid, hipStream_t st) {
 }
 
 template <int MODE>
-static hipError_t launch_m(int upw, const FusedArgs& a, int grid, hipStream_t st) {
+static hipError_t launch_m(int upw, const FusedBatchArgs& a, int grid, hipStream_t st) {
     switch (upw) {
         case 1: return launch_t<MODE, 1>(a, grid, st);
         case 2: return launch_t<MODE, 2>(a, grid, st);
@@ -338,7 +358,7 @@ static hipError_t launch_m(int upw, const FusedArgs& a, int grid, hipStream_t st
     }
 }
 
-hipError_t launch_fused(int mode, int upw, const FusedArgs& a, int grid, hipStream_t st) {
+hipError_t launch_fused(int mode, int upw, const FusedBatchArgs& a, int grid, hipStream_t st) {
     return mode == 0 ? launch_m<0>(upw, a, grid, st) : launch_m<1>(upw, a, grid, st);
 }
 

@@ -1,16 +1,24 @@
-// rb_fused_host.cpp -- host planning of the fused mixed-precision kernel.
+// rb_fused_host.cpp -- host planning and launching of the fused mixed-precision kernel.
 //
-// Builds, from the deduplicated layout: the per-H tables of Toeplitz A fragments (hi + lo),
-// the per-group exp2 coefficient tables, the option lists per H and the assignment of
-// 32-row output blocks to the four waves of a workgroup (balanced triangular work).
+// plan_fused_request builds, from one request's deduplicated layout: the per-H tables of
+// Toeplitz A fragments (hi + lo), the per-H tables P_j = (j - off)^(2H), the group exponent
+// coefficients, the payoff groups (group, rho), the option list and the assignment of 32-row
+// output blocks to the four waves of a workgroup (balanced triangular work).
+// FusedBatch concatenates any number of such requests into one launch: one table of requests,
+// one list of work items (1024 paths of one request each) ordered by decreasing cost and handed
+// out through an atomic counter, and one ordered reduction of every request's option sums.
+// A request's results depend only on its own items, so they are bit-identical alone and in any
+// batch.
+#include "rb_fused_host.h"
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <numeric>
 #include <string>
 #include <vector>
 
 #include "rb_engine.h"
-#include "rb_fused.h"
 #include "rb_kernels.h"
 
 namespace rb {
@@ -42,23 +50,8 @@ static void split16(double x, int mode, uint16_t& hi, uint16_t& lo) {
     }
 }
 
-#define RB_TRY(expr)                                                         \
-    do {                                                                     \
-        hipError_t e_ = (expr);                                              \
-        if (e_ != hipSuccess) {                                              \
-            err = std::string(#expr) + ": " + hipGetErrorString(e_);         \
-            return e_ == hipErrorOutOfMemory ? ST_ERR_NOMEM : ST_ERR_DEVICE; \
-        }                                                                    \
-    } while (0)
-
-template <class T>
-static hipError_t upload_vec(T** p, const std::vector<T>& v, hipStream_t st) {
-    hipError_t e = hipMalloc(reinterpret_cast<void**>(p), std::max<size_t>(v.size(), 1) * sizeof(T));
-    if (e != hipSuccess || v.empty()) return e;
-    return hipMemcpyAsync(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st);
-}
-
-int Plan::init_fused(std::string& err) {
+int plan_fused_request(const Request& req, const Layout& lay, int mode, FusedHostReq& out,
+                       std::string& err) {
     const int N = req.N;
     if (req.method != METHOD_RT) {
         err = "the split-precision kernel prices the Romano-Touzi estimator only";
@@ -69,7 +62,7 @@ int Plan::init_fused(std::string& err) {
         err = "the split-precision kernel covers N <= 1024";
         return ST_ERR_LIMIT;
     }
-    FusedArgs& a = fa;
+    FusedReq& a = out.r;
     std::memset(&a, 0, sizeof(a));
     a.N = N;
     a.Np = Np;
@@ -98,7 +91,7 @@ int Plan::init_fused(std::string& err) {
     }
     size_t upw = 1;
     for (auto& v : wrb) upw = std::max(upw, v.size());
-    fused_upw = upw <= 1 ? 1 : upw <= 2 ? 2 : upw <= 4 ? 4 : 8;
+    out.upw = upw <= 1 ? 1 : upw <= 2 ? 2 : upw <= 4 ? 4 : 8;
     for (int w = 0; w < 4; ++w) {
         // row blocks of a wave in decreasing order (the GEMM phases rely on it)
         std::sort(wrb[w].begin(), wrb[w].end(), [](int x, int y) { return x > y; });
@@ -107,25 +100,25 @@ int Plan::init_fused(std::string& err) {
     }
     // ---- units, groups (sorted by unit), payoff groups (group, rho), options (sorted by unit)
     const int nu = int(lay.units.size());
-    std::vector<FusedUnit> fu(nu);
-    std::vector<FusedGroup> fg;
-    std::vector<FusedPG> fpg;
-    std::vector<FusedOption> fo;
-    std::vector<float> ptab(size_t(nu) * Np, 0.f);
+    out.fu.assign(nu, FusedUnit());
+    out.fg.clear();
+    out.fpg.clear();
+    out.fo.clear();
+    out.ptab.assign(size_t(nu) * Np, 0.f);
     const double log2e = 1.4426950408889634;
     const double off = req.comp == COMP_EXACT ? 0.0 : 1.0;
-    int gmax = 1, pgmax = 1;
+    int pgmax = 1, gmax = 1;
     for (int u = 0; u < nu; ++u) {
-        FusedUnit& U = fu[u];
+        FusedUnit& U = out.fu[u];
         const double H = lay.units[u].H;
         U.b = float(lay.units[u].b);
         U.sc = float(lay.units[u].sc);
         U.frag_off = u * a.NK * 128;
         U.p_off = u * Np;
-        for (int j = 1; j < N; ++j) ptab[size_t(u) * Np + j] = float(std::pow(double(j) - off, 2.0 * H));
-        U.g_begin = int(fg.size());
-        U.pg_begin = int(fpg.size());
-        U.o_begin = int(fo.size());
+        for (int j = 1; j < N; ++j) out.ptab[size_t(u) * Np + j] = float(std::pow(double(j) - off, 2.0 * H));
+        U.g_begin = int(out.fg.size());
+        U.pg_begin = int(out.fpg.size());
+        U.o_begin = int(out.fo.size());
         for (size_t g = 0; g < lay.groups.size(); ++g) {
             const Group& G = lay.groups[g];
             if (G.unit != u) continue;
@@ -138,32 +131,31 @@ int Plan::init_fused(std::string& err) {
             F.sxi = std::sqrt(G.xi);
             F.dt = G.dt;
             F.sdt = G.sdt;
-            F.pg_begin = int(fpg.size());
-            const int fgi = int(fg.size());
-            // payoff groups: distinct rho among this group's options, then its options
+            F.pg_begin = int(out.fpg.size());
+            const int fgi = int(out.fg.size());
             for (size_t I = 0; I < lay.options.size(); ++I) {
                 const Option& o = lay.options[I];
                 if (o.group != int(g)) continue;
                 int pg = -1;
-                for (int q = F.pg_begin; q < int(fpg.size()); ++q)
-                    if (fpg[q].rho == o.rho) pg = q;
+                for (int q = F.pg_begin; q < int(out.fpg.size()); ++q)
+                    if (out.fpg[q].rho == o.rho) pg = q;
                 if (pg < 0) {
-                    pg = int(fpg.size());
-                    fpg.push_back(FusedPG{fgi, 0, o.rho});
+                    pg = int(out.fpg.size());
+                    out.fpg.push_back(FusedPG{fgi, 0, o.rho});
                 }
-                fo.push_back(FusedOption{pg, int(I), o.K, o.lnK});
+                out.fo.push_back(FusedOption{pg, int(I), o.K, o.lnK});
             }
-            F.pg_end = int(fpg.size());
-            fg.push_back(F);
+            F.pg_end = int(out.fpg.size());
+            out.fg.push_back(F);
         }
-        U.g_end = int(fg.size());
-        U.pg_end = int(fpg.size());
-        U.o_end = int(fo.size());
-        gmax = std::max(gmax, U.g_end - U.g_begin);
+        U.g_end = int(out.fg.size());
+        U.pg_end = int(out.fpg.size());
+        U.o_end = int(out.fo.size());
         pgmax = std::max(pgmax, U.pg_end - U.pg_begin);
+        gmax = std::max(gmax, U.g_end - U.g_begin);
     }
     // ---- A fragments: entry (d, lane, slot j) = G[d + m - 8h - j], G[0] = a, G[t>0] = Gamma_t
-    std::vector<uint4> frag(size_t(nu) * a.NK * 128);
+    out.frag.assign(size_t(nu) * a.NK * 128, uint4());
     for (int u = 0; u < nu; ++u) {
         const double* gam = &lay.gamma[size_t(u) * N];
         const double diag = lay.units[u].a;
@@ -175,16 +167,15 @@ int Plan::init_fused(std::string& err) {
                 for (int j = 0; j < 8; ++j) {
                     const int t = d + m - 8 * hh - j;
                     const double g = t < 0 ? 0.0 : (t == 0 ? diag : (t < N ? gam[t] : 0.0));
-                    split16(g, fused_mode, hi[j], lo[j]);
+                    split16(g, mode, hi[j], lo[j]);
                 }
-                uint4* e = &frag[(size_t(u) * a.NK + f) * 128];
+                uint4* e = &out.frag[(size_t(u) * a.NK + f) * 128];
                 std::memcpy(&e[lane], hi, 16);
                 std::memcpy(&e[64 + lane], lo, 16);
             }
         }
     }
     a.nunits = nu;
-    a.gmax = gmax;
     a.pgmax = pgmax;
     a.out_size = int(lay.options.size());
     a.acc_in_lds = a.out_size > FUSED_MAX_REG_OPTS ? 1 : 0;
@@ -193,67 +184,236 @@ int Plan::init_fused(std::string& err) {
     a.tiles_per_item = std::max(1, 1024 / a.TP);
     a.nitems = (a.ntiles + a.tiles_per_item - 1) / a.tiles_per_item;
     a.key = philox_key(req.seed.data(), int(req.seed.size()));
-    hipDeviceProp_t prop;
-    RB_TRY(hipGetDeviceProperties(&prop, device));
-    const size_t lds_cu = size_t(prop.maxSharedMemoryPerMultiProcessor ? prop.maxSharedMemoryPerMultiProcessor
-                                                                        : prop.sharedMemPerBlock);
-    // cross-wave group chunk: as many groups as fit two workgroups per CU, else one workgroup
-    a.gchunk = 1;
-    const size_t two = lds_cu / 2;
-    for (int c = std::min(gmax, FUSED_GREG); c >= 1; --c) {
-        a.gchunk = c;
-        if (fused_lds_bytes(a) <= two) break;
-    }
-    if (fused_lds_bytes(a) > two)
-        for (int c = std::min(gmax, FUSED_GREG); c >= 1; --c) {
-            a.gchunk = c;
-            if (fused_lds_bytes(a) <= prop.sharedMemPerBlock) break;
-        }
-    const size_t lds = fused_lds_bytes(a);
-    if (lds > prop.sharedMemPerBlock) {
-        err = "fused kernel needs " + std::to_string(lds) + " bytes of LDS (too many options or groups)";
-        return ST_ERR_LIMIT;
-    }
-    const int per_cu = int(std::max<size_t>(1, std::min<size_t>(2, lds_cu / lds)));
-    fused_grid = int(std::min<long>(a.nitems, long(prop.multiProcessorCount) * per_cu));
-    RB_TRY(upload_vec(&d_fu, fu, stream));
-    RB_TRY(upload_vec(&d_fg, fg, stream));
-    RB_TRY(upload_vec(&d_fpg, fpg, stream));
-    RB_TRY(upload_vec(&d_fo, fo, stream));
-    RB_TRY(upload_vec(&d_frag, frag, stream));
-    RB_TRY(upload_vec(&d_ptab, ptab, stream));
-    RB_TRY(upload_vec(&d_opts, lay.options, stream));
-    const int nopt = a.out_size;
-    RB_TRY(hipMalloc(reinterpret_cast<void**>(&d_partial), size_t(nopt) * a.nitems * 2 * sizeof(double)));
-    RB_TRY(hipMalloc(reinterpret_cast<void**>(&d_price), nopt * sizeof(double)));
-    RB_TRY(hipMalloc(reinterpret_cast<void**>(&d_stat), nopt * sizeof(double)));
-    RB_TRY(hipMalloc(reinterpret_cast<void**>(&d_iv), nopt * sizeof(double)));
-    a.units = d_fu;
-    a.groups = d_fg;
-    a.pgs = d_fpg;
-    a.opts = d_fo;
-    a.frag = d_frag;
-    a.ptab = d_ptab;
-    a.partial = d_partial;
-    nblk = a.nitems;
-    chunk = req.M;
-    RB_TRY(hipStreamSynchronize(stream));
+    a.gchunk = std::min(gmax, FUSED_GREG);
+    out.gmax = gmax;
+    out.ivopts = lay.options;
+    // cost of one item (relative): RNG + integrals ~ N * groups, GEMM ~ N^2 / 64, payoffs ~ options
+    out.item_cost = double(a.tiles_per_item * a.TP) *
+                    (double(N) * (2.0 + gmax) + double(Np) * Np / 64.0 * nu + 4.0 * a.out_size);
     return ST_OK;
 }
 
-#define RB_LAUNCH(id, call)            \
-    do {                               \
-        const int slot_ = timed(id);   \
-        RB_TRY(call);                  \
-        mark(slot_, nullptr, nullptr); \
+#define RB_TRY(expr)                                                         \
+    do {                                                                     \
+        hipError_t e_ = (expr);                                              \
+        if (e_ != hipSuccess) {                                              \
+            err = std::string(#expr) + ": " + hipGetErrorString(e_);         \
+            return e_ == hipErrorOutOfMemory ? ST_ERR_NOMEM : ST_ERR_DEVICE; \
+        }                                                                    \
     } while (0)
 
+template <class T>
+static hipError_t upload_vec(T** p, const std::vector<T>& v, hipStream_t st) {
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(p), std::max<size_t>(v.size(), 1) * sizeof(T));
+    if (e != hipSuccess || v.empty()) return e;
+    return hipMemcpyAsync(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st);
+}
+
+FusedBatch::~FusedBatch() { release(); }
+
+void FusedBatch::release() {
+    void* bufs[] = {d_reqs, d_fu, d_fg, d_fpg, d_fo, d_frag, d_ptab, d_item_req, d_item_local,
+                    d_counter, d_partial, d_price, d_stat, d_iv, d_ivopts, d_row_off, d_row_n, d_row_M};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    d_reqs = nullptr;
+    d_fu = nullptr;
+    d_fg = nullptr;
+    d_fpg = nullptr;
+    d_fo = nullptr;
+    d_frag = nullptr;
+    d_ptab = nullptr;
+    d_item_req = nullptr;
+    d_item_local = nullptr;
+    d_counter = nullptr;
+    d_partial = nullptr;
+    d_price = d_stat = d_iv = nullptr;
+    d_ivopts = nullptr;
+    d_row_off = d_row_n = d_row_M = nullptr;
+}
+
+size_t FusedBatch::footprint(const std::vector<const FusedHostReq*>& reqs) {
+    size_t b = 0;
+    for (const FusedHostReq* q : reqs) {
+        b += q->frag.size() * 16 + q->ptab.size() * 4 + q->fo.size() * sizeof(FusedOption) +
+             q->fg.size() * sizeof(FusedGroup) + q->fpg.size() * sizeof(FusedPG) +
+             q->fu.size() * sizeof(FusedUnit) + sizeof(FusedReq);
+        b += size_t(q->r.out_size) * q->r.nitems * 16 + size_t(q->r.nitems) * 8;
+        b += size_t(q->r.out_size) * (3 * 8 + sizeof(Option) + 24);
+    }
+    return b;
+}
+
+int FusedBatch::build(const std::vector<const FusedHostReq*>& reqs, int mode_, int device,
+                      hipStream_t st, std::string& err) {
+    release();
+    mode = mode_;
+    nreq = int(reqs.size());
+    upw = 1;
+    // concatenate the tables, fixing the offsets inside each request
+    std::vector<FusedReq> R;
+    std::vector<FusedUnit> fu;
+    std::vector<FusedGroup> fg;
+    std::vector<FusedPG> fpg;
+    std::vector<FusedOption> fo;
+    std::vector<uint4> frag;
+    std::vector<float> ptab;
+    std::vector<Option> ivo;
+    std::vector<long> row_off, row_n, row_M;
+    out_off.assign(nreq + 1, 0);
+    long poff = 0;
+    size_t lds_need = 0;
+    int bf = 0, wp = 0, pg = 0, ac = 0;
+    for (int q = 0; q < nreq; ++q) {
+        const FusedHostReq& H = *reqs[q];
+        FusedReq r = H.r;
+        upw = std::max(upw, H.upw);
+        const int u0 = int(fu.size()), g0 = int(fg.size()), pg0 = int(fpg.size()), o0 = int(fo.size());
+        const int f0 = int(frag.size()), p0 = int(ptab.size());
+        for (FusedUnit U : H.fu) {
+            U.frag_off += f0;
+            U.p_off += p0;
+            U.g_begin += 0;
+            fu.push_back(U);
+        }
+        for (const FusedGroup& G : H.fg) fg.push_back(G);
+        for (FusedPG P : H.fpg) {
+            P.group += 0;  // group indices are request-local
+            fpg.push_back(P);
+        }
+        for (const FusedOption& O : H.fo) fo.push_back(O);
+        frag.insert(frag.end(), H.frag.begin(), H.frag.end());
+        ptab.insert(ptab.end(), H.ptab.begin(), H.ptab.end());
+        // pointers are fixed after upload; keep the element offsets for now
+        r.units = reinterpret_cast<const FusedUnit*>(size_t(u0));
+        r.groups = reinterpret_cast<const FusedGroup*>(size_t(g0));
+        r.pgs = reinterpret_cast<const FusedPG*>(size_t(pg0));
+        r.opts = reinterpret_cast<const FusedOption*>(size_t(o0));
+        r.partial = reinterpret_cast<double*>(size_t(poff));
+        for (int I = 0; I < r.out_size; ++I) {
+            row_off.push_back(poff + long(I) * r.nitems * 2);
+            row_n.push_back(r.nitems);
+            row_M.push_back(r.M);
+        }
+        poff += long(r.out_size) * r.nitems * 2;
+        out_off[q + 1] = out_off[q] + r.out_size;
+        ivo.insert(ivo.end(), H.ivopts.begin(), H.ivopts.end());
+        bf = std::max(bf, r.PB * r.NK * 128);
+        wp = std::max(wp, 4 * r.gchunk * 32 * 2);
+        pg = std::max(pg, r.TP * r.pgmax);
+        ac = std::max(ac, r.acc_in_lds ? 2 * r.out_size : 0);
+        R.push_back(r);
+    }
+    total_out = int(out_off[nreq]);
+    args.bf_u4 = bf;
+    args.wpart_d = wp;
+    args.pgd_n = pg;
+    args.acc_d = ac;
+    hipDeviceProp_t prop;
+    RB_TRY(hipGetDeviceProperties(&prop, device));
+    lds_need = fused_lds_bytes(args);
+    if (lds_need > prop.sharedMemPerBlock) {
+        err = "fused kernel needs " + std::to_string(lds_need) + " bytes of LDS (too many options or groups)";
+        return ST_ERR_LIMIT;
+    }
+    // work items, most expensive first
+    std::vector<std::pair<double, std::pair<int, int>>> items;
+    for (int q = 0; q < nreq; ++q)
+        for (long i = 0; i < R[q].nitems; ++i) items.push_back({reqs[q]->item_cost, {q, int(i)}});
+    std::stable_sort(items.begin(), items.end(),
+                     [](const std::pair<double, std::pair<int, int>>& x,
+                        const std::pair<double, std::pair<int, int>>& y) { return x.first > y.first; });
+    std::vector<int> item_req(items.size()), item_local(items.size());
+    for (size_t i = 0; i < items.size(); ++i) {
+        item_req[i] = items[i].second.first;
+        item_local[i] = items[i].second.second;
+    }
+    args.total_items = long(items.size());
+    // upload
+    RB_TRY(upload_vec(&d_fu, fu, st));
+    RB_TRY(upload_vec(&d_fg, fg, st));
+    RB_TRY(upload_vec(&d_fpg, fpg, st));
+    RB_TRY(upload_vec(&d_fo, fo, st));
+    RB_TRY(upload_vec(&d_frag, frag, st));
+    RB_TRY(upload_vec(&d_ptab, ptab, st));
+    RB_TRY(upload_vec(&d_ivopts, ivo, st));
+    RB_TRY(upload_vec(&d_row_off, row_off, st));
+    RB_TRY(upload_vec(&d_row_n, row_n, st));
+    RB_TRY(upload_vec(&d_row_M, row_M, st));
+    RB_TRY(upload_vec(&d_item_req, item_req, st));
+    RB_TRY(upload_vec(&d_item_local, item_local, st));
+    RB_TRY(hipMalloc(reinterpret_cast<void**>(&d_partial), std::max<size_t>(1, poff) * sizeof(double)));
+    RB_TRY(hipMalloc(reinterpret_cast<void**>(&d_counter), sizeof(int)));
+    RB_TRY(hipMalloc(reinterpret_cast<void**>(&d_price), std::max(1, total_out) * sizeof(double)));
+    RB_TRY(hipMalloc(reinterpret_cast<void**>(&d_stat), std::max(1, total_out) * sizeof(double)));
+    RB_TRY(hipMalloc(reinterpret_cast<void**>(&d_iv), std::max(1, total_out) * sizeof(double)));
+    for (FusedReq& r : R) {
+        r.units = d_fu + size_t(r.units);
+        r.groups = d_fg + size_t(r.groups);
+        r.pgs = d_fpg + size_t(r.pgs);
+        r.opts = d_fo + size_t(r.opts);
+        r.frag = d_frag;
+        r.ptab = d_ptab;
+        r.partial = d_partial + size_t(r.partial);
+    }
+    RB_TRY(upload_vec(&d_reqs, R, st));
+    args.reqs = d_reqs;
+    args.item_req = d_item_req;
+    args.item_local = d_item_local;
+    args.counter = d_counter;
+    const size_t lds_cu = size_t(prop.maxSharedMemoryPerMultiProcessor ? prop.maxSharedMemoryPerMultiProcessor
+                                                                        : prop.sharedMemPerBlock);
+    const int per_cu = int(std::max<size_t>(1, std::min<size_t>(2, lds_cu / lds_need)));
+    grid = int(std::min<long>(args.total_items, long(prop.multiProcessorCount) * per_cu));
+    RB_TRY(hipStreamSynchronize(st));
+    return ST_OK;
+}
+
+int FusedBatch::launch(hipStream_t st, std::string& err, Plan* timing) {
+    auto timed = [&](int id) { return timing ? timing->timed(id) : -1; };
+    auto mark = [&](int slot) {
+        if (timing) timing->mark(slot, nullptr, nullptr);
+    };
+    int s;
+    RB_TRY(launch_zero_int(d_counter, 1, st));
+    s = timed(K_FUSED);
+    RB_TRY(launch_fused(mode, upw, args, grid, st));
+    mark(s);
+    s = timed(K_REDUCE);
+    RB_TRY(launch_reduce_rows(d_partial, d_row_off, d_row_n, d_row_M, total_out, d_price, d_stat, st));
+    mark(s);
+    s = timed(K_IV);
+    RB_TRY(launch_iv_f64(d_price, d_ivopts, total_out, d_iv, st));
+    mark(s);
+    return ST_OK;
+}
+
+int FusedBatch::results(int q, double* price, double* iv, double* stat, hipStream_t st,
+                        std::string& err) {
+    const size_t n = size_t(out_off[q + 1] - out_off[q]) * sizeof(double);
+    if (price) RB_TRY(hipMemcpyAsync(price, d_price + out_off[q], n, hipMemcpyDeviceToHost, st));
+    if (iv) RB_TRY(hipMemcpyAsync(iv, d_iv + out_off[q], n, hipMemcpyDeviceToHost, st));
+    if (stat) RB_TRY(hipMemcpyAsync(stat, d_stat + out_off[q], n, hipMemcpyDeviceToHost, st));
+    return ST_OK;
+}
+
+// ---------------------------------------------------------------- single-request plan
+int Plan::init_fused(std::string& err) {
+    fused_req.reset(new FusedHostReq());
+    int rc = plan_fused_request(req, lay, fused_mode, *fused_req, err);
+    if (rc) return rc;
+    fused_batch.reset(new FusedBatch());
+    rc = fused_batch->build({fused_req.get()}, fused_mode, device, stream, err);
+    if (rc) return rc;
+    nblk = fused_req->r.nitems;
+    chunk = req.M;
+    return ST_OK;
+}
+
 int Plan::run_fused(std::string& err) {
-    const int nopt = fa.out_size;
     begin_timing();
-    RB_LAUNCH(K_FUSED, launch_fused(fused_mode, fused_upw, fa, fused_grid, stream));
-    RB_LAUNCH(K_REDUCE, launch_reduce_f64(d_partial, fa.nitems, req.M, nopt, d_price, d_stat, stream));
-    RB_LAUNCH(K_IV, launch_iv_f64(d_price, d_opts, nopt, d_iv, stream));
+    int rc = fused_batch->launch(stream, err, this);
+    if (rc) return rc;
     RB_TRY(hipStreamSynchronize(stream));
     end_timing();
     return ST_OK;

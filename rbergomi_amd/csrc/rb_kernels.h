@@ -23,6 +23,9 @@ hipError_t launch_payoff_f64(const double* ints, long np, long p0, const Option*
                              double* partial, double* payoff, hipStream_t st);
 hipError_t launch_reduce_f64(const double* partial, long nblk, long M, int out_size,
                              double* price, double* stat, hipStream_t st);
+hipError_t launch_reduce_rows(const double* partial, const long* row_off, const long* row_n,
+                              const long* row_M, int nrows, double* price, double* stat,
+                              hipStream_t st);
 hipError_t launch_iv_f64(const double* price, const Option* opts, int n, double* iv,
                          hipStream_t st);
 hipError_t launch_philox_probe(const void* ctr, PhiloxKey key, long n, void* out, hipStream_t st);
