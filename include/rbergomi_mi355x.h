@@ -124,6 +124,55 @@ int rb_plan_kernel_times(rb_plan* plan, char* names, double* total_ms, int* laun
 int rb_plan_info(rb_plan* plan, int* nunits, int* ngroups, int* noptions, long* chunk);
 void rb_plan_destroy(rb_plan* plan);
 
+/* ---- batches and serving (no counterpart in the reference) ----
+ * One pricing request in the reference's tensor form plus its caller-owned outputs
+ * (out_size = par_size*op_size doubles each; iv may be NULL). */
+typedef struct rb_request {
+    double xi;
+    const double* H;
+    const double* eta;
+    const double* rho;
+    int par_size;
+    const double* T;
+    const double* K;
+    int op_size;
+    int N;
+    long M;
+    const uint64_t* seed;
+    int seed_size;
+    double* price;
+    double* iv;
+    double* stat;
+} rb_request;
+
+/* Prices n requests with one launch of the split-precision kernel (cfg->precision must be
+ * RB_PREC_SPLIT_FP16 or RB_PREC_SPLIT_BF16, Romano-Touzi, N <= 1024).  Every request's results
+ * are bit-identical to pricing it alone with rb_price_v2. */
+int rb_price_batch(const rb_request* reqs, int n, const rb_config* cfg);
+
+/* Continuous-batching server.  A host thread forms batches from the pending requests, admits a
+ * request when the device bytes of the batches in flight leave room for it, runs up to
+ * max_inflight batches on separate streams (a new batch starts while earlier ones finish), and
+ * copies each request's results to its output arrays when its batch completes. */
+typedef struct rb_server rb_server;
+typedef struct rb_server_config {
+    int precision;            /* RB_PREC_SPLIT_FP16 or RB_PREC_SPLIT_BF16 */
+    int max_inflight;         /* batches on the device at once (>= 1) */
+    int max_batch_requests;   /* requests per batch (>= 1) */
+    long max_device_bytes;    /* admission budget for the batches in flight */
+} rb_server_config;
+int rb_server_create(const rb_server_config* cfg, rb_server** out);
+/* Copies the inputs; the output arrays must stay valid until rb_server_wait returns for the
+ * ticket.  Returns RB_ERR_LIMIT for a request that can never fit the device budget. */
+int rb_server_submit(rb_server* s, const rb_request* req, long* ticket);
+/* Blocks until the request is done; latency_ms (may be NULL) = submit to results copied. */
+int rb_server_wait(rb_server* s, long ticket, double* latency_ms);
+/* Latency percentiles over the requests completed so far, their count, the paths*steps they
+ * priced, and the number of batches launched. */
+int rb_server_stats(rb_server* s, double* p50_ms, double* p99_ms, long* completed,
+                    double* path_steps, long* batches);
+void rb_server_destroy(rb_server* s);
+
 /* Host-only planning (no device call): group index of every option I and the unit of every
  * group, as the executor deduplicates them.  group_of_option has out_size entries,
  * unit_of_group at least out_size entries. */

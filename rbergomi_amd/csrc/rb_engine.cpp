@@ -197,7 +197,8 @@ static hipError_t upload(T** p, const std::vector<T>& v, hipStream_t st) {
 
 Plan::~Plan() {
     if (stream) (void)hipStreamSynchronize(stream);
-    fused_batch.reset();
+    fused_batch.reset();  // stream-ordered frees on this plan's stream
+    if (stream) (void)hipStreamSynchronize(stream);
     void* bufs[] = {d_units, d_gamma, d_groups, d_comp, d_opts, d_Z,   d_Wt,
                     d_ints,  d_partial, d_price, d_stat, d_iv, d_payoff};
     for (void* b : bufs)

@@ -81,7 +81,7 @@ __global__ void __launch_bounds__(256, 2) k_fused(FusedBatchArgs B) {
         const long gi = s_item;
         __syncthreads();
         if (gi >= B.total_items) break;
-        const FusedReq& A = B.reqs[B.item_req[gi]];
+        const FusedReq A = B.reqs[B.item_req[gi]];   // copied once per item (no aliasing reloads)
         const long item = B.item_local[gi];
         const int NK = A.NK;
         const int pbw = A.wave_pb[wave];

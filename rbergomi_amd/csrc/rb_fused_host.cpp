@@ -204,9 +204,15 @@ int plan_fused_request(const Request& req, const Layout& lay, int mode, FusedHos
 
 template <class T>
 static hipError_t upload_vec(T** p, const std::vector<T>& v, hipStream_t st) {
-    hipError_t e = hipMalloc(reinterpret_cast<void**>(p), std::max<size_t>(v.size(), 1) * sizeof(T));
+    // stream-ordered allocation: freeing a batch never waits for other streams
+    hipError_t e = hipMallocAsync(reinterpret_cast<void**>(p), std::max<size_t>(v.size(), 1) * sizeof(T), st);
     if (e != hipSuccess || v.empty()) return e;
     return hipMemcpyAsync(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st);
+}
+
+template <class T>
+static hipError_t alloc_async(T** p, size_t n, hipStream_t st) {
+    return hipMallocAsync(reinterpret_cast<void**>(p), std::max<size_t>(n, 1) * sizeof(T), st);
 }
 
 FusedBatch::~FusedBatch() { release(); }
@@ -215,7 +221,7 @@ void FusedBatch::release() {
     void* bufs[] = {d_reqs, d_fu, d_fg, d_fpg, d_fo, d_frag, d_ptab, d_item_req, d_item_local,
                     d_counter, d_partial, d_price, d_stat, d_iv, d_ivopts, d_row_off, d_row_n, d_row_M};
     for (void* b : bufs)
-        if (b) (void)hipFree(b);
+        if (b) (void)hipFreeAsync(b, stream);
     d_reqs = nullptr;
     d_fu = nullptr;
     d_fg = nullptr;
@@ -247,6 +253,7 @@ size_t FusedBatch::footprint(const std::vector<const FusedHostReq*>& reqs) {
 int FusedBatch::build(const std::vector<const FusedHostReq*>& reqs, int mode_, int device,
                       hipStream_t st, std::string& err) {
     release();
+    stream = st;
     mode = mode_;
     nreq = int(reqs.size());
     upw = 1;
@@ -342,11 +349,11 @@ int FusedBatch::build(const std::vector<const FusedHostReq*>& reqs, int mode_, i
     RB_TRY(upload_vec(&d_row_M, row_M, st));
     RB_TRY(upload_vec(&d_item_req, item_req, st));
     RB_TRY(upload_vec(&d_item_local, item_local, st));
-    RB_TRY(hipMalloc(reinterpret_cast<void**>(&d_partial), std::max<size_t>(1, poff) * sizeof(double)));
-    RB_TRY(hipMalloc(reinterpret_cast<void**>(&d_counter), sizeof(int)));
-    RB_TRY(hipMalloc(reinterpret_cast<void**>(&d_price), std::max(1, total_out) * sizeof(double)));
-    RB_TRY(hipMalloc(reinterpret_cast<void**>(&d_stat), std::max(1, total_out) * sizeof(double)));
-    RB_TRY(hipMalloc(reinterpret_cast<void**>(&d_iv), std::max(1, total_out) * sizeof(double)));
+    RB_TRY(alloc_async(&d_partial, size_t(poff), st));
+    RB_TRY(alloc_async(&d_counter, 1, st));
+    RB_TRY(alloc_async(&d_price, size_t(total_out), st));
+    RB_TRY(alloc_async(&d_stat, size_t(total_out), st));
+    RB_TRY(alloc_async(&d_iv, size_t(total_out), st));
     for (FusedReq& r : R) {
         r.units = d_fu + size_t(r.units);
         r.groups = d_fg + size_t(r.groups);

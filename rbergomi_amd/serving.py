@@ -1,0 +1,117 @@
+"""Batched pricing and the continuous-batching server (rb_price_batch, rb_server_*).
+
+Each request is the reference's tensor form (xi, H, eta, rho zipped; T, K zipped; N, M, seed),
+as accepted by computeIVRT_tensor (rBergLib/cinterface.h:68-71), priced on the split-precision
+kernel.  A request's results are bit-identical whether it is priced alone or in a batch.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import RbConfig, check, dp, u64p
+
+
+class RbRequest(ctypes.Structure):
+    _fields_ = [("xi", ctypes.c_double), ("H", dp), ("eta", dp), ("rho", dp), ("par_size", ctypes.c_int),
+                ("T", dp), ("K", dp), ("op_size", ctypes.c_int), ("N", ctypes.c_int), ("M", ctypes.c_long),
+                ("seed", u64p), ("seed_size", ctypes.c_int), ("price", dp), ("iv", dp), ("stat", dp)]
+
+
+class RbServerConfig(ctypes.Structure):
+    _fields_ = [("precision", ctypes.c_int), ("max_inflight", ctypes.c_int),
+                ("max_batch_requests", ctypes.c_int), ("max_device_bytes", ctypes.c_long)]
+
+
+def _declare(L):
+    if getattr(L, "_serving_declared", False):
+        return
+    vp = ctypes.c_void_p
+    L.rb_price_batch.argtypes = [ctypes.POINTER(RbRequest), ctypes.c_int, ctypes.POINTER(RbConfig)]
+    L.rb_server_create.argtypes = [ctypes.POINTER(RbServerConfig), ctypes.POINTER(vp)]
+    L.rb_server_submit.argtypes = [vp, ctypes.POINTER(RbRequest), ctypes.POINTER(ctypes.c_long)]
+    L.rb_server_wait.argtypes = [vp, ctypes.c_long, dp]
+    L.rb_server_stats.argtypes = [vp, dp, dp, ctypes.POINTER(ctypes.c_long), dp, ctypes.POINTER(ctypes.c_long)]
+    L.rb_server_destroy.argtypes = [vp]
+    L.rb_server_destroy.restype = None
+    L._serving_declared = True
+
+
+class Request:
+    """One pricing request with its own output arrays (kept alive until done)."""
+
+    def __init__(self, xi, H, eta, rho, T, K, N, M, seed):
+        a = lambda x: np.ascontiguousarray(np.atleast_1d(np.asarray(x, dtype=np.float64)))
+        self.H, self.eta, self.rho, self.T, self.K = map(a, (H, eta, rho, T, K))
+        self.seed = np.ascontiguousarray(np.atleast_1d(np.asarray(seed, dtype=np.uint64)))
+        n = len(self.H) * len(self.T)
+        self.price, self.iv, self.stat = np.zeros(n), np.zeros(n), np.zeros(n)
+        self.N, self.M, self.xi = int(N), int(M), float(xi)
+        P = lambda x: x.ctypes.data_as(dp)
+        self.c = RbRequest(self.xi, P(self.H), P(self.eta), P(self.rho), len(self.H), P(self.T), P(self.K),
+                           len(self.T), self.N, self.M, self.seed.ctypes.data_as(u64p), len(self.seed),
+                           P(self.price), P(self.iv), P(self.stat))
+        self.ticket = None
+        self.latency_ms = None
+
+
+def price_batch(requests, precision=_lib.PREC_SPLIT_FP16):
+    """Prices all requests with one launch; fills each request's price/iv/stat."""
+    L = _lib.load()
+    _declare(L)
+    arr = (RbRequest * len(requests))(*[r.c for r in requests])
+    cfg = RbConfig(precision, _lib.COMP_REF, _lib.METHOD_RT, 0)
+    check(L.rb_price_batch(arr, len(requests), ctypes.byref(cfg)))
+    return requests
+
+
+class Server:
+    """Continuous-batching server (rb_server_*)."""
+
+    def __init__(self, precision=_lib.PREC_SPLIT_FP16, max_inflight=2, max_batch_requests=64,
+                 max_device_bytes=8 << 30):
+        L = _lib.load()
+        _declare(L)
+        self.ptr = ctypes.c_void_p()
+        cfg = RbServerConfig(precision, max_inflight, max_batch_requests, max_device_bytes)
+        check(L.rb_server_create(ctypes.byref(cfg), ctypes.byref(self.ptr)))
+        self.live = {}
+
+    def submit(self, req):
+        t = ctypes.c_long()
+        check(_lib.load().rb_server_submit(self.ptr, ctypes.byref(req.c), ctypes.byref(t)))
+        req.ticket = t.value
+        self.live[t.value] = req
+        return req
+
+    def wait(self, req):
+        lat = ctypes.c_double()
+        rc = _lib.load().rb_server_wait(self.ptr, req.ticket, ctypes.byref(lat))
+        self.live.pop(req.ticket, None)
+        check(rc)
+        req.latency_ms = lat.value
+        return req
+
+    def stats(self):
+        p50, p99, ps = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        n, b = ctypes.c_long(), ctypes.c_long()
+        check(_lib.load().rb_server_stats(self.ptr, ctypes.byref(p50), ctypes.byref(p99), ctypes.byref(n),
+                                          ctypes.byref(ps), ctypes.byref(b)))
+        return {"p50_ms": p50.value, "p99_ms": p99.value, "completed": n.value, "path_steps": ps.value,
+                "batches": b.value}
+
+    def close(self):
+        if self.ptr:
+            for r in list(self.live.values()):
+                try:
+                    self.wait(r)
+                except Exception:
+                    pass
+            _lib.load().rb_server_destroy(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

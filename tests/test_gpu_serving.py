@@ -1,0 +1,90 @@
+"""GPU checks of batched pricing and of the continuous-batching server: every request's
+results are bit-identical alone (rb_price_v2) and in a batch or a server mix; admission under
+a device-byte budget; oversize requests rejected."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def rb():
+    import rbergomi_amd
+    rbergomi_amd.load()
+    return rbergomi_amd
+
+
+def make_requests(n, seed=0):
+    from rbergomi_amd.serving import Request
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for i in range(n):
+        npar = int(rng.integers(1, 3))
+        H = rng.uniform(0.02, 0.45, npar)
+        eta = rng.uniform(0.5, 4.0, npar)
+        rho = rng.uniform(-0.99, -0.1, npar)
+        nopt = int(rng.integers(3, 40))
+        T = rng.choice(rng.uniform(0.05, 2.0, 3), nopt)
+        K = np.exp(rng.uniform(-0.4, 0.2, nopt))
+        N = int(rng.choice([32, 50, 64, 128, 200, 256, 512]))
+        M = int(rng.integers(500, 20000))
+        reqs.append(Request(rng.uniform(0.01, 0.16), H, eta, rho, T, K, N, M, [i, 77]))
+    return reqs
+
+
+def alone(rb, r, prec):
+    return rb.price_v2(r.xi, r.H, r.eta, r.rho, r.T, r.K, r.N, r.M, r.seed, precision=prec)
+
+
+@pytest.mark.parametrize("prec_name", ["fp16", "bf16"])
+def test_batch_equals_alone(rb, prec_name):
+    from rbergomi_amd.serving import price_batch
+    prec = rb.PREC_SPLIT_FP16 if prec_name == "fp16" else rb.PREC_SPLIT_BF16
+    reqs = make_requests(12, seed=1)
+    price_batch(reqs, precision=prec)
+    for r in reqs:
+        p, iv, st = alone(rb, r, prec)
+        assert np.array_equal(p, r.price) and np.array_equal(st, r.stat)
+        assert np.array_equal(iv, r.iv, equal_nan=True)
+
+
+def test_server_mix_equals_alone(rb):
+    from rbergomi_amd.serving import Server
+    reqs = make_requests(24, seed=2)
+    with Server(max_inflight=2, max_batch_requests=5) as srv:
+        for r in reqs:
+            srv.submit(r)
+        for r in reqs:
+            srv.wait(r)
+            assert r.latency_ms > 0
+        st = srv.stats()
+    assert st["completed"] == len(reqs) and st["batches"] >= 5
+    assert st["p99_ms"] >= st["p50_ms"] > 0
+    for r in reqs:
+        p, iv, s = alone(rb, r, rb.PREC_SPLIT_FP16)
+        assert np.array_equal(p, r.price) and np.array_equal(s, r.stat)
+
+
+def test_server_admits_under_tight_budget(rb):
+    from rbergomi_amd.serving import Server
+    reqs = make_requests(6, seed=3)
+    # budget below two requests: batches are admitted one at a time as memory frees up
+    with Server(max_inflight=3, max_batch_requests=8, max_device_bytes=3 << 20) as srv:
+        for r in reqs:
+            srv.submit(r)
+        for r in reqs:
+            srv.wait(r)
+        st = srv.stats()
+    assert st["completed"] == 6
+    for r in reqs:
+        p, _, s = alone(rb, r, rb.PREC_SPLIT_FP16)
+        assert np.array_equal(p, r.price)
+
+
+def test_server_rejects_oversize_request(rb):
+    from rbergomi_amd.serving import Request, Server
+    big = Request(0.04, [0.1], [1.5], [-0.7], np.ones(200), np.linspace(0.8, 1.2, 200), 256, 1 << 22, [1])
+    with Server(max_device_bytes=1 << 20) as srv:
+        with pytest.raises(rb.RbError) as e:
+            srv.submit(big)
+    assert e.value.code == 5
