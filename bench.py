@@ -59,17 +59,194 @@ def cpu_baseline(sample_paths, threads):
                       "%.1f s" % (sample_paths, threads, dt)}
 
 
+def batched_requests(nsets=256, seed=3):
+    """256 parameter sets drawn from a seed: H~U[0.02,0.45], eta~U[0.5,4], rho~U[-0.99,-0.1],
+    xi~U[0.01,0.16], T~U[0.05,2], N uniform on {32,64,128,256,512}, M = 2^U with U~U[14,20],
+    21 strikes evenly in [0.8, 1.2] at T."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(nsets):
+        H, eta, rho = rng.uniform(0.02, 0.45), rng.uniform(0.5, 4.0), rng.uniform(-0.99, -0.1)
+        xi, T = rng.uniform(0.01, 0.16), rng.uniform(0.05, 2.0)
+        N = int(rng.choice([32, 64, 128, 256, 512]))
+        M = int(round(2 ** rng.uniform(14, 20)))
+        out.append(dict(xi=xi, H=[H], eta=[eta], rho=[rho], T=[T] * 21, K=np.linspace(0.8, 1.2, 21),
+                        N=N, M=M, seed=[seed, i]))
+    return out
+
+
+def serving_requests(n, seed=5):
+    """Calibration requests: parameters as in the batched workload, 10-200 options spread over
+    1-5 maturities T~U[0.02,2] with log-moneyness U[-0.5,0.3], M = 2^U with U~U[14,21],
+    N = max(32, ceil(256 T_max))."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        H, eta, rho = rng.uniform(0.02, 0.45), rng.uniform(0.5, 4.0), rng.uniform(-0.99, -0.1)
+        xi = rng.uniform(0.01, 0.16)
+        mats = rng.uniform(0.02, 2.0, int(rng.integers(1, 6)))
+        nopt = int(rng.integers(10, 201))
+        T = np.sort(rng.choice(mats, nopt))
+        K = np.exp(rng.uniform(-0.5, 0.3, nopt))
+        N = max(32, int(np.ceil(256 * mats.max())))
+        M = int(round(2 ** rng.uniform(14, 21)))
+        out.append(dict(xi=xi, H=[H], eta=[eta], rho=[rho], T=T, K=K, N=N, M=M, seed=[seed, i]))
+    return out
+
+
+def run_serving(rb, reqs, prec, rate_fraction=0.7, max_inflight=2, max_batch=64):
+    """Closed loop (all submitted at once) gives the capacity; then Poisson arrivals at
+    rate_fraction x capacity give the latency percentiles."""
+    import threading
+    from rbergomi_amd.serving import Request, Server
+
+    def mk(d):
+        return Request(d["xi"], d["H"], d["eta"], d["rho"], d["T"], d["K"], d["N"], d["M"], d["seed"])
+
+    work = sum(d["M"] * d["N"] for d in reqs)
+    with Server(prec, max_inflight=max_inflight, max_batch_requests=max_batch) as srv:
+        rs = [mk(d) for d in reqs]
+        t0 = time.perf_counter()
+        for r in rs:
+            srv.submit(r)
+        for r in rs:
+            srv.wait(r)
+        closed = time.perf_counter() - t0
+    capacity = len(reqs) / closed
+    rate = rate_fraction * capacity
+    rng = np.random.default_rng(11)
+    gaps = rng.exponential(1.0 / rate, len(reqs))
+    with Server(prec, max_inflight=max_inflight, max_batch_requests=max_batch) as srv:
+        rs = [mk(d) for d in reqs]
+        done = []
+
+        def waiter():
+            for r in rs:
+                while r.ticket is None:
+                    time.sleep(1e-4)
+                srv.wait(r)
+                done.append(r.latency_ms)
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        t0 = time.perf_counter()
+        t_next = t0
+        for r, g in zip(rs, gaps):
+            t_next += g
+            dt = t_next - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            srv.submit(r)
+        th.join()
+        open_s = time.perf_counter() - t0
+        st = srv.stats()
+    lat = np.array(done)
+    return {"requests": len(reqs), "paths_steps": float(work),
+            "closed_loop_s": closed, "closed_loop_paths_steps_per_s": work / closed,
+            "capacity_req_per_s": capacity, "offered_req_per_s": rate,
+            "open_loop_s": open_s, "open_loop_paths_steps_per_s": work / open_s,
+            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "server_p50_ms": st["p50_ms"], "server_p99_ms": st["p99_ms"], "batches": st["batches"]}
+
+
+def bench_batched(rb, args):
+    from rbergomi_amd.serving import Request, price_batch
+    reqs = batched_requests()
+    work = sum(d["M"] * d["N"] for d in reqs)
+    out = {"metric": "paths*steps/s", "unit": "paths*steps/s", "n_gpus": 1, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "data": "synthetic (device Philox4x32-10 Gaussians)",
+           "config": {"workload": "batched: 256 parameter sets, N in {32..512}, M = 2^14..2^20, 21 strikes",
+                      "paths_steps": work, "parallelism": "1 GPU"}}
+    # plain FP64 single-request GPU path: the requests one after another
+    t0 = time.perf_counter()
+    ref = [rb.price_v2(d["xi"], d["H"], d["eta"], d["rho"], d["T"], d["K"], d["N"], d["M"], d["seed"],
+                       precision=rb.PREC_FP64) for d in reqs]
+    fp64_s = time.perf_counter() - t0
+    out["fp64_single_request"] = {"s": fp64_s, "paths_steps_per_s": work / fp64_s}
+    for name, code in (("fp16", rb.PREC_SPLIT_FP16), ("bf16", rb.PREC_SPLIT_BF16)):
+        rs = [Request(d["xi"], d["H"], d["eta"], d["rho"], d["T"], d["K"], d["N"], d["M"], d["seed"])
+              for d in reqs]
+        price_batch(rs, precision=code)  # warm-up (code objects, allocator)
+        times = []
+        for _ in range(args.steps):
+            t0 = time.perf_counter()
+            price_batch(rs, precision=code)
+            times.append(time.perf_counter() - t0)
+        ms = 1e3 * float(np.median(times))
+        err = [np.max(np.abs(r.price - p[0])) for r, p in zip(rs, ref)]
+        ratio = [np.max(np.abs(r.price - p[0]) / np.maximum(p[2], 1e-300)) for r, p in zip(rs, ref)]
+        out[name] = {"ms_per_batch": ms, "paths_steps_per_s": work / (ms * 1e-3),
+                     "max_abs_price_err_vs_fp64": float(np.max(err)),
+                     "median_abs_price_err_vs_fp64": float(np.median(err)),
+                     "max_err_over_stat": float(np.max(ratio)),
+                     "speedup_vs_fp64_single_request": fp64_s / (ms * 1e-3)}
+    out["value"] = out["fp16"]["paths_steps_per_s"]
+    out["ms_per_step"] = out["fp16"]["ms_per_batch"]
+    out["dtype"] = "f16x2+f32"
+    out["steps"], out["warmup"] = args.steps, 1
+    return out
+
+
+def bench_serving(rb, args):
+    reqs = serving_requests(args.requests)
+    res = run_serving(rb, reqs, rb.PREC_SPLIT_FP16)
+    out = {"metric": "paths*steps/s", "value": res["open_loop_paths_steps_per_s"], "unit": "paths*steps/s",
+           "n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f16x2+f32", "data": "synthetic (device Philox4x32-10 Gaussians)",
+           "config": {"workload": "serving: %d calibration requests, Poisson arrivals at 0.7 x capacity, "
+                                  "10-200 options over 1-5 maturities, M = 2^14..2^21, N = max(32, 256 T)"
+                                  % args.requests, "parallelism": "1 GPU"},
+           "serving": res, "p50_ms": res["p50_ms"], "p99_ms": res["p99_ms"]}
+    return out
+
+
+def bench_long(rb, args):
+    """N in {1024, 2048, 4096}, H in {0.02, 0.05}, eta 2.3, rho -0.9, xi 0.03, T 1, M = 2^18,
+    31 strikes in [0.7, 1.3]: split FP16 where it applies (N <= 1024), FP64 path everywhere."""
+    K = np.linspace(0.7, 1.3, 31)
+    rows = []
+    for N in (1024, 2048, 4096):
+        for H in (0.02, 0.05):
+            row = {"N": N, "H": H}
+            for name, code in (("fp64", rb.PREC_FP64), ("fp16", rb.PREC_SPLIT_FP16)):
+                if name == "fp16" and N > 1024:
+                    continue
+                pl = rb.Plan(0.03, [H], [2.3], [-0.9], [1.0] * 31, K, N, 2 ** 18, [4, N], precision=code)
+                pl.run()
+                t0 = time.perf_counter()
+                for _ in range(args.steps):
+                    pl.run()
+                ms = (time.perf_counter() - t0) * 1e3 / args.steps
+                p, _, st = pl.results()
+                pl.close()
+                row[name] = {"ms": ms, "paths_steps_per_s": 2 ** 18 * N / (ms * 1e-3)}
+                row[name + "_price_atm"] = float(p[15])
+                if name == "fp16":
+                    row["fp16_max_abs_err_vs_fp64"] = float(np.max(np.abs(p - row["_p64"])))
+                else:
+                    row["_p64"] = p
+            row.pop("_p64", None)
+            rows.append(row)
+    return {"metric": "paths*steps/s", "unit": "paths*steps/s", "n_gpus": 1, "higher_is_better": True,
+            "value": max(r["fp64"]["paths_steps_per_s"] for r in rows), "dtype": "f64",
+            "config": {"workload": "long paths: N in {1024,2048,4096}, M = 2^18, 31 strikes"},
+            "rows": rows, "vs_baseline": None, "scaling": "weak", "data": "synthetic"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--precision", default="fp16", choices=sorted(PREC_CODE))
+    ap.add_argument("--workload", default="single", choices=["single", "batched", "serving", "long"])
+    ap.add_argument("--requests", type=int, default=4096, help="serving workload size")
     ap.add_argument("--paths", type=int, default=2 ** 22)
     ap.add_argument("--cpu-sample-paths", type=int, default=2 ** 18)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 comparison run")
+    ap.add_argument("--no-latency", action="store_true", help="skip the serving latency sample")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="also run the streaming copy probe over this many MiB (traffic calibration)")
     args = ap.parse_args()
@@ -80,6 +257,10 @@ def main():
 
     import rbergomi_amd as rb
 
+    if args.workload != "single":
+        fn = {"batched": bench_batched, "serving": bench_serving, "long": bench_long}[args.workload]
+        print(json.dumps(fn(rb, args)), flush=True)
+        return
     w = workload(args.paths)
     plan = rb.Plan(w["xi"], w["H"], w["eta"], w["rho"], w["T"], w["K"], w["N"], w["M"],
                    [20240601, 7], precision=PREC_CODE[args.precision])
@@ -148,6 +329,12 @@ def main():
                            "max_abs_stat_err_vs_fp64": float(np.max(np.abs(stat - s64))),
                            "max_price_stat_ratio": float(np.max(np.abs(price - p64) / s64)),
                            "reference": "staged FP64 path, same Philox counters"}
+    if not args.no_latency and args.precision in ("fp16", "bf16"):
+        # request latency under load: 256 serving requests, Poisson arrivals at 0.7 x capacity
+        lat = run_serving(rb, serving_requests(256), PREC_CODE[args.precision])
+        out["latency"] = {k: lat[k] for k in ("requests", "p50_ms", "p99_ms", "offered_req_per_s",
+                                               "open_loop_paths_steps_per_s", "batches")}
+        out["p99_ms"] = lat["p99_ms"]
     if args.copy_probe:
         import ctypes
         ms_copy = ctypes.c_double()

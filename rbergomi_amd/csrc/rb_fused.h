@@ -27,7 +27,7 @@ struct FusedGroup {
     float al;        // eta^2 dt^(2H) log2(e) / 4
     float be;        // -ln(xi) log2(e) / 2
     int pg_begin, pg_end;  // payoff groups of this group (contiguous)
-    int pad;
+    int o_begin, o_end;    // options of this group (contiguous)
     double xi;
     double sxi;
     double dt;
@@ -63,6 +63,7 @@ struct FusedReq {
     int N, Np, NK, PB, TP;       // steps, padded steps (x32), 16-step k-blocks, path blocks, paths per tile
     int log2TP;                  // log2(TP)
     int nunits, pgmax, out_size, gchunk, acc_in_lds;
+    int log2slices;              // lanes per option in the payoff stage (register mode)
     int tiles_per_item;
     long M, ntiles, nitems;
     PhiloxKey key;
@@ -88,7 +89,7 @@ struct FusedBatchArgs {
     int* counter;                // next work item (zeroed before the launch)
     int bf_u4;                   // LDS: B fragments, in 16-byte units
     int wpart_d;                 // LDS: cross-wave partials, in doubles
-    int pgd_n;                   // LDS: per-(path, payoff group) records
+    int pgd_n;                   // LDS: per-(path, payoff group) records of one group chunk
     int acc_d;                   // LDS: option sums (largest request in LDS mode), in doubles
 };
 

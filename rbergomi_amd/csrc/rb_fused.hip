@@ -17,8 +17,9 @@
 //      intRootVdW, :73-80, 163-167, 427-436): FP32 terms summed per 16-row block, FP64 beyond.
 //   4. Per (path, group, rho): ln S', sigma, 1/sigma, S' once; per strike the conditioned
 //      Black-Scholes payoff S' Phi(d1) - K Phi(d2) (:383-391) in FP32, with the FP64 normal
-//      CDF for deep out-of-the-money lanes (d1 < -8).  Each thread owns one option and sums
-//      its payoffs over the paths in FP64 registers (no cross-lane reduction).
+//      CDF for deep out-of-the-money lanes (d1 < -8), for the options of up to 8 groups at a
+//      time.  The paths of an option are split over S = 2^k lanes (S ~ 256 / options) and summed
+//      in FP64 with a fixed butterfly into the register of the option's owner lane.
 #include <hip/hip_runtime.h>
 
 #include "rb_fused.h"
@@ -60,7 +61,7 @@ __device__ __forceinline__ double half_sum_f64(double v) {
 }
 
 template <int MODE, int UPW>
-__global__ void __launch_bounds__(256, 2) k_fused(FusedBatchArgs B) {
+__global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs B) {
     typedef SplitT<MODE> S;
     typedef typename S::T T;
     typedef typename S::V V;
@@ -232,6 +233,7 @@ __global__ void __launch_bounds__(256, 2) k_fused(FusedBatchArgs B) {
                         ss[gg] += __shfl_xor(ss[gg], 32, 64);
                     }
                     // cross-wave sums in chunks of gchunk groups (LDS budget)
+                    const int pg_lo = A.groups[g0].pg_begin;
                     for (int c0 = 0; c0 < gn; c0 += A.gchunk) {
                         const int cn = gn - c0 < A.gchunk ? gn - c0 : A.gchunk;
                         if (h == 0) {
@@ -267,36 +269,57 @@ __global__ void __launch_bounds__(256, 2) k_fused(FusedBatchArgs B) {
                                 q.invs = float(1.0 / q.sig);
                                 q.lnSf = float(q.lnS);
                                 q.sigf = float(q.sig);
-                                pgd[pl * A.pgmax + (pg - U.pg_begin)] = q;
+                                pgd[pl * A.pgmax + (pg - pg_lo)] = q;
                             }
                         }
                         __syncthreads();
                     }
-                }
-                // ---- 4. payoffs: thread t owns fused option t (t + 256 r in LDS mode) and loops
-                //         over the tile's paths; no cross-lane reduction
-                const int ofirst = A.acc_in_lds ? U.o_begin + tid : (tid >= U.o_begin ? tid : U.o_end);
-                const int ostep = A.acc_in_lds ? 256 : (1 << 30);
-                for (int oi = ofirst; oi < U.o_end; oi += ostep) {
-                    const FusedOption O = A.opts[oi];
-                    const int pgl = O.pg - U.pg_begin;
-                    const float lnKf = float(O.lnK), Kf = float(O.K);
-                    double s1 = 0.0, s2 = 0.0;
+                    // ---- 4. payoffs of the options of groups g0 .. g0+gn-1.  Register mode: the
+                    //         S = 2^log2slices lanes t = o*S .. o*S+S-1 share option o, each
+                    //         summing every S-th path; a fixed butterfly leaves the total in lane
+                    //         o*S, which owns option o's FP64 sums.  LDS mode (> 256 options):
+                    //         one thread per option, sums in LDS.
+                    const int o_lo = A.groups[g0].o_begin, o_hi = A.groups[g0 + gn - 1].o_end;
                     const int np = A.M - p0 < A.TP ? int(A.M - p0) : A.TP;
-                    for (int pl = 0; pl < np; ++pl) {
-                        const double x = payoff_mixed(pgd[pl * A.pgmax + pgl], lnKf, Kf, O.lnK, O.K);
-                        s1 += x;
-                        s2 += x * x;
-                    }
                     if (A.acc_in_lds) {
-                        accopt[2 * oi + 0] += s1;
-                        accopt[2 * oi + 1] += s2;
+                        for (int oi = o_lo + tid; oi < o_hi; oi += 256) {
+                            const FusedOption O = A.opts[oi];
+                            const int pgl = O.pg - pg_lo;
+                            const float lnKf = float(O.lnK), Kf = float(O.K);
+                            double s1 = 0.0, s2 = 0.0;
+                            for (int pl = 0; pl < np; ++pl) {
+                                const double x = payoff_mixed(pgd[pl * A.pgmax + pgl], lnKf, Kf, O.lnK, O.K);
+                                s1 += x;
+                                s2 += x * x;
+                            }
+                            accopt[2 * oi + 0] += s1;
+                            accopt[2 * oi + 1] += s2;
+                        }
                     } else {
-                        a1 += s1;
-                        a2 += s2;
+                        const int S = 1 << A.log2slices;
+                        const int oi = tid >> A.log2slices, sl = tid & (S - 1);
+                        if (oi >= o_lo && oi < o_hi) {
+                            const FusedOption O = A.opts[oi];
+                            const int pgl = O.pg - pg_lo;
+                            const float lnKf = float(O.lnK), Kf = float(O.K);
+                            double s1 = 0.0, s2 = 0.0;
+                            for (int pl = sl; pl < np; pl += S) {
+                                const double x = payoff_mixed(pgd[pl * A.pgmax + pgl], lnKf, Kf, O.lnK, O.K);
+                                s1 += x;
+                                s2 += x * x;
+                            }
+                            for (int off = S >> 1; off >= 1; off >>= 1) {
+                                s1 += __shfl_xor(s1, off, 64);
+                                s2 += __shfl_xor(s2, off, 64);
+                            }
+                            if (sl == 0) {
+                                a1 += s1;
+                                a2 += s2;
+                            }
+                        }
                     }
+                    __syncthreads();
                 }
-                __syncthreads();
             }
         }
         // ---- item partial sums (indexed by the output index I)
@@ -306,8 +329,8 @@ __global__ void __launch_bounds__(256, 2) k_fused(FusedBatchArgs B) {
                 A.partial[(long(I) * A.nitems + item) * 2 + 0] = accopt[2 * o + 0];
                 A.partial[(long(I) * A.nitems + item) * 2 + 1] = accopt[2 * o + 1];
             }
-        } else if (tid < A.out_size) {
-            const int I = A.opts[tid].out;
+        } else if ((tid & ((1 << A.log2slices) - 1)) == 0 && (tid >> A.log2slices) < A.out_size) {
+            const int I = A.opts[tid >> A.log2slices].out;
             A.partial[(long(I) * A.nitems + item) * 2 + 0] = a1;
             A.partial[(long(I) * A.nitems + item) * 2 + 1] = a2;
         }

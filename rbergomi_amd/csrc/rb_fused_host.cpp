@@ -126,12 +126,12 @@ int plan_fused_request(const Request& req, const Layout& lay, int mode, FusedHos
             F.c1 = float(0.5 * G.eta * G.TH * log2e);
             F.al = float(0.25 * G.eta * G.eta * std::pow(G.dt, 2.0 * H) * log2e);
             F.be = float(-0.5 * std::log(G.xi) * log2e);
-            F.pad = 0;
             F.xi = G.xi;
             F.sxi = std::sqrt(G.xi);
             F.dt = G.dt;
             F.sdt = G.sdt;
             F.pg_begin = int(out.fpg.size());
+            F.o_begin = int(out.fo.size());
             const int fgi = int(out.fg.size());
             for (size_t I = 0; I < lay.options.size(); ++I) {
                 const Option& o = lay.options[I];
@@ -146,12 +146,17 @@ int plan_fused_request(const Request& req, const Layout& lay, int mode, FusedHos
                 out.fo.push_back(FusedOption{pg, int(I), o.K, o.lnK});
             }
             F.pg_end = int(out.fpg.size());
+            F.o_end = int(out.fo.size());
             out.fg.push_back(F);
         }
         U.g_end = int(out.fg.size());
         U.pg_end = int(out.fpg.size());
         U.o_end = int(out.fo.size());
-        pgmax = std::max(pgmax, U.pg_end - U.pg_begin);
+        // payoff groups held at once: those of one chunk of FUSED_GREG groups
+        for (int g0 = U.g_begin; g0 < U.g_end; g0 += FUSED_GREG) {
+            const int g1 = std::min(U.g_end, g0 + FUSED_GREG) - 1;
+            pgmax = std::max(pgmax, out.fg[g1].pg_end - out.fg[g0].pg_begin);
+        }
         gmax = std::max(gmax, U.g_end - U.g_begin);
     }
     // ---- A fragments: entry (d, lane, slot j) = G[d + m - 8h - j], G[0] = a, G[t>0] = Gamma_t
@@ -179,6 +184,9 @@ int plan_fused_request(const Request& req, const Layout& lay, int mode, FusedHos
     a.pgmax = pgmax;
     a.out_size = int(lay.options.size());
     a.acc_in_lds = a.out_size > FUSED_MAX_REG_OPTS ? 1 : 0;
+    // lanes per option in the payoff stage: the largest power of two S <= 32 with S*options <= 256
+    a.log2slices = 0;
+    while (!a.acc_in_lds && a.log2slices < 5 && (a.out_size << (a.log2slices + 1)) <= 256) ++a.log2slices;
     a.M = req.M;
     a.ntiles = (req.M + a.TP - 1) / a.TP;
     a.tiles_per_item = std::max(1, 1024 / a.TP);
