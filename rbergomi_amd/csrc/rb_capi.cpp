@@ -439,8 +439,7 @@ int rb_probe_integrals(double xi, const double* H, const double* eta, const doub
         hipStream_t st = p.stream;
         hipError_t e = hipMemcpyAsync(p.d_Z, Z, size_t(M) * p.nstreams * N * sizeof(double),
                                       hipMemcpyHostToDevice, st);
-        if (!e) e = launch_volterra_f64(p.d_Z, long(p.nstreams) * N, M, N, int(p.lay.units.size()),
-                                        p.d_units, p.d_gamma, p.d_Wt, st);
+        if (!e) e = p.conv(M, long(p.nstreams) * N);
         if (!e) e = launch_scan_f64(r.method, p.d_Z, long(p.nstreams) * N, p.d_Wt, M, N, p.d_groups,
                                     int(p.lay.groups.size()), p.d_comp, p.d_ints, st);
         if (!e) e = hipMemcpyAsync(ints.data(), p.d_ints, ints.size() * sizeof(double),

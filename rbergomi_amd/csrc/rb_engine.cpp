@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <complex>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -199,8 +200,8 @@ Plan::~Plan() {
     if (stream) (void)hipStreamSynchronize(stream);
     fused_batch.reset();  // stream-ordered frees on this plan's stream
     if (stream) (void)hipStreamSynchronize(stream);
-    void* bufs[] = {d_units, d_gamma, d_groups, d_comp, d_opts, d_Z,   d_Wt,
-                    d_ints,  d_partial, d_price, d_stat, d_iv, d_payoff};
+    void* bufs[] = {d_units, d_gamma, d_groups, d_comp, d_opts, d_Z,    d_Wt,  d_ints,
+                    d_partial, d_price, d_stat, d_iv, d_payoff, d_tw, d_ghat};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
@@ -233,7 +234,11 @@ int Plan::init(const Request& r, hipStream_t st, bool quadrature, std::string& e
         RB_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         own_stream = true;
     }
-    if (r.precision == PREC_SPLIT_FP16 || r.precision == PREC_SPLIT_BF16) {
+    const bool split = r.precision == PREC_SPLIT_FP16 || r.precision == PREC_SPLIT_BF16;
+    if (r.precision == PREC_FP32 || (split && (r.N > 1024 || r.method != METHOD_RT || quadrature))) {
+        // staged pipeline with the FP32 FFT convolution (long paths, plain MC, user Gaussians)
+        conv_fft = true;
+    } else if (split) {
         if (quadrature) {
             err = "user-supplied Gaussians are priced on the FP64 path only";
             return ST_ERR_ARG;
@@ -242,7 +247,7 @@ int Plan::init(const Request& r, hipStream_t st, bool quadrature, std::string& e
         fused_mode = r.precision == PREC_SPLIT_FP16 ? 0 : 1;
         return init_fused(err);
     }
-    if (r.precision != PREC_FP64) {
+    if (r.precision != PREC_FP64 && !conv_fft) {
         err = "precision not available in this build";
         return ST_ERR_ARG;
     }
@@ -268,6 +273,10 @@ int Plan::init(const Request& r, hipStream_t st, bool quadrature, std::string& e
     RB_TRY(upload(&d_groups, lay.groups, stream));
     RB_TRY(upload(&d_comp, lay.comp, stream));
     RB_TRY(upload(&d_opts, lay.options, stream));
+    if (conv_fft) {
+        rc = init_fft(err);
+        if (rc) return rc;
+    }
     RB_TRY(dalloc(&d_Z, size_t(chunk) * nstreams * N));
     RB_TRY(dalloc(&d_Wt, size_t(nunits) * chunk * N));
     RB_TRY(dalloc(&d_ints, size_t(ngroups) * chunk * 2));
@@ -335,8 +344,7 @@ int Plan::run(std::string& err) {
     for (long p0 = 0; p0 < req.M; p0 += chunk) {
         const long np = std::min(chunk, req.M - p0);
         RB_LAUNCH(K_NORMALS, launch_normals_f64(key, p0, np, nstreams, N, d_Z, stream));
-        RB_LAUNCH(K_VOLTERRA, launch_volterra_f64(d_Z, long(nstreams) * N, np, N, int(lay.units.size()),
-                                                  d_units, d_gamma, d_Wt, stream));
+        RB_LAUNCH(K_VOLTERRA, conv(np, long(nstreams) * N));
         RB_LAUNCH(K_SCAN, launch_scan_f64(req.method, d_Z, long(nstreams) * N, d_Wt, np, N, d_groups,
                                           int(lay.groups.size()), d_comp, d_ints, stream));
         RB_LAUNCH(K_PAYOFF, launch_payoff_f64(d_ints, np, p0, d_opts, nopt, req.method, p0 / 256, nblk,
@@ -357,8 +365,7 @@ int Plan::run_samples(const double* Z_host, double* payoff_host, std::string& er
         const long np = std::min(chunk, req.M - p0);
         RB_TRY(hipMemcpyAsync(d_Z, Z_host + size_t(p0) * nstreams * N,
                               size_t(np) * nstreams * N * sizeof(double), hipMemcpyHostToDevice, stream));
-        RB_LAUNCH(K_VOLTERRA, launch_volterra_f64(d_Z, long(nstreams) * N, np, N, int(lay.units.size()),
-                                                  d_units, d_gamma, d_Wt, stream));
+        RB_LAUNCH(K_VOLTERRA, conv(np, long(nstreams) * N));
         RB_LAUNCH(K_SCAN, launch_scan_f64(req.method, d_Z, long(nstreams) * N, d_Wt, np, N, d_groups,
                                           int(lay.groups.size()), d_comp, d_ints, stream));
         RB_LAUNCH(K_PAYOFF, launch_payoff_f64(d_ints, np, 0, d_opts, nopt, req.method, p0 / 256, nblk,
@@ -372,6 +379,64 @@ int Plan::run_samples(const double* Z_host, double* payoff_host, std::string& er
     RB_TRY(hipStreamSynchronize(stream));
     end_timing();
     return ST_OK;
+}
+
+// FFT in FP64 (iterative radix-2, sign -1 forward) for the host-side table FFT(Gamma)
+static void fft_host(std::vector<std::complex<double>>& a, int sign) {
+    const int n = int(a.size());
+    for (int i = 1, j = 0; i < n; ++i) {
+        int bit = n >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j) std::swap(a[i], a[j]);
+    }
+    for (int len = 2; len <= n; len <<= 1) {
+        for (int i = 0; i < n; i += len)
+            for (int k = 0; k < len / 2; ++k) {
+                const double ang = sign * 2.0 * M_PI * k / len;
+                const std::complex<double> w(std::cos(ang), std::sin(ang));
+                const std::complex<double> u = a[i + k], v = a[i + k + len / 2] * w;
+                a[i + k] = u + v;
+                a[i + k + len / 2] = u - v;
+            }
+    }
+}
+
+int Plan::init_fft(std::string& err) {
+    const int N = req.N;
+    log2L = 0;
+    while ((1 << log2L) < 2 * N - 1) ++log2L;
+    if (log2L < 1) log2L = 1;
+    const int L = 1 << log2L;
+    hipDeviceProp_t prop;
+    RB_TRY(hipGetDeviceProperties(&prop, device));
+    if (volterra_fft_lds_bytes(log2L) > prop.sharedMemPerBlock) {
+        err = "N too large for the LDS FFT";
+        return ST_ERR_LIMIT;
+    }
+    // twiddles computed in FP64, stored in FP32
+    std::vector<float2> tw(L / 2);
+    for (int k = 0; k < L / 2; ++k) {
+        const double ang = -2.0 * M_PI * double(k) / double(L);
+        tw[k] = make_float2(float(std::cos(ang)), float(std::sin(ang)));
+    }
+    std::vector<float2> gh(lay.units.size() * size_t(L));
+    for (size_t u = 0; u < lay.units.size(); ++u) {
+        std::vector<std::complex<double>> a(L, 0.0);
+        for (int k = 0; k < N; ++k) a[k] = lay.gamma[u * N + k];
+        fft_host(a, -1);
+        for (int k = 0; k < L; ++k) gh[u * L + k] = make_float2(float(a[k].real()), float(a[k].imag()));
+    }
+    RB_TRY(upload(&d_tw, tw, stream));
+    RB_TRY(upload(&d_ghat, gh, stream));
+    return ST_OK;
+}
+
+hipError_t Plan::conv(long np, long zstride) {
+    if (conv_fft)
+        return launch_volterra_fft(d_Z, zstride, np, req.N, log2L, int(lay.units.size()), d_units, d_ghat,
+                                   d_tw, d_Wt, stream);
+    return launch_volterra_f64(d_Z, zstride, np, req.N, int(lay.units.size()), d_units, d_gamma, d_Wt, stream);
 }
 
 int Plan::results(double* price, double* iv, double* stat, std::string& err) {

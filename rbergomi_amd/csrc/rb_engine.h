@@ -86,6 +86,11 @@ struct Plan {
     double* d_stat = nullptr;
     double* d_iv = nullptr;
     double* d_payoff = nullptr;   // [chunk][options], quadrature mode only
+    // long-path FP32 FFT convolution (replaces k_volterra_f64 in the staged pipeline)
+    bool conv_fft = false;
+    int log2L = 0;
+    float2* d_tw = nullptr;       // exp(-2 pi i k / L), k < L/2
+    float2* d_ghat = nullptr;     // [units][L] FFT of the zero-padded Gamma
     // fused mixed-precision path (a batch of one request)
     bool fused = false;
     int fused_mode = 0;           // 0 = FP16 split, 1 = BF16 split
@@ -104,6 +109,8 @@ struct Plan {
     int run_samples(const double* Z_host, double* payoff_host, std::string& err);
     int results(double* price, double* iv, double* stat, std::string& err);
     int init_fused(std::string& err);
+    int init_fft(std::string& err);
+    hipError_t conv(long np, long zstride);
     int run_fused(std::string& err);
     void begin_timing();
     void end_timing();

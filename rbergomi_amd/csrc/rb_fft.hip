@@ -1,0 +1,107 @@
+// rb_fft.hip -- Volterra convolution by FFT in LDS, for long paths (FP32 data).
+//
+// Replaces the length-(2N-1) FFTW convolution of updateWtilde (rBergLib/rBergomiMT.cpp:83-109)
+// for N > 1024, where the Toeplitz GEMM's W1 tile no longer fits LDS.  One workgroup convolves
+// two paths at once: z = W1_a + i W1_b, zero-padded to L = 2^k >= 2N-1.  Because Gamma is real,
+// IFFT(FFT(z) * FFT(Gamma)) = (Gamma * W1_a) + i (Gamma * W1_b), so no separation step is
+// needed.  Forward transform: radix-2 decimation in time from bit-reversed input to natural
+// order; pointwise product with the per-H table FFT(Gamma) (computed in FP64 on the host);
+// inverse: radix-2 decimation in frequency from natural order to bit-reversed output, which the
+// epilogue reads back at bit-reversed indices.  Twiddles exp(-2 pi i k / L) come from an FP64
+// table rounded to FP32.  Epilogue in FP64: Wtilde = sc (Y + a W1 + b W1perp).
+#include <hip/hip_runtime.h>
+
+#include "rb_kernels.h"
+#include "rb_types.h"
+
+namespace rb {
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// Grid: (ceil(np/2), units).  Block 256.  LDS: L float2.
+__global__ void __launch_bounds__(256)
+k_volterra_fft(const double* __restrict__ Z, long zstride, long np, int N, int log2L,
+               const Unit* __restrict__ units, const float2* __restrict__ ghat,
+               const float2* __restrict__ tw, double* __restrict__ Wt) {
+    extern __shared__ float2 z[];
+    const int L = 1 << log2L;
+    const int u = blockIdx.y;
+    const long pa = 2L * blockIdx.x, pb = pa + 1;
+    const bool hasb = pb < np;
+    const double* Wa = Z + pa * zstride;
+    const double* Wb = Z + pb * zstride;
+    // load in bit-reversed order, zero padding past N
+    for (int i = threadIdx.x; i < L; i += 256) {
+        const float re = i < N ? float(Wa[i]) : 0.f;
+        const float im = (i < N && hasb) ? float(Wb[i]) : 0.f;
+        z[__brev(unsigned(i)) >> (32 - log2L)] = make_float2(re, im);
+    }
+    __syncthreads();
+    // forward DIT
+    for (int s = 0; s < log2L; ++s) {
+        const int span = 1 << s;
+        const int tstride = L >> (s + 1);  // twiddle index step
+        for (int b = threadIdx.x; b < (L >> 1); b += 256) {
+            const int pos = b & (span - 1);
+            const int i = ((b >> s) << (s + 1)) + pos;
+            const float2 w = tw[pos * tstride];
+            const float2 t = cmul(w, z[i + span]);
+            const float2 x = z[i];
+            z[i] = make_float2(x.x + t.x, x.y + t.y);
+            z[i + span] = make_float2(x.x - t.x, x.y - t.y);
+        }
+        __syncthreads();
+    }
+    // pointwise product with FFT(Gamma) of this unit
+    const float2* G = ghat + long(u) * L;
+    for (int k = threadIdx.x; k < L; k += 256) z[k] = cmul(z[k], G[k]);
+    __syncthreads();
+    // inverse DIF with conjugate twiddles
+    for (int s = log2L - 1; s >= 0; --s) {
+        const int span = 1 << s;
+        const int tstride = L >> (s + 1);
+        for (int b = threadIdx.x; b < (L >> 1); b += 256) {
+            const int pos = b & (span - 1);
+            const int i = ((b >> s) << (s + 1)) + pos;
+            const float2 w0 = tw[pos * tstride];
+            const float2 w = make_float2(w0.x, -w0.y);
+            const float2 x = z[i], y = z[i + span];
+            z[i] = make_float2(x.x + y.x, x.y + y.y);
+            z[i + span] = cmul(make_float2(x.x - y.x, x.y - y.y), w);
+        }
+        __syncthreads();
+    }
+    const Unit un = units[u];
+    const double invL = 1.0 / double(L);
+    for (int i = threadIdx.x; i < N; i += 256) {
+        const float2 y = z[__brev(unsigned(i)) >> (32 - log2L)];
+        {
+            const double Y = double(y.x) * invL;
+            Wt[(long(u) * np + pa) * N + i] = un.sc * Y + un.sc * (un.a * Wa[i] + un.b * Wa[N + i]);
+        }
+        if (hasb) {
+            const double Y = double(y.y) * invL;
+            Wt[(long(u) * np + pb) * N + i] = un.sc * Y + un.sc * (un.a * Wb[i] + un.b * Wb[N + i]);
+        }
+    }
+}
+
+size_t volterra_fft_lds_bytes(int log2L) { return (size_t(1) << log2L) * sizeof(float2); }
+
+hipError_t launch_volterra_fft(const double* Z, long zstride, long np, int N, int log2L, int nunits,
+                               const Unit* units, const float2* ghat, const float2* tw, double* Wt,
+                               hipStream_t st) {
+    const size_t lds = volterra_fft_lds_bytes(log2L);
+    if (lds > 65536) {
+        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_volterra_fft),
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+        if (e != hipSuccess) return e;
+    }
+    dim3 grid(unsigned((np + 1) / 2), unsigned(nunits));
+    hipLaunchKernelGGL(k_volterra_fft, grid, dim3(256), lds, st, Z, zstride, np, N, log2L, units, ghat, tw, Wt);
+    return hipGetLastError();
+}
+
+}  // namespace rb

@@ -38,4 +38,10 @@ hipError_t launch_probe_payoff(const double* lnS, const double* sig, const doubl
 
 hipError_t launch_probe_copy(const void* src, void* dst, long bytes, hipStream_t st);
 
+// FFT convolution for long paths (rb_fft.hip)
+size_t volterra_fft_lds_bytes(int log2L);
+hipError_t launch_volterra_fft(const double* Z, long zstride, long np, int N, int log2L, int nunits,
+                               const Unit* units, const float2* ghat, const float2* tw, double* Wt,
+                               hipStream_t st);
+
 }  // namespace rb

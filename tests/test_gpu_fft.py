@@ -1,0 +1,52 @@
+"""GPU checks of the FP32 LDS-FFT Volterra convolution (long paths): per-path integrals against
+the CPU oracle, and prices against the FP64 path on the same Philox counters."""
+import ctypes
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def rb():
+    import rbergomi_amd
+    rbergomi_amd.load()
+    return rbergomi_amd
+
+
+@pytest.mark.parametrize("N", [7, 100, 1100, 2048, 3000, 4096, 8192])
+def test_fft_integrals_match_oracle(rb, orc, N):
+    from rbergomi_amd._lib import RbConfig
+    L = rb.load()
+    H, eta, rho = [0.02, 0.1], [2.3, 1.5], [-0.9, -0.7]
+    T, K = [1.0, 0.25], [1.0, 0.9]
+    xi = 0.03
+    M = 6 if N >= 4096 else 11
+    Z = np.random.default_rng(N).standard_normal((M, 2, N))
+    a = lambda x: np.ascontiguousarray(x, dtype=np.float64)
+    dp = ctypes.POINTER(ctypes.c_double)
+    arrs = [a(x) for x in (H, eta, rho, T, K)]
+    out = np.zeros((M, 4, 2))
+    cfg = RbConfig(rb.PREC_FP32, rb.COMP_REF, rb.METHOD_RT, 0)
+    rc = L.rb_probe_integrals(xi, arrs[0].ctypes.data_as(dp), arrs[1].ctypes.data_as(dp), arrs[2].ctypes.data_as(dp),
+                              2, arrs[3].ctypes.data_as(dp), arrs[4].ctypes.data_as(dp), 2, N, M,
+                              Z.ctypes.data_as(dp), ctypes.byref(cfg), out.ctypes.data_as(dp))
+    assert rc == 0, L.rb_last_error()
+    ref = orc.integrals_rt_samples(xi, H, eta, rho, T, K, N, Z)
+    # FP32 transform of length L ~ 2N: |dY| ~ 1e-6 |Y|; Iv relative, Is against its scale
+    iv_err = np.max(np.abs(out[..., 0] - ref[..., 0]) / ref[..., 0])
+    is_err = np.max(np.abs(out[..., 1] - ref[..., 1]) / np.sqrt(ref[..., 0]))
+    print("N=%d Iv rel %.2e  Is/sqrt(Iv) %.2e" % (N, iv_err, is_err))
+    assert iv_err < 2e-6 and is_err < 2e-6
+
+
+@pytest.mark.parametrize("N", [1500, 4096])
+def test_fft_prices_match_fp64(rb, N):
+    K = np.linspace(0.7, 1.3, 31)
+    T = [1.0] * 31
+    p64, _, s64 = rb.price_v2(0.03, [0.05], [2.3], [-0.9], T, K, N, 3000, [8], precision=rb.PREC_FP64)
+    p32, _, s32 = rb.price_v2(0.03, [0.05], [2.3], [-0.9], T, K, N, 3000, [8], precision=rb.PREC_FP32)
+    err = np.max(np.abs(p32 - p64))
+    print("N=%d max |price fp32-fft - fp64| = %.2e" % (N, err))
+    assert err <= 1e-4

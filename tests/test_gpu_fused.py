@@ -87,14 +87,17 @@ def test_fused_reruns_are_bit_identical(rb):
         assert np.array_equal(x, y, equal_nan=True)
 
 
-def test_fused_limits_are_reported(rb):
+def test_split_precision_routes_outside_the_fused_kernel(rb):
+    # N > 1024 and plain Monte Carlo run the staged pipeline with the FP32 FFT convolution
+    T, K = [0.5, 1.0], [1.0, 1.1]
+    for N, method in ((2048, rb.METHOD_RT), (64, rb.METHOD_MC)):
+        p64, _, s64 = rb.price_v2(0.04, [0.1], [1.5], [-0.7], T, K, N, 2000, [1], method=method)
+        p16, _, s16 = rb.price_v2(0.04, [0.1], [1.5], [-0.7], T, K, N, 2000, [1], method=method,
+                                  precision=rb.PREC_SPLIT_FP16)
+        assert np.max(np.abs(p16 - p64)) <= 1e-4
     with pytest.raises(rb.RbError) as e:
-        rb.price_v2(0.04, [0.1], [1.5], [-0.7], [1.0], [1.0], 2048, 100, [1], precision=rb.PREC_SPLIT_FP16)
+        rb.price_v2(0.04, [0.1], [1.5], [-0.7], [1.0], [1.0], 9000, 100, [1], precision=rb.PREC_SPLIT_FP16)
     assert e.value.code == 5
-    with pytest.raises(rb.RbError) as e:
-        rb.price_v2(0.04, [0.1], [1.5], [-0.7], [1.0], [1.0], 64, 100, [1], precision=rb.PREC_SPLIT_FP16,
-                    method=rb.METHOD_MC)
-    assert e.value.code == 1
 
 
 def test_mixed_payoff_and_mills_ratio(rb):
