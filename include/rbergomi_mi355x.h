@@ -114,6 +114,9 @@ int rb_plan_create(double xi, const double* H, const double* eta, const double* 
                    rb_plan** out);
 int rb_plan_run(rb_plan* plan);
 int rb_plan_results(rb_plan* plan, double* price, double* iv, double* stat);
+/* Graph replay (enable = 1): the next rb_plan_run records the plan's launches into a hipGraph
+ * and every run replays it (no per-launch host overhead; kernel timing is off in this mode). */
+int rb_plan_set_graph(rb_plan* plan, int enable);
 /* Kernel timing of the following runs with HIP events on the plan's stream (enable = 1). */
 int rb_plan_set_profiling(rb_plan* plan, int enable);
 /* Per-kernel totals of the last run: names (NUL-terminated, 32 bytes each), total ms and

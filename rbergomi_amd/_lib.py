@@ -61,6 +61,7 @@ def load():
     L.rb_plan_run.argtypes = [vp]
     L.rb_plan_results.argtypes = [vp, dp, dp, dp]
     L.rb_plan_set_profiling.argtypes = [vp, i]
+    L.rb_plan_set_graph.argtypes = [vp, i]
     L.rb_plan_kernel_times.argtypes = [vp, ctypes.c_char_p, dp, ip, i, ip]
     L.rb_plan_info.argtypes = [vp, ip, ip, ip, lp]
     L.rb_plan_destroy.argtypes = [vp]

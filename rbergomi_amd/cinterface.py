@@ -115,10 +115,16 @@ def payoff_samples_v2(xi, H, eta, rho, T, K, N, Z, method=_lib.METHOD_RT,
 
 
 class Plan:
-    """Device-resident pricing plan (rb_plan_*): build once, run many times."""
+    """Device-resident pricing plan (rb_plan_*): build once, run many times.
+
+    stream: a HIP stream handle (int) to run on, e.g. torch.cuda.Stream().cuda_stream when torch
+    was imported before this library (both then use one HIP runtime); None = the plan's own.
+    graph: record the launches into a hipGraph on the first run and replay it afterwards.
+    """
 
     def __init__(self, xi, H, eta, rho, T, K, N, M, seed, precision=_lib.PREC_FP64,
-                 compensator=_lib.COMP_REF, method=_lib.METHOD_RT, profile=False):
+                 compensator=_lib.COMP_REF, method=_lib.METHOD_RT, profile=False, graph=False,
+                 stream=None):
         self._keep = [_arr(H), _arr(eta), _arr(rho), _arr(T), _arr(K), _seed(seed)]
         (H, pH), (eta, pe), (rho, pr), (T, pT), (K, pK), (s, ps) = self._keep
         self.out_size = len(H) * len(T)
@@ -126,9 +132,11 @@ class Plan:
         self.ptr = ctypes.c_void_p()
         L = _lib.load()
         check(L.rb_plan_create(float(xi), pH, pe, pr, len(H), pT, pK, len(T), int(N), int(M), ps,
-                               len(s), ctypes.byref(self.cfg), None, ctypes.byref(self.ptr)))
+                               len(s), ctypes.byref(self.cfg), stream, ctypes.byref(self.ptr)))
         if profile:
             check(L.rb_plan_set_profiling(self.ptr, 1))
+        if graph:
+            check(L.rb_plan_set_graph(self.ptr, 1))
 
     def run(self):
         check(_lib.load().rb_plan_run(self.ptr))

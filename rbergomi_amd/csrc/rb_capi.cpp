@@ -245,6 +245,12 @@ int rb_plan_set_profiling(rb_plan* plan, int enable) {
     return RB_OK;
 }
 
+int rb_plan_set_graph(rb_plan* plan, int enable) {
+    if (!plan) return fail(RB_ERR_ARG, "null plan");
+    plan->plan.use_graph = enable != 0;
+    return RB_OK;
+}
+
 int rb_plan_kernel_times(rb_plan* plan, char* names, double* total_ms, int* launches, int max,
                          int* count) {
     if (!plan || !count) return fail(RB_ERR_ARG, "null argument");

@@ -198,6 +198,8 @@ static hipError_t upload(T** p, const std::vector<T>& v, hipStream_t st) {
 
 Plan::~Plan() {
     if (stream) (void)hipStreamSynchronize(stream);
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
     fused_batch.reset();  // stream-ordered frees on this plan's stream
     if (stream) (void)hipStreamSynchronize(stream);
     void* bufs[] = {d_units, d_gamma, d_groups, d_comp, d_opts, d_Z,    d_Wt,  d_ints,
@@ -297,7 +299,7 @@ void Plan::begin_timing() {
 }
 
 int Plan::timed(int id) {
-    if (!profile) return -1;
+    if (!profile || capturing) return -1;
     while (ev_pool.size() < ev_used + 2) {
         hipEvent_t e;
         if (hipEventCreate(&e) != hipSuccess) return -1;
@@ -335,12 +337,12 @@ void Plan::end_timing() {
         mark(slot_, nullptr, nullptr);       \
     } while (0)
 
-int Plan::run(std::string& err) {
-    if (fused) return run_fused(err);
+// Enqueue one pricing run on the plan's stream (no synchronization).
+int Plan::enqueue(std::string& err) {
+    if (fused) return fused_batch->launch(stream, err, capturing ? nullptr : this);
     const int N = req.N;
     const int nopt = int(lay.options.size());
     const PhiloxKey key = philox_key(req.seed.data(), int(req.seed.size()));
-    begin_timing();
     for (long p0 = 0; p0 < req.M; p0 += chunk) {
         const long np = std::min(chunk, req.M - p0);
         RB_LAUNCH(K_NORMALS, launch_normals_f64(key, p0, np, nstreams, N, d_Z, stream));
@@ -352,6 +354,42 @@ int Plan::run(std::string& err) {
     }
     RB_LAUNCH(K_REDUCE, launch_reduce_f64(d_partial, nblk, req.M, nopt, d_price, d_stat, stream));
     RB_LAUNCH(K_IV, launch_iv_f64(d_price, d_opts, nopt, d_iv, stream));
+    return ST_OK;
+}
+
+int Plan::run(std::string& err) {
+    if (use_graph) {
+        // first run: record the launches into a graph; every run replays it
+        if (!gexec) {
+            // one eager run first: kernel attributes are set outside the capture, and the
+            // captured graph then replays the same launches
+            RB_TRY(hipStreamSynchronize(stream));
+            {
+                const int rc = enqueue(err);
+                if (rc) return rc;
+            }
+            RB_TRY(hipStreamSynchronize(stream));
+            capturing = true;
+            RB_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+            const int rc = enqueue(err);
+            hipGraph_t g = nullptr;
+            const hipError_t e = hipStreamEndCapture(stream, &g);
+            capturing = false;
+            if (rc) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            RB_TRY(e);
+            graph = g;
+            RB_TRY(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+        }
+        RB_TRY(hipGraphLaunch(gexec, stream));
+        RB_TRY(hipStreamSynchronize(stream));
+        return ST_OK;
+    }
+    begin_timing();
+    const int rc = enqueue(err);
+    if (rc) return rc;
     RB_TRY(hipStreamSynchronize(stream));
     end_timing();
     return ST_OK;

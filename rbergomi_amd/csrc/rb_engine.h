@@ -96,6 +96,11 @@ struct Plan {
     int fused_mode = 0;           // 0 = FP16 split, 1 = BF16 split
     std::unique_ptr<FusedHostReq> fused_req;
     std::unique_ptr<FusedBatch> fused_batch;
+    // graph replay: the first run is captured into a hipGraph, later runs relaunch it
+    bool use_graph = false;
+    bool capturing = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
     std::vector<KernelTime> times;
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
@@ -111,7 +116,8 @@ struct Plan {
     int init_fused(std::string& err);
     int init_fft(std::string& err);
     hipError_t conv(long np, long zstride);
-    int run_fused(std::string& err);
+    // enqueue one run on the plan's stream, no synchronization
+    int enqueue(std::string& err);
     void begin_timing();
     void end_timing();
     int timed(int id);

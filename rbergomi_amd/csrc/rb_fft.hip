@@ -95,8 +95,7 @@ hipError_t launch_volterra_fft(const double* Z, long zstride, long np, int N, in
                                hipStream_t st) {
     const size_t lds = volterra_fft_lds_bytes(log2L);
     if (lds > 65536) {
-        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_volterra_fft),
-                                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+        const hipError_t e = allow_lds(reinterpret_cast<const void*>(&k_volterra_fft), lds);
         if (e != hipSuccess) return e;
     }
     dim3 grid(unsigned((np + 1) / 2), unsigned(nunits));

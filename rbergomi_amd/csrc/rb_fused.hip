@@ -362,8 +362,7 @@ template <int MODE, int UPW>
 static hipError_t launch_t(const FusedBatchArgs& a, int grid, hipStream_t st) {
     const size_t lds = fused_lds_bytes(a);
     if (lds > 65536) {
-        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fused<MODE, UPW>),
-                                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+        const hipError_t e = allow_lds(reinterpret_cast<const void*>(&k_fused<MODE, UPW>), lds);
         if (e != hipSuccess) return e;
     }
     hipLaunchKernelGGL((k_fused<MODE, UPW>), dim3(grid), dim3(256), lds, st, a);

@@ -425,13 +425,4 @@ int Plan::init_fused(std::string& err) {
     return ST_OK;
 }
 
-int Plan::run_fused(std::string& err) {
-    begin_timing();
-    int rc = fused_batch->launch(stream, err, this);
-    if (rc) return rc;
-    RB_TRY(hipStreamSynchronize(stream));
-    end_timing();
-    return ST_OK;
-}
-
 }  // namespace rb

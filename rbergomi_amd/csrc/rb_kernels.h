@@ -8,6 +8,9 @@
 
 namespace rb {
 
+// raise a kernel's dynamic-LDS limit (once per device and size)
+hipError_t allow_lds(const void* fn, size_t bytes);
+
 // staged FP64 path (rb_staged.hip)
 hipError_t launch_normals_f64(PhiloxKey key, long p0, long np, int nstreams, int N, double* out,
                               hipStream_t st);

@@ -15,11 +15,30 @@
 //   k_iv_f64         implied vols                                    (IV_call, :421-423)
 #include <hip/hip_runtime.h>
 
+#include <map>
+#include <mutex>
+
 #include "rb_device.h"
 #include "rb_kernels.h"
 #include "rb_types.h"
 
 namespace rb {
+
+// Raise a kernel's dynamic-LDS limit once per device (later calls with the same or a smaller size
+// make no runtime call, so launches can be recorded into a graph after the first eager run).
+hipError_t allow_lds(const void* fn, size_t bytes) {
+    static std::mutex mu;
+    static std::map<std::pair<int, const void*>, size_t> done;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    std::lock_guard<std::mutex> lock(mu);
+    size_t& have = done[{dev, fn}];
+    if (have >= bytes) return hipSuccess;
+    e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes));
+    if (e == hipSuccess) have = bytes;
+    return e;
+}
 
 // ---------------------------------------------------------------- Gaussians
 // One thread per (path, stream, 4-step block).  out[(p*nstreams + s)*N + j]
@@ -328,8 +347,7 @@ hipError_t launch_volterra_f64(const double* Z, long zstride, long np, int N, in
     const int ppb = 256 / tpp;
     const size_t lds = volterra_f64_lds_bytes(N);
     if (lds > 65536) {
-        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_volterra_f64),
-                                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+        const hipError_t e = allow_lds(reinterpret_cast<const void*>(&k_volterra_f64), lds);
         if (e != hipSuccess) return e;
     }
     dim3 grid(unsigned((np + ppb - 1) / ppb), unsigned(nunits));
