@@ -237,8 +237,8 @@ int Plan::init(const Request& r, hipStream_t st, bool quadrature, std::string& e
         own_stream = true;
     }
     const bool split = r.precision == PREC_SPLIT_FP16 || r.precision == PREC_SPLIT_BF16;
-    if (r.precision == PREC_FP32 || (split && (r.N > 1024 || r.method != METHOD_RT || quadrature))) {
-        // staged pipeline with the FP32 FFT convolution (long paths, plain MC, user Gaussians)
+    if (r.precision == PREC_FP32 || (split && (r.N > 1024 || quadrature))) {
+        // staged pipeline with the FP32 FFT convolution (long paths, user Gaussians)
         conv_fft = true;
     } else if (split) {
         if (quadrature) {

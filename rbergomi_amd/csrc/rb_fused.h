@@ -9,6 +9,7 @@ namespace rb {
 
 constexpr int FUSED_MAX_UPW = 8;   // output row blocks (32 rows) per wave
 constexpr int FUSED_GREG = 8;      // groups integrated per pass over the accumulators
+constexpr int FUSED_GREG_MC = 4;   // the same for plain Monte Carlo (three sums per group)
 constexpr int FUSED_MAX_REG_OPTS = 256;  // options whose sums live in registers
 
 struct FusedUnit {
@@ -47,7 +48,8 @@ struct FusedOption {
     double lnK;
 };
 
-// per (path, payoff group) values shared by all strikes
+// per (path, payoff group) values shared by all strikes (plain Monte Carlo: lnS = ln S_T and
+// sig holds S_T itself)
 struct PathPG {
     double lnS;      // ln S' = -rho^2 Iv / 2 + rho Is
     double sig;      // sqrt((1 - rho^2) Iv)
@@ -65,6 +67,7 @@ struct FusedReq {
     int nunits, pgmax, out_size, gchunk, acc_in_lds;
     int log2slices;              // lanes per option in the payoff stage (register mode)
     int tiles_per_item;
+    int mc;                      // 1 = plain Monte Carlo (wpart holds 3 sums per entry), 0 = RT
     long M, ntiles, nitems;
     PhiloxKey key;
     const FusedUnit* units;
@@ -130,10 +133,19 @@ __device__ __forceinline__ double payoff_mixed(const PathPG& q, float lnKf, floa
     return double(x);
 }
 
+// plain Monte Carlo payoff (S_T - K)^+ in FP64 (S_T in the sig slot)
+__device__ __forceinline__ double payoff_mc(const PathPG& q, double K) {
+    const double x = q.sig - K;
+    return x > 0.0 ? x : 0.0;
+}
+
 size_t fused_req_lds_bytes(const FusedReq& a);
 size_t fused_lds_bytes(const FusedBatchArgs& b);
 // mode: 0 = FP16 hi+lo split, 1 = BF16 hi+lo split
-hipError_t launch_fused(int mode, int upw, const FusedBatchArgs& b, int grid, hipStream_t st);
+// mc: plain Monte Carlo kernel (every request of the batch), else Romano-Touzi
+hipError_t launch_fused(int mode, int upw, bool mc, const FusedBatchArgs& b, int grid, hipStream_t st);
+hipError_t launch_fused_rt(int mode, int upw, const FusedBatchArgs& b, int grid, hipStream_t st);
+hipError_t launch_fused_mc(int mode, int upw, const FusedBatchArgs& b, int grid, hipStream_t st);
 hipError_t launch_zero_int(int* p, int n, hipStream_t st);
 
 }  // namespace rb

@@ -53,10 +53,6 @@ static void split16(double x, int mode, uint16_t& hi, uint16_t& lo) {
 int plan_fused_request(const Request& req, const Layout& lay, int mode, FusedHostReq& out,
                        std::string& err) {
     const int N = req.N;
-    if (req.method != METHOD_RT) {
-        err = "the split-precision kernel prices the Romano-Touzi estimator only";
-        return ST_ERR_ARG;
-    }
     const int Np = (N + 31) / 32 * 32;
     if (Np > 1024) {
         err = "the split-precision kernel covers N <= 1024";
@@ -187,6 +183,7 @@ int plan_fused_request(const Request& req, const Layout& lay, int mode, FusedHos
     // lanes per option in the payoff stage: the largest power of two S <= 32 with S*options <= 256
     a.log2slices = 0;
     while (!a.acc_in_lds && a.log2slices < 5 && (a.out_size << (a.log2slices + 1)) <= 256) ++a.log2slices;
+    a.mc = req.method == METHOD_MC ? 1 : 0;
     a.M = req.M;
     a.ntiles = (req.M + a.TP - 1) / a.TP;
     a.tiles_per_item = std::max(1, 1024 / a.TP);
@@ -197,7 +194,7 @@ int plan_fused_request(const Request& req, const Layout& lay, int mode, FusedHos
     out.ivopts = lay.options;
     // cost of one item (relative): RNG + integrals ~ N * groups, GEMM ~ N^2 / 64, payoffs ~ options
     out.item_cost = double(a.tiles_per_item * a.TP) *
-                    (double(N) * (2.0 + gmax) + double(Np) * Np / 64.0 * nu + 4.0 * a.out_size);
+                    (double(N) * (2.0 + 2.0 * a.mc + gmax) + double(Np) * Np / 64.0 * nu + 4.0 * a.out_size);
     return ST_OK;
 }
 
@@ -314,11 +311,16 @@ int FusedBatch::build(const std::vector<const FusedHostReq*>& reqs, int mode_, i
         out_off[q + 1] = out_off[q] + r.out_size;
         ivo.insert(ivo.end(), H.ivopts.begin(), H.ivopts.end());
         bf = std::max(bf, r.PB * r.NK * 128);
-        wp = std::max(wp, 4 * r.gchunk * 32 * 2);
+        if (q > 0 && r.mc != R[0].mc) {
+            err = "a batch holds requests of one estimator (Romano-Touzi or plain Monte Carlo)";
+            return ST_ERR_ARG;
+        }
+        wp = std::max(wp, 4 * r.gchunk * 32 * (r.mc ? 3 : 2));
         pg = std::max(pg, r.TP * r.pgmax);
         ac = std::max(ac, r.acc_in_lds ? 2 * r.out_size : 0);
         R.push_back(r);
     }
+    mc = nreq > 0 && R[0].mc != 0;
     total_out = int(out_off[nreq]);
     args.bf_u4 = bf;
     args.wpart_d = wp;
@@ -392,7 +394,7 @@ int FusedBatch::launch(hipStream_t st, std::string& err, Plan* timing) {
     int s;
     RB_TRY(launch_zero_int(d_counter, 1, st));
     s = timed(K_FUSED);
-    RB_TRY(launch_fused(mode, upw, args, grid, st));
+    RB_TRY(launch_fused(mode, upw, mc, args, grid, st));
     mark(s);
     s = timed(K_REDUCE);
     RB_TRY(launch_reduce_rows(d_partial, d_row_off, d_row_n, d_row_M, total_out, d_price, d_stat, st));

@@ -35,6 +35,7 @@ int plan_fused_request(const Request& req, const Layout& lay, int mode, FusedHos
 // Any number of requests priced by one launch of the fused kernel.
 struct FusedBatch {
     int mode = 0, upw = 1, grid = 0, nreq = 0, total_out = 0;
+    bool mc = false;  // plain Monte Carlo kernel
     hipStream_t stream = nullptr;  // the stream the buffers were allocated on (and are freed on)
     FusedBatchArgs args{};
     std::vector<long> out_off;   // first output of request q in the result arrays

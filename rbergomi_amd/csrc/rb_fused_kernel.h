@@ -1,0 +1,408 @@
+// rb_fused_kernel.h -- the fused kernel template, instantiated by rb_fused.hip (Romano-Touzi
+// estimator) and rb_fused_mc.hip (plain Monte Carlo); see rb_fused.hip for the stages.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "rb_fused.h"
+#include "rb_kernels.h"
+#include "rb_mfma.h"
+
+namespace rb {
+
+template <int MODE>
+struct SplitT;
+
+template <>
+struct SplitT<0> {
+    typedef _Float16 T;
+    typedef half8_t V;
+    typedef _Float16 V4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ T cvt(float x) { return (T)x; }
+    static __device__ __forceinline__ floatx16_t mma(V a, V b, floatx16_t c) {
+        return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+    }
+};
+
+template <>
+struct SplitT<1> {
+    typedef __bf16 T;
+    typedef bf16x8_t V;
+    typedef __bf16 V4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ T cvt(float x) { return (T)x; }
+    static __device__ __forceinline__ floatx16_t mma(V a, V b, floatx16_t c) {
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+    }
+};
+
+__device__ __forceinline__ double half_sum_f64(double v) {
+    // sum over the 32 lanes of one half-wave; every lane of the half receives the total
+#pragma unroll
+    for (int off = 16; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+template <int MODE, int UPW, bool MC>
+__global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs B) {
+    constexpr int GR = MC ? FUSED_GREG_MC : FUSED_GREG;  // groups per pass over the accumulators
+    constexpr int NS = MC ? 3 : 2;                        // sums per (path, group)
+    typedef SplitT<MODE> S;
+    typedef typename S::T T;
+    typedef typename S::V V;
+    typedef typename S::V4 V4;
+    extern __shared__ uint4 lds4[];
+    __shared__ int s_item;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int h = lane >> 5, col = lane & 31;
+    uint4* Bf = lds4;                                                   // [PB][NK][2][64]
+    double* wpart = reinterpret_cast<double*>(lds4 + B.bf_u4);          // [4][gchunk][32][2]
+    PathPG* pgd = reinterpret_cast<PathPG*>(wpart + B.wpart_d);         // [TP][pgmax]
+    double* accopt = reinterpret_cast<double*>(pgd + B.pgd_n);          // [out][2] in LDS mode
+
+    for (;;) {
+        // ---- next work item (dynamic schedule; the item's sums do not depend on who runs it)
+        if (tid == 0) s_item = atomicAdd(B.counter, 1);
+        __syncthreads();
+        const long gi = s_item;
+        __syncthreads();
+        if (gi >= B.total_items) break;
+        const FusedReq A = B.reqs[B.item_req[gi]];   // copied once per item (no aliasing reloads)
+        const long item = B.item_local[gi];
+        const int NK = A.NK;
+        const int pbw = A.wave_pb[wave];
+        const int nuw = A.wave_nu[wave];
+        double a1 = 0.0, a2 = 0.0;  // sums of option tid (register accumulation)
+        if (A.acc_in_lds)
+            for (int o = tid; o < 2 * A.out_size; o += 256) accopt[o] = 0.0;
+        const long t_begin = item * A.tiles_per_item;
+        const long t_end = t_begin + A.tiles_per_item < A.ntiles ? t_begin + A.tiles_per_item : A.ntiles;
+        for (long tile = t_begin; tile < t_end; ++tile) {
+            const long p0 = tile * A.TP;
+            // ---- 1. Gaussians W1 -> B fragments (hi, lo); zero past N and past M
+            const int nq = A.Np >> 2;
+            {
+                const int pl = tid & (A.TP - 1);   // TP is 32, 64 or 128
+                const int pb = pl >> 5;
+                const long p = p0 + pl;
+                const bool live = p < A.M;
+                for (int q = tid >> A.log2TP; q < nq; q += 256 >> A.log2TP) {
+                    float z[4] = {0.f, 0.f, 0.f, 0.f};
+                    if (live && 4 * q < A.N) {
+                        normals4_f32(A.key, uint64_t(p), 0u, uint32_t(q), z);
+#pragma unroll
+                        for (int j = 0; j < 4; ++j)
+                            if (4 * q + j >= A.N) z[j] = 0.f;
+                    }
+                    V4 hi, lo;
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        hi[j] = S::cvt(z[j]);
+                        lo[j] = S::cvt(z[j] - float(hi[j]));
+                    }
+                    const int ks = q >> 2, hh = (q & 3) >> 1, j0 = (q & 1);
+                    const int ln = 32 * hh + (pl & 31);
+                    uint4* base = Bf + ((pb * NK + ks) * 2) * 64;
+                    reinterpret_cast<uint2*>(base + ln)[j0] = __builtin_bit_cast(uint2, hi);
+                    reinterpret_cast<uint2*>(base + 64 + ln)[j0] = __builtin_bit_cast(uint2, lo);
+                }
+            }
+            __syncthreads();
+
+            for (int u = 0; u < A.nunits; ++u) {
+                const FusedUnit U = A.units[u];
+                // ---- 2. Toeplitz GEMM on the matrix cores
+                floatx16_t acc[UPW];
+#pragma unroll
+                for (int s = 0; s < UPW; ++s)
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) acc[s][r] = 0.f;
+                if (pbw >= 0) {
+                    const uint4* F = A.frag + U.frag_off;
+                    // phase k: row blocks 0..k-1 of the wave are active (blocks sorted descending)
+#pragma unroll
+                    for (int k = 1; k <= UPW; ++k) {
+                        if (k > nuw) break;
+                        const int dhi = 32 * A.wave_rb[wave][k - 1];
+                        const int dlo = k < nuw ? 32 * A.wave_rb[wave][k] + 16 : -16;
+                        for (int d = dhi; d >= dlo; d -= 16) {
+                            const uint4* Fd = F + ((d + 16) >> 4) * 128;
+                            const V ahi = __builtin_bit_cast(V, Fd[lane]);
+                            const V alo = __builtin_bit_cast(V, Fd[64 + lane]);
+                            V bhi[UPW], blo[UPW];
+#pragma unroll
+                            for (int s = 0; s < k; ++s) {
+                                const int ks = (32 * A.wave_rb[wave][s] - d) >> 4;
+                                const uint4* Bp = Bf + ((pbw * NK + ks) * 2) * 64;
+                                bhi[s] = __builtin_bit_cast(V, Bp[lane]);
+                                blo[s] = __builtin_bit_cast(V, Bp[64 + lane]);
+                            }
+#pragma unroll
+                            for (int s = 0; s < k; ++s) acc[s] = S::mma(ahi, bhi[s], acc[s]);
+#pragma unroll
+                            for (int s = 0; s < k; ++s) acc[s] = S::mma(ahi, blo[s], acc[s]);
+#pragma unroll
+                            for (int s = 0; s < k; ++s) acc[s] = S::mma(alo, bhi[s], acc[s]);
+                        }
+                    }
+                    // ---- 3a. Wtilde = sc (Y' + b W1perp), W1perp from the same counters (stream 1)
+                    const long p = p0 + 32 * pbw + col;
+#pragma unroll
+                    for (int s = 0; s < UPW; ++s) {
+                        if (s >= nuw) break;
+                        const int rb = A.wave_rb[wave][s];
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) {
+                            float z[4] = {0.f, 0.f, 0.f, 0.f};
+                            if (p < A.M) normals4_f32(A.key, uint64_t(p), 1u, uint32_t(8 * rb + 2 * c + h), z);
+#pragma unroll
+                            for (int t = 0; t < 4; ++t)
+                                acc[s][4 * c + t] = U.sc * (acc[s][4 * c + t] + U.b * z[t]);
+                        }
+                    }
+                }
+                // ---- 3b. integrals of every group of the unit, FUSED_GREG groups per pass
+                for (int g0 = U.g_begin; g0 < U.g_end; g0 += GR) {
+                    const int gn = U.g_end - g0 < GR ? U.g_end - g0 : GR;
+                    float c1[GR], al[GR], be[GR];
+                    double sv[GR], ss[GR], sp[GR];
+#pragma unroll
+                    for (int gg = 0; gg < GR; ++gg) {
+                        sv[gg] = ss[gg] = sp[gg] = 0.0;
+                        c1[gg] = al[gg] = be[gg] = 0.f;
+                        if (gg < gn) {
+                            c1[gg] = A.groups[g0 + gg].c1;
+                            al[gg] = A.groups[g0 + gg].al;
+                            be[gg] = A.groups[g0 + gg].be;
+                        }
+                    }
+                    if (pbw >= 0) {
+                        const float* P = A.ptab + U.p_off;
+                        const long p = p0 + 32 * pbw + col;
+#pragma unroll
+                        for (int s = 0; s < UPW; ++s) {
+                            if (s >= nuw) break;
+                            const int rb = A.wave_rb[wave][s];
+                            float fv[GR], fs[GR], fp[GR];
+#pragma unroll
+                            for (int gg = 0; gg < GR; ++gg) fv[gg] = fs[gg] = fp[gg] = 0.f;
+                            // plain Monte Carlo: Wperp_j (stream 2) at the rows j = row + 1 this lane
+                            // integrates.  Quad 8 rb + 2 c + h holds steps 4 (8 rb + 2 c + h) + 0..3,
+                            // so t = 0..2 come from the lane's own quad and t = 3 from element 0 of
+                            // the next quad: the other half-wave's quad (c, or c + 1 for h = 1), and
+                            // for (c = 3, h = 1) the first quad of the next row block.
+                            float wq[16];
+                            if constexpr (MC) {
+                                float zc[4][4], ze[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                                for (int c = 0; c < 4; ++c) {
+#pragma unroll
+                                    for (int t = 0; t < 4; ++t) zc[c][t] = 0.f;
+                                    if (p < A.M) normals4_f32(A.key, uint64_t(p), 2u, uint32_t(8 * rb + 2 * c + h), zc[c]);
+                                }
+                                if (p < A.M) normals4_f32(A.key, uint64_t(p), 2u, uint32_t(8 * rb + 8), ze);
+                                float y[4];
+#pragma unroll
+                                for (int c = 0; c < 4; ++c) y[c] = __shfl_xor(zc[c][0], 32, 64);
+#pragma unroll
+                                for (int c = 0; c < 4; ++c) {
+#pragma unroll
+                                    for (int t = 0; t < 3; ++t) wq[4 * c + t] = zc[c][t + 1];
+                                    wq[4 * c + 3] = h == 0 ? y[c] : (c < 3 ? y[c < 3 ? c + 1 : 3] : ze[0]);
+                                }
+                            }
+#pragma unroll
+                            for (int r = 0; r < 16; ++r) {
+                                const int j = 32 * rb + mfma32_row(r, lane) + 1;  // v_j uses Wtilde_{j-1}
+                                if (j < A.N) {
+                                    const int ks = j >> 4, kk = j & 15;
+                                    const uint4* Bp = Bf + ((pbw * NK + ks) * 2) * 64 + 32 * (kk >> 3) + col;
+                                    const float w1 = float(reinterpret_cast<const T*>(Bp)[kk & 7]) +
+                                                     float(reinterpret_cast<const T*>(Bp + 64)[kk & 7]);
+                                    const float pj = P[j];
+                                    const float x = acc[s][r];
+#pragma unroll
+                                    for (int gg = 0; gg < GR; ++gg) {
+                                        if (gg < gn) {
+                                            const float rr = __builtin_amdgcn_exp2f(fmaf(x, c1[gg], -fmaf(al[gg], pj, be[gg])));
+                                            fv[gg] = fmaf(rr, rr, fv[gg]);
+                                            fs[gg] = fmaf(rr, w1, fs[gg]);
+                                            if constexpr (MC) fp[gg] = fmaf(rr, wq[r], fp[gg]);
+                                        }
+                                    }
+                                }
+                            }
+#pragma unroll
+                            for (int gg = 0; gg < GR; ++gg) {
+                                sv[gg] += double(fv[gg]);
+                                ss[gg] += double(fs[gg]);
+                                if constexpr (MC) sp[gg] += double(fp[gg]);
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int gg = 0; gg < GR; ++gg) {
+                        sv[gg] += __shfl_xor(sv[gg], 32, 64);
+                        ss[gg] += __shfl_xor(ss[gg], 32, 64);
+                        if constexpr (MC) sp[gg] += __shfl_xor(sp[gg], 32, 64);
+                    }
+                    // cross-wave sums in chunks of gchunk groups (LDS budget)
+                    const int pg_lo = A.groups[g0].pg_begin;
+                    for (int c0 = 0; c0 < gn; c0 += A.gchunk) {
+                        const int cn = gn - c0 < A.gchunk ? gn - c0 : A.gchunk;
+                        if (h == 0) {
+#pragma unroll
+                            for (int gg = 0; gg < GR; ++gg)
+                                if (gg >= c0 && gg < c0 + cn) {
+                                    double* w = wpart + ((wave * A.gchunk + (gg - c0)) * 32 + col) * NS;
+                                    w[0] = sv[gg];
+                                    w[1] = ss[gg];
+                                    if constexpr (MC) w[2] = sp[gg];
+                                }
+                        }
+                        __syncthreads();
+                        for (int e = tid; e < A.TP * cn; e += 256) {
+                            const int pl = e & (A.TP - 1), gg = e >> A.log2TP;
+                            const int pb = pl >> 5, c = pl & 31;
+                            const FusedGroup G = A.groups[g0 + c0 + gg];
+                            const uint4* B0 = Bf + ((pb * NK) * 2) * 64 + c;  // k = 0
+                            const double w10 = double(float(reinterpret_cast<const T*>(B0)[0]) +
+                                                      float(reinterpret_cast<const T*>(B0 + 64)[0]));
+                            double a = G.xi, b = G.sxi * w10;   // j = 0: v_0 = xi
+                            double cp = 0.0;
+                            if constexpr (MC) {
+                                float z0[4] = {0.f, 0.f, 0.f, 0.f};
+                                if (p0 + pl < A.M) normals4_f32(A.key, uint64_t(p0 + pl), 2u, 0u, z0);
+                                cp = G.sxi * double(z0[0]);
+                            }
+                            for (int w = 0; w < 4; ++w)
+                                if (A.wave_pb[w] == pb) {
+                                    a += wpart[((w * A.gchunk + gg) * 32 + c) * NS + 0];
+                                    b += wpart[((w * A.gchunk + gg) * 32 + c) * NS + 1];
+                                    if constexpr (MC) cp += wpart[((w * A.gchunk + gg) * 32 + c) * NS + 2];
+                                }
+                            const double Iv = G.dt * a, Is = G.sdt * b;
+                            if constexpr (MC) {
+                                // ln S_T = rho Is + sqrt(1 - rho^2) Ip - Iv / 2 (log-Euler, left point)
+                                const double Ip = G.sdt * cp;
+                                for (int pg = G.pg_begin; pg < G.pg_end; ++pg) {
+                                    const double rho = A.pgs[pg].rho;
+                                    PathPG q;
+                                    q.lnS = rho * Is + sqrt(1.0 - rho * rho) * Ip - 0.5 * Iv;
+                                    q.sig = exp(q.lnS);
+                                    q.S = float(q.sig);
+                                    q.invs = 0.f;
+                                    q.lnSf = float(q.lnS);
+                                    q.sigf = 0.f;
+                                    pgd[pl * A.pgmax + (pg - pg_lo)] = q;
+                                }
+                                continue;
+                            }
+                            for (int pg = G.pg_begin; pg < G.pg_end; ++pg) {
+                                const double rho = A.pgs[pg].rho;
+                                PathPG q;
+                                q.sig = sqrt((1.0 - rho * rho) * Iv);
+                                q.lnS = -0.5 * rho * rho * Iv + rho * Is;
+                                q.S = float(exp(q.lnS));
+                                q.invs = float(1.0 / q.sig);
+                                q.lnSf = float(q.lnS);
+                                q.sigf = float(q.sig);
+                                pgd[pl * A.pgmax + (pg - pg_lo)] = q;
+                            }
+                        }
+                        __syncthreads();
+                    }
+                    // ---- 4. payoffs of the options of groups g0 .. g0+gn-1.  Register mode: the
+                    //         S = 2^log2slices lanes t = o*S .. o*S+S-1 share option o, each
+                    //         summing every S-th path; a fixed butterfly leaves the total in lane
+                    //         o*S, which owns option o's FP64 sums.  LDS mode (> 256 options):
+                    //         one thread per option, sums in LDS.
+                    const int o_lo = A.groups[g0].o_begin, o_hi = A.groups[g0 + gn - 1].o_end;
+                    const int np = A.M - p0 < A.TP ? int(A.M - p0) : A.TP;
+                    if (A.acc_in_lds) {
+                        for (int oi = o_lo + tid; oi < o_hi; oi += 256) {
+                            const FusedOption O = A.opts[oi];
+                            const int pgl = O.pg - pg_lo;
+                            const float lnKf = float(O.lnK), Kf = float(O.K);
+                            double s1 = 0.0, s2 = 0.0;
+                            for (int pl = 0; pl < np; ++pl) {
+                                const double x = MC ? payoff_mc(pgd[pl * A.pgmax + pgl], O.K)
+                                                    : payoff_mixed(pgd[pl * A.pgmax + pgl], lnKf, Kf, O.lnK, O.K);
+                                s1 += x;
+                                s2 += x * x;
+                            }
+                            accopt[2 * oi + 0] += s1;
+                            accopt[2 * oi + 1] += s2;
+                        }
+                    } else {
+                        const int S = 1 << A.log2slices;
+                        const int oi = tid >> A.log2slices, sl = tid & (S - 1);
+                        if (oi >= o_lo && oi < o_hi) {
+                            const FusedOption O = A.opts[oi];
+                            const int pgl = O.pg - pg_lo;
+                            const float lnKf = float(O.lnK), Kf = float(O.K);
+                            double s1 = 0.0, s2 = 0.0;
+                            for (int pl = sl; pl < np; pl += S) {
+                                const double x = MC ? payoff_mc(pgd[pl * A.pgmax + pgl], O.K)
+                                                    : payoff_mixed(pgd[pl * A.pgmax + pgl], lnKf, Kf, O.lnK, O.K);
+                                s1 += x;
+                                s2 += x * x;
+                            }
+                            for (int off = S >> 1; off >= 1; off >>= 1) {
+                                s1 += __shfl_xor(s1, off, 64);
+                                s2 += __shfl_xor(s2, off, 64);
+                            }
+                            if (sl == 0) {
+                                a1 += s1;
+                                a2 += s2;
+                            }
+                        }
+                    }
+                    __syncthreads();
+                }
+            }
+        }
+        // ---- item partial sums (indexed by the output index I)
+        if (A.acc_in_lds) {
+            for (int o = tid; o < A.out_size; o += 256) {
+                const int I = A.opts[o].out;
+                A.partial[(long(I) * A.nitems + item) * 2 + 0] = accopt[2 * o + 0];
+                A.partial[(long(I) * A.nitems + item) * 2 + 1] = accopt[2 * o + 1];
+            }
+        } else if ((tid & ((1 << A.log2slices) - 1)) == 0 && (tid >> A.log2slices) < A.out_size) {
+            const int I = A.opts[tid >> A.log2slices].out;
+            A.partial[(long(I) * A.nitems + item) * 2 + 0] = a1;
+            A.partial[(long(I) * A.nitems + item) * 2 + 1] = a2;
+        }
+        __syncthreads();
+    }
+}
+
+
+size_t fused_lds_bytes(const FusedBatchArgs& b);
+
+template <int MODE, int UPW, bool MC>
+static hipError_t launch_t(const FusedBatchArgs& a, int grid, hipStream_t st) {
+    const size_t lds = fused_lds_bytes(a);
+    if (lds > 65536) {
+        const hipError_t e = allow_lds(reinterpret_cast<const void*>(&k_fused<MODE, UPW, MC>), lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL((k_fused<MODE, UPW, MC>), dim3(grid), dim3(256), lds, st, a);
+    return hipGetLastError();
+}
+
+template <bool MC>
+static hipError_t launch_mu(int mode, int upw, const FusedBatchArgs& a, int grid, hipStream_t st) {
+    switch (mode * 16 + upw) {
+        case 1: return launch_t<0, 1, MC>(a, grid, st);
+        case 2: return launch_t<0, 2, MC>(a, grid, st);
+        case 4: return launch_t<0, 4, MC>(a, grid, st);
+        case 8: return launch_t<0, 8, MC>(a, grid, st);
+        case 17: return launch_t<1, 1, MC>(a, grid, st);
+        case 18: return launch_t<1, 2, MC>(a, grid, st);
+        case 20: return launch_t<1, 4, MC>(a, grid, st);
+        case 24: return launch_t<1, 8, MC>(a, grid, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace rb

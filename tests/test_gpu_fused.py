@@ -79,6 +79,30 @@ def test_fused_matches_fp64_on_same_counters(rb, case, prec):
     assert np.max(np.abs(sm - s64)) <= 1e-4
 
 
+MC_CASES = [0, 1, 2, 3, 4, 6, 7]
+
+
+@pytest.mark.parametrize("prec", ["fp16", "bf16"])
+@pytest.mark.parametrize("case", MC_CASES)
+def test_fused_plain_mc_matches_fp64_on_same_counters(rb, case, prec):
+    # computeIV_tensor's estimator: ln S_T from a third Gaussian stream, payoff (S_T - K)^+
+    N, M, H, eta, rho, T, K = CASES[case]
+    if T is None:
+        T, K = opts_config2_like()
+    xi = 0.04
+    seed = [13, case]
+    say("fused MC %s case %d N=%d M=%d" % (prec, case, N, M))
+    p64, iv64, s64 = rb.price_v2(xi, H, eta, rho, T, K, N, M, seed, precision=rb.PREC_FP64,
+                                 method=rb.METHOD_MC)
+    code = rb.PREC_SPLIT_FP16 if prec == "fp16" else rb.PREC_SPLIT_BF16
+    pm, ivm, sm = rb.price_v2(xi, H, eta, rho, T, K, N, M, seed, precision=code, method=rb.METHOD_MC)
+    err = np.max(np.abs(pm - p64))
+    say("   max |price - fp64| = %.3e   max |stat - fp64| = %.3e" % (err, np.max(np.abs(sm - s64))))
+    assert np.all(np.isfinite(pm))
+    assert err <= 1e-4
+    assert np.max(np.abs(sm - s64)) <= 1e-4
+
+
 def test_fused_reruns_are_bit_identical(rb):
     T, K = opts_config2_like()
     a = rb.price_v2(0.04, [0.1], [1.5], [-0.7], T, K, 512, 5000, [1], precision=rb.PREC_SPLIT_FP16)
@@ -88,9 +112,10 @@ def test_fused_reruns_are_bit_identical(rb):
 
 
 def test_split_precision_routes_outside_the_fused_kernel(rb):
-    # N > 1024 and plain Monte Carlo run the staged pipeline with the FP32 FFT convolution
+    # N > 1024 runs the staged pipeline with the FP32 FFT convolution (plain Monte Carlo at
+    # N = 2048 too); N = 64 plain Monte Carlo is the fused kernel
     T, K = [0.5, 1.0], [1.0, 1.1]
-    for N, method in ((2048, rb.METHOD_RT), (64, rb.METHOD_MC)):
+    for N, method in ((2048, rb.METHOD_RT), (2048, rb.METHOD_MC), (64, rb.METHOD_MC)):
         p64, _, s64 = rb.price_v2(0.04, [0.1], [1.5], [-0.7], T, K, N, 2000, [1], method=method)
         p16, _, s16 = rb.price_v2(0.04, [0.1], [1.5], [-0.7], T, K, N, 2000, [1], method=method,
                                   precision=rb.PREC_SPLIT_FP16)
