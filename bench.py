@@ -46,17 +46,18 @@ def load_traffic(kernel):
     return None if ent is None else ent.get("bytes_per_launch")
 
 
-def cpu_baseline(sample_paths, threads):
+def cpu_baseline(sample_paths, threads, method="rt"):
     from oracle import pyoracle as orc
     w = workload(sample_paths)
+    fn = orc.compute_rt if method == "rt" else orc.compute_mc
     t0 = time.perf_counter()
-    orc.compute_rt(w["xi"], w["H"], w["eta"], w["rho"], w["T"], w["K"], w["N"], w["M"], [2024, 1],
-                   num_threads=threads, conv=orc.CONV_FFT)
+    fn(w["xi"], w["H"], w["eta"], w["rho"], w["T"], w["K"], w["N"], w["M"], [2024, 1],
+       num_threads=threads, conv=orc.CONV_FFT)
     dt = time.perf_counter() - t0
     return {"value": sample_paths * w["N"] / dt, "unit": "paths*steps/s", "cores": threads,
             "kind": "port",
-            "sample": "%d paths of the same workload (N=512, 205 options), %d OpenMP threads, "
-                      "%.1f s" % (sample_paths, threads, dt)}
+            "sample": "%d paths of the same workload (N=512, 205 options, %s estimator), "
+                      "%d OpenMP threads, %.1f s" % (sample_paths, method.upper(), threads, dt)}
 
 
 def batched_requests(nsets=256, seed=3):
@@ -241,6 +242,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--precision", default="fp16", choices=sorted(PREC_CODE))
     ap.add_argument("--workload", default="single", choices=["single", "batched", "serving", "long"])
+    ap.add_argument("--method", default="rt", choices=["rt", "mc"],
+                    help="single workload: Romano-Touzi (computeIVRT_tensor) or plain Monte Carlo "
+                         "(computeIV_tensor)")
     ap.add_argument("--requests", type=int, default=4096, help="serving workload size")
     ap.add_argument("--paths", type=int, default=2 ** 22)
     ap.add_argument("--cpu-sample-paths", type=int, default=2 ** 18)
@@ -263,8 +267,9 @@ def main():
         print(json.dumps(fn(rb, args)), flush=True)
         return
     w = workload(args.paths)
+    method = rb.METHOD_RT if args.method == "rt" else rb.METHOD_MC
     plan = rb.Plan(w["xi"], w["H"], w["eta"], w["rho"], w["T"], w["K"], w["N"], w["M"],
-                   [20240601, 7], precision=PREC_CODE[args.precision])
+                   [20240601, 7], precision=PREC_CODE[args.precision], method=method)
     for _ in range(args.warmup):
         plan.run()
     # rb_plan_run returns after synchronizing the stream it launched on
@@ -278,7 +283,7 @@ def main():
 
     # one more run with HIP events around every launch, on the plan's own stream
     prof = rb.Plan(w["xi"], w["H"], w["eta"], w["rho"], w["T"], w["K"], w["N"], w["M"],
-                   [20240601, 7], precision=PREC_CODE[args.precision], profile=True)
+                   [20240601, 7], precision=PREC_CODE[args.precision], method=method, profile=True)
     prof.run()
     times = prof.kernel_times()
     info = prof.info()
@@ -311,7 +316,8 @@ def main():
         "dtype": {"fp64": "f64", "fp32": "f32", "fp16": "f16x2+f32", "bf16": "bf16x2+f32"}[args.precision],
         "data": "synthetic (device Philox4x32-10 Gaussians)",
         "config": {"workload": "single parameter set at scale: H=0.1 eta=1.5 rho=-0.7 xi=0.04 "
-                               "N=512 M=2^22, 5 maturities x 41 strikes",
+                               "N=512 M=2^22, 5 maturities x 41 strikes"
+                               + ("" if args.method == "rt" else ", plain Monte Carlo estimator"),
                    "paths": M, "steps": N, "options": len(w["T"]), "precision": args.precision,
                    "parallelism": "1 GPU"},
         "kernel_ms": {k: v[0] for k, v in times.items()},
@@ -322,7 +328,7 @@ def main():
     if not args.no_check and args.precision != "fp64":
         # same Philox counters through the FP64 path: the price error of the mixed precision
         ref = rb.Plan(w["xi"], w["H"], w["eta"], w["rho"], w["T"], w["K"], w["N"], w["M"],
-                      [20240601, 7], precision=PREC_CODE["fp64"])
+                      [20240601, 7], precision=PREC_CODE["fp64"], method=method)
         ref.run()
         p64, iv64, s64 = ref.results()
         ref.close()
@@ -330,7 +336,7 @@ def main():
                            "max_abs_stat_err_vs_fp64": float(np.max(np.abs(stat - s64))),
                            "max_price_stat_ratio": float(np.max(np.abs(price - p64) / s64)),
                            "reference": "staged FP64 path, same Philox counters"}
-    if not args.no_latency and args.precision in ("fp16", "bf16"):
+    if not args.no_latency and args.precision in ("fp16", "bf16") and args.method == "rt":
         # request latency under load: 256 serving requests, Poisson arrivals at 0.7 x capacity
         lat = run_serving(rb, serving_requests(256), PREC_CODE[args.precision])
         out["latency"] = {k: lat[k] for k in ("requests", "p50_ms", "p99_ms", "offered_req_per_s",
@@ -345,7 +351,7 @@ def main():
                              "GB_per_s": 2 * nbytes / (ms_copy.value * 1e-3) / 1e9}
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
-        out["cpu_baseline"] = cpu_baseline(args.cpu_sample_paths, threads)
+        out["cpu_baseline"] = cpu_baseline(args.cpu_sample_paths, threads, args.method)
     print(json.dumps(out), flush=True)
 
 

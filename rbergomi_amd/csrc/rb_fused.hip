@@ -47,11 +47,13 @@ hipError_t launch_zero_int(int* p, int n, hipStream_t st) {
 }
 
 hipError_t launch_fused_rt(int mode, int upw, const FusedBatchArgs& a, int grid, hipStream_t st) {
-    return launch_mu<false>(mode, upw, a, grid, st);
+    return launch_mu<false, FUSED_GREG>(mode, upw, a, grid, st);
 }
 
-hipError_t launch_fused(int mode, int upw, bool mc, const FusedBatchArgs& a, int grid, hipStream_t st) {
-    return mc ? launch_fused_mc(mode, upw, a, grid, st) : launch_fused_rt(mode, upw, a, grid, st);
+hipError_t launch_fused(int mode, int upw, bool mc, int greg, const FusedBatchArgs& a, int grid,
+                        hipStream_t st) {
+    if (!mc) return launch_fused_rt(mode, upw, a, grid, st);
+    return greg == 8 ? launch_fused_mc8(mode, upw, a, grid, st) : launch_fused_mc6(mode, upw, a, grid, st);
 }
 
 }  // namespace rb

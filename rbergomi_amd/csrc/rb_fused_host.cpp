@@ -148,8 +148,9 @@ int plan_fused_request(const Request& req, const Layout& lay, int mode, FusedHos
         U.g_end = int(out.fg.size());
         U.pg_end = int(out.fpg.size());
         U.o_end = int(out.fo.size());
-        // payoff groups held at once: those of one chunk of FUSED_GREG groups
-        for (int g0 = U.g_begin; g0 < U.g_end; g0 += FUSED_GREG) {
+        // payoff groups held at once: those of one pass over FUSED_GREG groups (Romano-Touzi);
+        // plain Monte Carlo passes over 6 or 8 groups, so every window of FUSED_GREG groups
+        for (int g0 = U.g_begin; g0 < U.g_end; g0 += req.method == METHOD_MC ? 1 : FUSED_GREG) {
             const int g1 = std::min(U.g_end, g0 + FUSED_GREG) - 1;
             pgmax = std::max(pgmax, out.fg[g1].pg_end - out.fg[g0].pg_begin);
         }
@@ -321,6 +322,11 @@ int FusedBatch::build(const std::vector<const FusedHostReq*>& reqs, int mode_, i
         R.push_back(r);
     }
     mc = nreq > 0 && R[0].mc != 0;
+    {
+        int gm = 1;
+        for (int q = 0; q < nreq; ++q) gm = std::max(gm, reqs[q]->gmax);
+        greg = mc ? fused_mc_greg(gm) : FUSED_GREG;
+    }
     total_out = int(out_off[nreq]);
     args.bf_u4 = bf;
     args.wpart_d = wp;
@@ -394,7 +400,7 @@ int FusedBatch::launch(hipStream_t st, std::string& err, Plan* timing) {
     int s;
     RB_TRY(launch_zero_int(d_counter, 1, st));
     s = timed(K_FUSED);
-    RB_TRY(launch_fused(mode, upw, mc, args, grid, st));
+    RB_TRY(launch_fused(mode, upw, mc, greg, args, grid, st));
     mark(s);
     s = timed(K_REDUCE);
     RB_TRY(launch_reduce_rows(d_partial, d_row_off, d_row_n, d_row_M, total_out, d_price, d_stat, st));

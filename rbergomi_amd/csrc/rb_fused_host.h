@@ -36,6 +36,7 @@ int plan_fused_request(const Request& req, const Layout& lay, int mode, FusedHos
 struct FusedBatch {
     int mode = 0, upw = 1, grid = 0, nreq = 0, total_out = 0;
     bool mc = false;  // plain Monte Carlo kernel
+    int greg = FUSED_GREG;  // groups per pass of the kernel
     hipStream_t stream = nullptr;  // the stream the buffers were allocated on (and are freed on)
     FusedBatchArgs args{};
     std::vector<long> out_off;   // first output of request q in the result arrays

@@ -41,9 +41,10 @@ __device__ __forceinline__ double half_sum_f64(double v) {
     return v;
 }
 
-template <int MODE, int UPW, bool MC>
+// GR: groups integrated per pass over the accumulators (at most FUSED_GREG)
+template <int MODE, int UPW, bool MC, int GR>
 __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs B) {
-    constexpr int GR = MC ? FUSED_GREG_MC : FUSED_GREG;  // groups per pass over the accumulators
+    static_assert(GR <= FUSED_GREG, "pgmax is sized for windows of FUSED_GREG groups");
     constexpr int NS = MC ? 3 : 2;                        // sums per (path, group)
     typedef SplitT<MODE> S;
     typedef typename S::T T;
@@ -164,9 +165,14 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
                     const int gn = U.g_end - g0 < GR ? U.g_end - g0 : GR;
                     float c1[GR], al[GR], be[GR];
                     double sv[GR], ss[GR], sp[GR];
+                    // FP32 partial sums: per 16-row block (Romano-Touzi), or over all row blocks of
+                    // the wave (plain Monte Carlo, at most 128 terms a lane; frees the registers
+                    // for the third sum)
+                    float fv[GR], fs[GR], fp[GR];
 #pragma unroll
                     for (int gg = 0; gg < GR; ++gg) {
                         sv[gg] = ss[gg] = sp[gg] = 0.0;
+                        fv[gg] = fs[gg] = fp[gg] = 0.f;
                         c1[gg] = al[gg] = be[gg] = 0.f;
                         if (gg < gn) {
                             c1[gg] = A.groups[g0 + gg].c1;
@@ -181,9 +187,10 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
                         for (int s = 0; s < UPW; ++s) {
                             if (s >= nuw) break;
                             const int rb = A.wave_rb[wave][s];
-                            float fv[GR], fs[GR], fp[GR];
+                            if constexpr (!MC) {
 #pragma unroll
-                            for (int gg = 0; gg < GR; ++gg) fv[gg] = fs[gg] = fp[gg] = 0.f;
+                                for (int gg = 0; gg < GR; ++gg) fv[gg] = fs[gg] = 0.f;
+                            }
                             // plain Monte Carlo: Wperp_j (stream 2) at the rows j = row + 1 this lane
                             // integrates.  Quad 8 rb + 2 c + h holds steps 4 (8 rb + 2 c + h) + 0..3,
                             // so t = 0..2 come from the lane's own quad and t = 3 from element 0 of
@@ -230,12 +237,21 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
                                     }
                                 }
                             }
+                            if constexpr (!MC) {
 #pragma unroll
-                            for (int gg = 0; gg < GR; ++gg) {
-                                sv[gg] += double(fv[gg]);
-                                ss[gg] += double(fs[gg]);
-                                if constexpr (MC) sp[gg] += double(fp[gg]);
+                                for (int gg = 0; gg < GR; ++gg) {
+                                    sv[gg] += double(fv[gg]);
+                                    ss[gg] += double(fs[gg]);
+                                }
                             }
+                        }
+                    }
+                    if constexpr (MC) {
+#pragma unroll
+                        for (int gg = 0; gg < GR; ++gg) {
+                            sv[gg] = double(fv[gg]);
+                            ss[gg] = double(fs[gg]);
+                            sp[gg] = double(fp[gg]);
                         }
                     }
 #pragma unroll
@@ -379,28 +395,28 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
 
 size_t fused_lds_bytes(const FusedBatchArgs& b);
 
-template <int MODE, int UPW, bool MC>
+template <int MODE, int UPW, bool MC, int GR>
 static hipError_t launch_t(const FusedBatchArgs& a, int grid, hipStream_t st) {
     const size_t lds = fused_lds_bytes(a);
     if (lds > 65536) {
-        const hipError_t e = allow_lds(reinterpret_cast<const void*>(&k_fused<MODE, UPW, MC>), lds);
+        const hipError_t e = allow_lds(reinterpret_cast<const void*>(&k_fused<MODE, UPW, MC, GR>), lds);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((k_fused<MODE, UPW, MC>), dim3(grid), dim3(256), lds, st, a);
+    hipLaunchKernelGGL((k_fused<MODE, UPW, MC, GR>), dim3(grid), dim3(256), lds, st, a);
     return hipGetLastError();
 }
 
-template <bool MC>
+template <bool MC, int GR>
 static hipError_t launch_mu(int mode, int upw, const FusedBatchArgs& a, int grid, hipStream_t st) {
     switch (mode * 16 + upw) {
-        case 1: return launch_t<0, 1, MC>(a, grid, st);
-        case 2: return launch_t<0, 2, MC>(a, grid, st);
-        case 4: return launch_t<0, 4, MC>(a, grid, st);
-        case 8: return launch_t<0, 8, MC>(a, grid, st);
-        case 17: return launch_t<1, 1, MC>(a, grid, st);
-        case 18: return launch_t<1, 2, MC>(a, grid, st);
-        case 20: return launch_t<1, 4, MC>(a, grid, st);
-        case 24: return launch_t<1, 8, MC>(a, grid, st);
+        case 1: return launch_t<0, 1, MC, GR>(a, grid, st);
+        case 2: return launch_t<0, 2, MC, GR>(a, grid, st);
+        case 4: return launch_t<0, 4, MC, GR>(a, grid, st);
+        case 8: return launch_t<0, 8, MC, GR>(a, grid, st);
+        case 17: return launch_t<1, 1, MC, GR>(a, grid, st);
+        case 18: return launch_t<1, 2, MC, GR>(a, grid, st);
+        case 20: return launch_t<1, 4, MC, GR>(a, grid, st);
+        case 24: return launch_t<1, 8, MC, GR>(a, grid, st);
         default: return hipErrorInvalidValue;
     }
 }

@@ -1,4 +1,5 @@
-// rb_fused_mc.hip -- the fused kernel for the plain Monte Carlo estimator (computeIV_tensor).
+// rb_fused_mc.hip -- the fused kernel for the plain Monte Carlo estimator (computeIV_tensor), six
+// groups per pass.
 //
 // Same stages as the Romano-Touzi kernel (rb_fused.hip) up to the integrals; a third Gaussian
 // stream Wperp (stream 2 of the counter space, as on the FP64 path) drives the price:
@@ -10,8 +11,9 @@
 
 namespace rb {
 
-hipError_t launch_fused_mc(int mode, int upw, const FusedBatchArgs& a, int grid, hipStream_t st) {
-    return launch_mu<true>(mode, upw, a, grid, st);
+// six groups per pass (the eight-group variant is in rb_fused_mc8.hip, built in parallel)
+hipError_t launch_fused_mc6(int mode, int upw, const FusedBatchArgs& a, int grid, hipStream_t st) {
+    return launch_mu<true, 6>(mode, upw, a, grid, st);
 }
 
 }  // namespace rb

@@ -103,6 +103,20 @@ def test_fused_plain_mc_matches_fp64_on_same_counters(rb, case, prec):
     assert np.max(np.abs(sm - s64)) <= 1e-4
 
 
+@pytest.mark.parametrize("nt", [7, 8, 10, 13])
+def test_fused_plain_mc_many_maturities(rb, nt):
+    # 6 or 8 groups per pass over the accumulators (fewest passes), several passes for nt > 8
+    T = list(np.repeat(np.linspace(0.05, 2.0, nt), 3))
+    K = list(np.tile([0.8, 1.0, 1.25], nt))
+    say("fused MC %d maturities" % nt)
+    args = (0.04, [0.1, 0.3], [1.5, 0.9], [-0.7, -0.2], T, K, 128, 3000, [5, nt])
+    p64, _, s64 = rb.price_v2(*args, precision=rb.PREC_FP64, method=rb.METHOD_MC)
+    for code in (rb.PREC_SPLIT_FP16, rb.PREC_SPLIT_BF16):
+        pm, _, sm = rb.price_v2(*args, precision=code, method=rb.METHOD_MC)
+        assert np.max(np.abs(pm - p64)) <= 1e-4
+        assert np.max(np.abs(sm - s64)) <= 1e-4
+
+
 def test_fused_reruns_are_bit_identical(rb):
     T, K = opts_config2_like()
     a = rb.price_v2(0.04, [0.1], [1.5], [-0.7], T, K, 512, 5000, [1], precision=rb.PREC_SPLIT_FP16)
