@@ -149,8 +149,9 @@ typedef struct rb_request {
 } rb_request;
 
 /* Prices n requests with one launch of the split-precision kernel (cfg->precision must be
- * RB_PREC_SPLIT_FP16 or RB_PREC_SPLIT_BF16, Romano-Touzi, N <= 1024).  Every request's results
- * are bit-identical to pricing it alone with rb_price_v2. */
+ * RB_PREC_SPLIT_FP16 or RB_PREC_SPLIT_BF16, N <= 1024; cfg->method and cfg->compensator apply to
+ * every request).  Every request's results are bit-identical to pricing it alone with
+ * rb_price_v2 and the same cfg.  The server prices Romano-Touzi requests. */
 int rb_price_batch(const rb_request* reqs, int n, const rb_config* cfg);
 
 /* Continuous-batching server.  A host thread forms batches from the pending requests, admits a

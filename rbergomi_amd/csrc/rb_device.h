@@ -52,15 +52,19 @@ __device__ __forceinline__ void box_muller_f64(uint32_t x0, uint32_t x1, double&
 }
 
 // The same transform in FP32 (mixed-precision kernels): identical counters, FP32 rounding.
+// Written with the hardware instructions directly, so that every call site compiles to the same
+// instruction sequence (a library sqrt was lowered correctly rounded at some call sites and as a
+// scaled one-ulp square root at others, so the same counter gave different FP32 normals in
+// different kernel instantiations).  u0 >= 2^-33 is normal; -2 ln u0 is 0 or >= 1e-7, never
+// subnormal, so the unscaled v_sqrt_f32 (1 ulp) applies.  The hardware sine and cosine take their
+// argument in turns: sin(2 pi u1) = v_sin_f32(u1).
 __device__ __forceinline__ void box_muller_f32(uint32_t x0, uint32_t x1, float& z0, float& z1) {
     const float u0 = (float(x0) + 0.5f) * 2.3283064365386963e-10f;
     const float u1 = (float(x1) + 0.5f) * 2.3283064365386963e-10f;
-    // ln u0 = log2(u0) * ln 2; hardware log2 and sin/cos of 2 pi u1
-    const float r = __fsqrt_rn(-1.3862943611198906f * __log2f(u0));
-    float s, c;
-    __sincosf(6.283185307179586f * u1, &s, &c);
-    z0 = r * c;
-    z1 = r * s;
+    // ln u0 = log2(u0) * ln 2
+    const float r = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u0));
+    z0 = r * __builtin_amdgcn_cosf(u1);
+    z1 = r * __builtin_amdgcn_sinf(u1);
 }
 
 // Four normals of path p, stream s, steps 4q..4q+3 in FP64

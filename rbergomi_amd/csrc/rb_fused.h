@@ -111,9 +111,12 @@ __device__ __forceinline__ float mills_f32(float z) {
 // conditioned Black-Scholes payoff of one path for one strike (tau = 1, S0 = S').
 // With E = S' phi(d1) = K phi(d2):  S' Phi(d1) = d1 >= 0 ? S' - E M(d1) : E M(-d1), and the
 // same for K Phi(d2), so one exponential serves both normal CDFs.
+// Multiply-adds are written out (no implicit contraction), so every kernel instantiation that
+// inlines this computes the same bits.
 __device__ __forceinline__ double payoff_mixed(const PathPG& q, float lnKf, float Kf, double lnK,
                                                double K) {
-    const float d1 = (q.lnSf - lnKf) * q.invs + 0.5f * q.sigf;
+#pragma clang fp contract(off)
+    const float d1 = fmaf(q.lnSf - lnKf, q.invs, 0.5f * q.sigf);
     if (d1 < -8.0f) {
         // deep out of the money (Phi(d1) < 7e-16): the normal CDF in FP64
         const double e1 = (q.lnS - lnK) / q.sig + 0.5 * q.sig;
@@ -124,9 +127,9 @@ __device__ __forceinline__ double payoff_mixed(const PathPG& q, float lnKf, floa
     const float a1 = mills_f32(fabsf(d1)), a2 = mills_f32(fabsf(d2));
     float x;
     if (d2 >= 0.f)
-        x = (q.S - Kf) + E * (a2 - a1);
+        x = fmaf(E, a2 - a1, q.S - Kf);
     else if (d1 >= 0.f)
-        x = q.S - E * (a1 + a2);
+        x = fmaf(-E, a1 + a2, q.S);
     else
         x = E * (a1 - a2);
     return double(x);

@@ -276,6 +276,11 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
                         }
                         __syncthreads();
                         for (int e = tid; e < A.TP * cn; e += 256) {
+                            // FP64 record of (path, payoff group): no multiply-add contraction,
+                            // so every instantiation (groups per pass, row blocks per wave)
+                            // computes the same bits and a request prices identically alone or
+                            // in any batch
+#pragma clang fp contract(off)
                             const int pl = e & (A.TP - 1), gg = e >> A.log2TP;
                             const int pb = pl >> 5, c = pl & 31;
                             const FusedGroup G = A.groups[g0 + c0 + gg];

@@ -46,7 +46,7 @@ struct Job {
     bool done = false;
 };
 
-int plan_job(const rb_request* q, int precision, Job& j, std::string& err) {
+int plan_job(const rb_request* q, int precision, int method, int comp, Job& j, std::string& err) {
     Request& r = j.req;
     r.xi = q->xi;
     if (q->par_size < 1 || q->op_size < 1 || !q->H || !q->eta || !q->rho || !q->T || !q->K) {
@@ -65,8 +65,8 @@ int plan_job(const rb_request* q, int precision, Job& j, std::string& err) {
     r.N = q->N;
     r.M = q->M;
     if (q->seed && q->seed_size > 0) r.seed.assign(q->seed, q->seed + q->seed_size);
-    r.method = METHOD_RT;
-    r.comp = COMP_REF;
+    r.method = method;
+    r.comp = comp;
     r.precision = precision;
     if (precision != PREC_SPLIT_FP16 && precision != PREC_SPLIT_BF16) {
         err = "batches run the split-precision kernel (RB_PREC_SPLIT_FP16 or RB_PREC_SPLIT_BF16)";
@@ -261,7 +261,7 @@ int rb_price_batch(const rb_request* reqs, int n, const rb_config* cfg) {
     std::string err;
     for (int i = 0; i < n; ++i) {
         jobs.emplace_back(new Job());
-        const int rc = plan_job(&reqs[i], cfg->precision, *jobs.back(), err);
+        const int rc = plan_job(&reqs[i], cfg->precision, cfg->method, cfg->compensator, *jobs.back(), err);
         if (rc) {
             set_last_error("request " + std::to_string(i) + ": " + err);
             return rc;
@@ -319,7 +319,7 @@ int rb_server_submit(rb_server* s, const rb_request* req, long* ticket) {
     std::unique_ptr<Job> j(new Job());
     j->t_submit = Clock::now();
     std::string err;
-    const int rc = plan_job(req, s->cfg.precision, *j, err);
+    const int rc = plan_job(req, s->cfg.precision, METHOD_RT, COMP_REF, *j, err);
     if (rc) {
         set_last_error(err);
         return rc;

@@ -55,12 +55,13 @@ class Request:
         self.latency_ms = None
 
 
-def price_batch(requests, precision=_lib.PREC_SPLIT_FP16):
+def price_batch(requests, precision=_lib.PREC_SPLIT_FP16, method=_lib.METHOD_RT,
+                compensator=_lib.COMP_REF):
     """Prices all requests with one launch; fills each request's price/iv/stat."""
     L = _lib.load()
     _declare(L)
     arr = (RbRequest * len(requests))(*[r.c for r in requests])
-    cfg = RbConfig(precision, _lib.COMP_REF, _lib.METHOD_RT, 0)
+    cfg = RbConfig(precision, compensator, method, 0)
     check(L.rb_price_batch(arr, len(requests), ctypes.byref(cfg)))
     return requests
 

@@ -48,6 +48,21 @@ def test_batch_equals_alone(rb, prec_name):
         assert np.array_equal(iv, r.iv, equal_nan=True)
 
 
+def test_plain_mc_batch_equals_alone(rb):
+    # computeIV_tensor's estimator in one batched launch (exact compensator too)
+    from rbergomi_amd.serving import price_batch
+    reqs = make_requests(8, seed=4)
+    price_batch(reqs, precision=rb.PREC_SPLIT_FP16, method=rb.METHOD_MC, compensator=rb.COMP_EXACT)
+    for r in reqs:
+        p, iv, st = rb.price_v2(r.xi, r.H, r.eta, r.rho, r.T, r.K, r.N, r.M, r.seed,
+                                precision=rb.PREC_SPLIT_FP16, method=rb.METHOD_MC,
+                                compensator=rb.COMP_EXACT)
+        assert np.array_equal(p, r.price) and np.array_equal(st, r.stat)
+        p64 = rb.price_v2(r.xi, r.H, r.eta, r.rho, r.T, r.K, r.N, r.M, r.seed,
+                          method=rb.METHOD_MC, compensator=rb.COMP_EXACT)[0]
+        assert np.max(np.abs(p - p64)) <= 1e-4
+
+
 def test_server_mix_equals_alone(rb):
     from rbergomi_amd.serving import Server
     reqs = make_requests(24, seed=2)
