@@ -34,16 +34,16 @@ def workload(paths):
     return dict(xi=0.04, H=[0.1], eta=[1.5], rho=[-0.7], T=T, K=K, N=512, M=paths)
 
 
-def load_traffic(kernel):
-    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 --pmc summary
-    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), or None."""
+def load_profile(kernel):
+    """The committed rocprofv3 --pmc summary of the dominant kernel on the default workload:
+    HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration) and
+    VALU / transcendental instruction counts per launch, or {}."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
-        return None
+        return {}
     with open(path) as f:
         data = json.load(f)
-    ent = data.get(kernel)
-    return None if ent is None else ent.get("bytes_per_launch")
+    return data.get(kernel, {})
 
 
 def cpu_baseline(sample_paths, threads, method="rt"):
@@ -298,10 +298,20 @@ def main():
     if "volterra" in dominant or "fused" in dominant:
         achieved = flops_per_launch / (avg_ms * 1e-3) / 1e12
         peak = PEAK_TFLOPS[args.precision]
+        default = (args.paths == 2 ** 22 and args.precision == "fp16" and args.method == "rt")
+        prof_counts = load_profile(dominant) if "fused" in dominant and default else {}
         roof = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                "frac": achieved / peak, "traffic": load_traffic(dominant), "kernel": dominant,
-                "avg_launch_ms": avg_ms, "launches_per_step": launches,
+                "frac": achieved / peak, "traffic": prof_counts.get("bytes_per_launch"),
+                "kernel": dominant, "avg_launch_ms": avg_ms, "launches_per_step": launches,
                 "flops_per_launch": flops_per_launch}
+        if prof_counts.get("valu_insts_per_launch"):
+            # the resource that binds the fused kernel: VALU issue.  A wave64 VALU instruction
+            # holds a 16-lane SIMD for 4 cycles (a transcendental for 16); 1024 SIMDs at 2.4 GHz
+            v, tr = prof_counts["valu_insts_per_launch"], prof_counts["trans_f32_insts_per_launch"]
+            busy = (4.0 * v + 12.0 * tr) / 1024 / (avg_ms * 1e-3 * 2.4e9)
+            roof["valu"] = {"insts_per_launch": v, "trans_insts_per_launch": tr,
+                            "issue_busy_frac": busy,
+                            "source": "SQ_INSTS_VALU, SQ_INSTS_VALU_TRANS_F32 (rocprofv3 --pmc)"}
     out = {
         "metric": "paths*steps/s",
         "value": M * N / (ms * 1e-3),
