@@ -127,6 +127,28 @@ int rb_plan_kernel_times(rb_plan* plan, char* names, double* total_ms, int* laun
 int rb_plan_info(rb_plan* plan, int* nunits, int* ngroups, int* noptions, long* chunk);
 void rb_plan_destroy(rb_plan* plan);
 
+/* ---- unordered rows ----
+ * Row i prices the call (T[i], K[i]) under (xi[i], H[i], eta[i], rho[i]); every row uses the same
+ * Gaussians of each path, and outputs are indexed by row.  Replaces ComputePriceRTMTunstructured /
+ * ComputeIVRTMTunstructured (rBergomi/rBergomiMT.cpp:837-935) over ParamTotUnordered
+ * (rBergomi/ParamTot.h:57-75), whose xi is one scalar; the per-row xi is that of the ParamTot of
+ * ML_console_ST/ParamTot.h:25-43 (pass the same value in every row for the former).  cfg as for
+ * rb_price_v2 (NULL: FP64, Romano-Touzi, reference compensator); at most 65535 rows. */
+int rb_price_rows(const double* xi, const double* H, const double* eta, const double* rho,
+                  const double* T, const double* K, int rows, int N, long M, const uint64_t* seed,
+                  int seed_size, const rb_config* cfg, double* price, double* iv, double* stat);
+int rb_plan_create_rows(const double* xi, const double* H, const double* eta, const double* rho,
+                        const double* T, const double* K, int rows, int N, long M,
+                        const uint64_t* seed, int seed_size, const rb_config* cfg, void* stream,
+                        rb_plan** out);
+/* Caller Gaussians in the quadrature layout (path m: W1 = Z[2mN, 2mN+N), W1perp = the next N
+ * values; plain MC: 3N values per path, the third block Wperp) -> payoff[m*rows + i], and
+ * optionally price/stat (FP64 path). */
+int rb_payoff_rows_samples(const double* xi, const double* H, const double* eta, const double* rho,
+                           const double* T, const double* K, int rows, int N, long M,
+                           const double* Z, const rb_config* cfg, double* payoff, double* price,
+                           double* stat);
+
 /* ---- batches and serving (no counterpart in the reference) ----
  * One pricing request in the reference's tensor form plus its caller-owned outputs
  * (out_size = par_size*op_size doubles each; iv may be NULL). */

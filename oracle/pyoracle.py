@@ -57,6 +57,7 @@ def lib():
         L.orc_payoff_rt_samples.argtypes = samp
         L.orc_integrals_rt_samples.argtypes = samp
         L.orc_payoff_mc_samples.argtypes = samp
+        L.orc_payoff_rows_samples.argtypes = [_dp, _dp, _dp, _dp, _dp, _dp, i, i, l, _dp, i, _dp]
         L.orc_price_stat.argtypes = [_dp, l, i, _dp, _dp]
         seeded = [d, _dp, _dp, _dp, i, _dp, _dp, i, i, l, _u64p, i, i, i, _dp, _dp, _dp]
         L.orc_compute_rt.argtypes = seeded
@@ -159,6 +160,19 @@ def payoff_mc_samples(xi, H, eta, rho, T, K, N, Z, comp=COMP_REF):
     M = Z.size // (3 * N)
     out = np.zeros((M, args[3] * args[6]))
     lib().orc_payoff_mc_samples(xi, *args, N, M, pZ, comp, out.ctypes.data_as(_dp))
+    return out
+
+
+def payoff_rows_samples(xi, H, eta, rho, T, K, N, Z, comp=COMP_REF):
+    """unordered rows: payoff[m][i] of row i (xi may be a scalar)"""
+    n = len(H)
+    xi = np.full(n, float(xi)) if np.ndim(xi) == 0 else xi
+    arrs = [np.ascontiguousarray(x, dtype=np.float64) for x in (xi, H, eta, rho, T, K)]
+    Z = np.ascontiguousarray(Z, dtype=np.float64)
+    M = Z.shape[0]
+    out = np.zeros((M, n))
+    lib().orc_payoff_rows_samples(*[a.ctypes.data_as(_dp) for a in arrs], n, N, M,
+                                  Z.ctypes.data_as(_dp), comp, out.ctypes.data_as(_dp))
     return out
 
 

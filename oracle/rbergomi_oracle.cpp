@@ -329,6 +329,29 @@ void orc_integrals_rt_samples(double xi, const double* H, const double* eta, con
     }
 }
 
+void orc_payoff_rows_samples(const double* xi, const double* H, const double* eta,
+                             const double* rho, const double* T, const double* K, int rows,
+                             int N, long M, const double* Z, int comp_mode, double* payoff) {
+#pragma omp parallel
+    {
+        Vec Wt(N), v(N);
+#pragma omp for schedule(static)
+        for (long m = 0; m < M; ++m) {
+            const double* W1 = Z + 2 * m * N;
+            const double* W1p = W1 + N;
+            for (int i = 0; i < rows; ++i) {
+                // updateWtilde, scaleWtilde + updateV, intVdt, intRootVdW, BS_call_price
+                orc_wtilde(W1, W1p, H[i], N, ORC_CONV_DIRECT, Wt.data());
+                orc_update_v(Wt.data(), xi[i], H[i], eta[i], T[i], N, comp_mode, v.data());
+                const double dt = T[i] / N;
+                const double Iv = orc_int_vdt(v.data(), N, dt);
+                const double Is = orc_int_rootvdw(v.data(), W1, N, std::sqrt(dt));
+                payoff[m * rows + i] = orc_rt_payoff(Iv, Is, rho[i], K[i]);
+            }
+        }
+    }
+}
+
 void orc_payoff_mc_samples(double xi, const double* H, const double* eta, const double* rho,
                            int par_size, const double* T, const double* K, int op_size,
                            int N, long M, const double* Z, int comp_mode, double* payoff) {

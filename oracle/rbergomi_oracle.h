@@ -25,6 +25,9 @@
  *   orc_iv_call          round trip IV(BS(sigma)) = sigma; put-call parity branch     (test_oracle_math.py)
  *   orc_philox4x32_10    Random123 known-answer vectors                                (test_oracle_math.py)
  *   orc_normals_philox   distribution moments; equals Box-Muller on orc_philox4x32_10 outputs
+ *   orc_payoff_rows_samples  equals orc_payoff_rt_samples on the zipped tensor laid out as rows;
+ *                        per-row xi equals separate scalar-xi runs; row-order independent
+ *                        (test_oracle_rows.py)
  *
  * Parity unpinned: the exact roundoff of FFTW (the reference convolution) and the iterate
  * sequence of Boost's bracket_and_solve_root in IV_call (rBergLib/BlackScholes.cpp:36-63).
@@ -104,6 +107,15 @@ void orc_integrals_rt_samples(double xi, const double* H, const double* eta, con
 void orc_payoff_mc_samples(double xi, const double* H, const double* eta, const double* rho,
                            int par_size, const double* T, const double* K, int op_size,
                            int N, long M, const double* Z, int comp_mode, double* payoff);
+
+/* Per-path Romano-Touzi payoffs of unordered rows: row i prices (T[i], K[i]) under
+ * (xi[i], H[i], eta[i], rho[i]), every row on the same W1, W1perp of the path (same Z layout as
+ * orc_payoff_rt_samples); payoff[m*rows + i].  updatePayoffUnordered, rBergomi/rBergomiMT.cpp:
+ * 939-960, inside the path loop of ComputePriceRTMTunstructured :837-911 (its xi is one scalar;
+ * the per-row xi follows ML_console_ST/ParamTot.h:25-43).  Direct-sum convolution. */
+void orc_payoff_rows_samples(const double* xi, const double* H, const double* eta,
+                             const double* rho, const double* T, const double* K, int rows,
+                             int N, long M, const double* Z, int comp_mode, double* payoff);
 
 /* price/stat from a payoff matrix [M][out_size]: mean and sqrt(E[X^2]-mean^2)/sqrt(M),
  * rBergLib/rBergomiMT.cpp:404-409 */

@@ -114,6 +114,43 @@ def payoff_samples_v2(xi, H, eta, rho, T, K, N, Z, method=_lib.METHOD_RT,
     return payoff, price, stat
 
 
+def price_rows(xi, H, eta, rho, T, K, N, M, seed, precision=_lib.PREC_FP64,
+               compensator=_lib.COMP_REF, method=_lib.METHOD_RT):
+    """Unordered rows (rb_price_rows): row i prices (T[i], K[i]) under (xi[i], H[i], eta[i],
+    rho[i]) on shared paths; xi may be a scalar.  -> (price, iv, stat), one entry per row."""
+    n = len(H)
+    xi = np.full(n, float(xi)) if np.ndim(xi) == 0 else xi
+    arrs = [_arr(x) for x in (xi, H, eta, rho, T, K)]
+    if any(len(a) != n for a, _ in arrs):
+        raise ValueError("every row array needs one entry per row")
+    s, ps = _seed(seed)
+    price, iv, stat = np.zeros(n), np.zeros(n), np.zeros(n)
+    cfg = _cfg(precision, compensator, method)
+    check(_lib.load().rb_price_rows(*[p for _, p in arrs], n, int(N), int(M), ps, len(s),
+                                    ctypes.byref(cfg), price.ctypes.data_as(dp),
+                                    iv.ctypes.data_as(dp), stat.ctypes.data_as(dp)))
+    return price, iv, stat
+
+
+def payoff_rows_samples(xi, H, eta, rho, T, K, N, Z, compensator=_lib.COMP_REF,
+                        method=_lib.METHOD_RT):
+    """Per-path payoffs of unordered rows from caller Gaussians Z[M][streams][N] (FP64 path)
+    -> (payoff[M][rows], price, stat)."""
+    n = len(H)
+    xi = np.full(n, float(xi)) if np.ndim(xi) == 0 else xi
+    arrs = [_arr(x) for x in (xi, H, eta, rho, T, K)]
+    Z = np.ascontiguousarray(Z, dtype=np.float64)
+    M = Z.shape[0]
+    payoff = np.zeros((M, n))
+    price, stat = np.zeros(n), np.zeros(n)
+    cfg = _cfg(_lib.PREC_FP64, compensator, method)
+    check(_lib.load().rb_payoff_rows_samples(*[p for _, p in arrs], n, int(N), int(M),
+                                             Z.ctypes.data_as(dp), ctypes.byref(cfg),
+                                             payoff.ctypes.data_as(dp), price.ctypes.data_as(dp),
+                                             stat.ctypes.data_as(dp)))
+    return payoff, price, stat
+
+
 class Plan:
     """Device-resident pricing plan (rb_plan_*): build once, run many times.
 

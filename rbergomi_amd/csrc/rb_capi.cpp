@@ -55,6 +55,23 @@ int fail(int rc, const std::string& msg) {
     return rc;
 }
 
+// unordered rows: one (xi, H, eta, rho, T, K) per output
+Request make_rows_request(const double* xi, const double* H, const double* eta, const double* rho,
+                          const double* T, const double* K, int rows, int N, long M,
+                          const uint64_t* seed, int seed_size, const rb_config* cfg) {
+    Request r = make_request(0.0, H, eta, rho, rows, T, K, rows, N, M, seed, seed_size, cfg);
+    r.rows = true;
+    r.xi_rows.assign(xi, xi + rows);
+    return r;
+}
+
+int check_rows(const double* xi, const double* H, const double* eta, const double* rho,
+               const double* T, const double* K, int rows) {
+    if (rows < 1) return fail(RB_ERR_ARG, "rows must be positive");
+    if (!xi || !H || !eta || !rho || !T || !K) return fail(RB_ERR_ARG, "null input array");
+    return RB_OK;
+}
+
 // Legacy path: the reference checks out_size and num_threads and exits (cinterface.cpp:15-20,
 // :38-42); device failures have no channel in that ABI, so they exit too (status 40).
 void legacy_price(int method, double xi, double* H, double* eta, double* rho, int par_size,
@@ -222,6 +239,56 @@ int rb_plan_create(double xi, const double* H, const double* eta, const double* 
     const int rc = p->plan.init(r, static_cast<hipStream_t>(stream), false, err);
     if (rc) return fail(rc, err);
     *out = p.release();
+    return RB_OK;
+}
+
+int rb_plan_create_rows(const double* xi, const double* H, const double* eta, const double* rho,
+                        const double* T, const double* K, int rows, int N, long M,
+                        const uint64_t* seed, int seed_size, const rb_config* cfg, void* stream,
+                        rb_plan** out) {
+    set_last_error("");
+    if (!out) return fail(RB_ERR_ARG, "null plan pointer");
+    *out = nullptr;
+    int rc = check_rows(xi, H, eta, rho, T, K, rows);
+    if (rc) return rc;
+    std::unique_ptr<rb_plan> p(new rb_plan());
+    Request r = make_rows_request(xi, H, eta, rho, T, K, rows, N, M, seed, seed_size, cfg);
+    std::string err;
+    rc = p->plan.init(r, static_cast<hipStream_t>(stream), false, err);
+    if (rc) return fail(rc, err);
+    *out = p.release();
+    return RB_OK;
+}
+
+int rb_price_rows(const double* xi, const double* H, const double* eta, const double* rho,
+                  const double* T, const double* K, int rows, int N, long M, const uint64_t* seed,
+                  int seed_size, const rb_config* cfg, double* price, double* iv, double* stat) {
+    set_last_error("");
+    if (!price || !stat) return fail(RB_ERR_ARG, "null output array");
+    rb_plan* p = nullptr;
+    int rc = rb_plan_create_rows(xi, H, eta, rho, T, K, rows, N, M, seed, seed_size, cfg, nullptr, &p);
+    if (rc) return rc;
+    rc = rb_plan_run(p);
+    if (!rc) rc = rb_plan_results(p, price, iv, stat);
+    rb_plan_destroy(p);
+    return rc;
+}
+
+int rb_payoff_rows_samples(const double* xi, const double* H, const double* eta, const double* rho,
+                           const double* T, const double* K, int rows, int N, long M,
+                           const double* Z, const rb_config* cfg, double* payoff, double* price,
+                           double* stat) {
+    set_last_error("");
+    int rc = check_rows(xi, H, eta, rho, T, K, rows);
+    if (rc) return rc;
+    if (!Z || !payoff) return fail(RB_ERR_ARG, "null array");
+    Request r = make_rows_request(xi, H, eta, rho, T, K, rows, N, M, nullptr, 0, cfg);
+    std::string err;
+    Plan p;
+    rc = p.init(r, nullptr, true, err);
+    if (!rc) rc = p.run_samples(Z, payoff, err);
+    if (!rc && (price || stat)) rc = p.results(price, nullptr, stat, err);
+    if (rc) return fail(rc, err);
     return RB_OK;
 }
 

@@ -40,8 +40,12 @@ int validate(const Request& r, std::string& err) {
         err = "parameter arrays must be zipped: |H| = |eta| = |rho| and |T| = |K|";
         return ST_ERR_ARG;
     }
-    if (r.H.size() * r.T.size() > 65535) {
-        err = "par_size*op_size above the supported maximum of 65535 outputs per call";
+    if (r.rows && (r.T.size() != r.H.size() || r.xi_rows.size() != r.H.size())) {
+        err = "unordered rows: every parameter array has one entry per row";
+        return ST_ERR_ARG;
+    }
+    if (r.out_size() > 65535 || r.H.size() * (r.rows ? 1 : r.T.size()) > 65535) {
+        err = "outputs above the supported maximum of 65535 per call";
         return ST_ERR_LIMIT;
     }
     if (r.N < 1 || r.M < 1) {
@@ -76,6 +80,11 @@ int validate(const Request& r, std::string& err) {
         err = "xi must be non-negative";
         return ST_ERR_ARG;
     }
+    for (double x : r.xi_rows)
+        if (!(x >= 0.0)) {
+            err = "xi must be non-negative";
+            return ST_ERR_ARG;
+        }
     return ST_OK;
 }
 
@@ -112,13 +121,19 @@ Layout make_layout(const Request& r) {
             unit_of_par[h] = it->second;
         }
     }
-    // groups: distinct (H, eta, T) for RT; (H, eta, T, rho) for plain MC
-    std::map<std::tuple<int, double, double, double>, int> group_of;
-    for (long h = 0; h < P; ++h) {
-        for (long t = 0; t < O; ++t) {
+    // groups: distinct (H, eta, T, xi) for RT; (H, eta, T, xi, rho) for plain MC.  Outputs in
+    // order: I = h*|T| + t (tensor form, rBergLib/ParamTot.h:32-33), or I = row (unordered rows,
+    // rBergomi/ParamTot.h:57-75, per-row xi as ML_console_ST/ParamTot.h:25-43)
+    std::map<std::tuple<int, double, double, double, double>, int> group_of;
+    const long nout = r.out_size();
+    for (long I = 0; I < nout; ++I) {
+        {
+            const long h = r.rows ? I : I / O;
+            const long t = r.rows ? I : I % O;
+            const double xi = r.rows ? r.xi_rows[I] : r.xi;
             const int u = unit_of_par[h];
             const double rho_key = r.method == METHOD_MC ? r.rho[h] : 0.0;
-            const auto key = std::make_tuple(u, r.eta[h], r.T[t], rho_key);
+            const auto key = std::make_tuple(u, r.eta[h], r.T[t], xi, rho_key);
             auto it = group_of.find(key);
             int g;
             if (it == group_of.end()) {
@@ -128,7 +143,7 @@ Layout make_layout(const Request& r) {
                 Group G;
                 G.unit = u;
                 G.pad = 0;
-                G.xi = r.xi;
+                G.xi = xi;
                 G.eta = eta;
                 G.TH = std::pow(T, H);        // scaleWtilde :163-167
                 G.dt = T / N;

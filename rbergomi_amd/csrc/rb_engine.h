@@ -30,13 +30,17 @@ enum Status {
 struct Request {
     double xi = 0.0;
     std::vector<double> H, eta, rho, T, K;
+    // unordered rows: output i prices (T[i], K[i]) under (xi_rows[i], H[i], eta[i], rho[i]);
+    // otherwise the reference's zipped tensor form, output h*|T| + t
+    bool rows = false;
+    std::vector<double> xi_rows;
     int N = 0;
     long M = 0;
     std::vector<uint64_t> seed;
     int method = METHOD_RT;
     int precision = PREC_FP64;
     int comp = COMP_REF;
-    int out_size() const { return int(H.size() * T.size()); }
+    int out_size() const { return rows ? int(H.size()) : int(H.size() * T.size()); }
 };
 
 // Deduplicated work tables (see rb_types.h)
