@@ -141,6 +141,20 @@ int rb_plan_create_rows(const double* xi, const double* H, const double* eta, co
                         const double* T, const double* K, int rows, int N, long M,
                         const uint64_t* seed, int seed_size, const rb_config* cfg, void* stream,
                         rb_plan** out);
+/* Turbo-charged Romano-Touzi estimator of unordered rows, FP64: a control variate and antithetic
+ * pairs, replacing ComputePriceRTVarRed (ML_console_ST/RBergomi.cpp:219-288, with
+ * estimateControlVariate :493-537 and updatePayoffControlVariate :478-491).  m_control pilot paths
+ * (the reference's Mcontrol, 10000 there; drawn from Philox counters disjoint from the M main
+ * paths) give Q_i = max 2 Iv and alpha_i = -cov(X, Y)/cov(Y, Y); each main path prices
+ * (X + alpha Y)/2 on (W1, W1perp) and on (-W1, -W1perp), with q_max (the reference's Qmax, 10000)
+ * in place of Q where Q <= Iv; price = mean - alpha E[Y].  stat is the standard error of the
+ * pair-averaged payoff.  alpha, Q (per row) and exceptions (Q <= Iv evaluations per row) may be
+ * NULL.  Requires m_control <= the paths of one chunk (about 6 GB of staged buffers). */
+int rb_price_rows_turbo(const double* xi, const double* H, const double* eta, const double* rho,
+                        const double* T, const double* K, int rows, int N, long M,
+                        const uint64_t* seed, int seed_size, long m_control, double q_max,
+                        double* price, double* iv, double* stat, double* alpha, double* Q,
+                        long* exceptions);
 /* Caller Gaussians in the quadrature layout (path m: W1 = Z[2mN, 2mN+N), W1perp = the next N
  * values; plain MC: 3N values per path, the third block Wperp) -> payoff[m*rows + i], and
  * optionally price/stat (FP64 path). */

@@ -58,6 +58,8 @@ def lib():
         L.orc_integrals_rt_samples.argtypes = samp
         L.orc_payoff_mc_samples.argtypes = samp
         L.orc_payoff_rows_samples.argtypes = [_dp, _dp, _dp, _dp, _dp, _dp, i, i, l, _dp, i, _dp]
+        L.orc_turbo_rows.argtypes = [_dp, _dp, _dp, _dp, _dp, _dp, i, i, l, _dp, l, _dp, d, i,
+                                     _dp, _dp, _dp, _dp, ctypes.POINTER(ctypes.c_long)]
         L.orc_price_stat.argtypes = [_dp, l, i, _dp, _dp]
         seeded = [d, _dp, _dp, _dp, i, _dp, _dp, i, i, l, _u64p, i, i, i, _dp, _dp, _dp]
         L.orc_compute_rt.argtypes = seeded
@@ -173,6 +175,24 @@ def payoff_rows_samples(xi, H, eta, rho, T, K, N, Z, comp=COMP_REF):
     out = np.zeros((M, n))
     lib().orc_payoff_rows_samples(*[a.ctypes.data_as(_dp) for a in arrs], n, N, M,
                                   Z.ctypes.data_as(_dp), comp, out.ctypes.data_as(_dp))
+    return out
+
+
+def turbo_rows(xi, H, eta, rho, T, K, N, Zc, Z, qmax=10000.0, comp=COMP_REF):
+    """turbo-charged RT estimator of unordered rows from pilot Gaussians Zc[Mc][2][N] and main
+    Gaussians Z[M][2][N] -> dict(price, stat, alpha, Q, exceptions)"""
+    n = len(H)
+    xi = np.full(n, float(xi)) if np.ndim(xi) == 0 else xi
+    arrs = [np.ascontiguousarray(x, dtype=np.float64) for x in (xi, H, eta, rho, T, K)]
+    Zc = np.ascontiguousarray(Zc, dtype=np.float64)
+    Z = np.ascontiguousarray(Z, dtype=np.float64)
+    out = {k: np.zeros(n) for k in ("price", "stat", "alpha", "Q")}
+    exc = np.zeros(n, dtype=np.int64)
+    lib().orc_turbo_rows(*[a.ctypes.data_as(_dp) for a in arrs], n, N, Zc.shape[0],
+                         Zc.ctypes.data_as(_dp), Z.shape[0], Z.ctypes.data_as(_dp), float(qmax), comp,
+                         *[out[k].ctypes.data_as(_dp) for k in ("price", "stat", "alpha", "Q")],
+                         exc.ctypes.data_as(ctypes.POINTER(ctypes.c_long)))
+    out["exceptions"] = exc
     return out
 
 

@@ -352,6 +352,97 @@ void orc_payoff_rows_samples(const double* xi, const double* H, const double* et
     }
 }
 
+// integrals of row i on one path (updateWtilde, scaleWtilde + updateV, intVdt, intRootVdW)
+static void row_integrals(const double* W1, const double* W1p, double xi, double H, double eta,
+                          double T, int N, int comp_mode, Vec& Wt, Vec& v, double& Iv, double& Is) {
+    orc_wtilde(W1, W1p, H, N, ORC_CONV_DIRECT, Wt.data());
+    orc_update_v(Wt.data(), xi, H, eta, T, N, comp_mode, v.data());
+    const double dt = T / N;
+    Iv = orc_int_vdt(v.data(), N, dt);
+    Is = orc_int_rootvdw(v.data(), W1, N, std::sqrt(dt));
+}
+
+// two-pass sample covariance, ML_console_ST/aux.h:207-220
+static double sample_cov(const Vec& x, const Vec& y) {
+    double mx = 0.0, my = 0.0;
+    for (double a : x) mx += a;
+    for (double b : y) my += b;
+    mx /= double(x.size());
+    my /= double(y.size());
+    double c = 0.0;
+    for (size_t k = 0; k < x.size(); ++k) c += (x[k] - mx) * (y[k] - my);
+    return c / double(x.size() - 1);
+}
+
+void orc_turbo_rows(const double* xi, const double* H, const double* eta, const double* rho,
+                    const double* T, const double* K, int rows, int N, long Mc, const double* Zc,
+                    long M, const double* Z, double qmax, int comp_mode, double* price,
+                    double* stat, double* alpha, double* Q, long* exceptions) {
+    Vec Wt(N), v(N), mW1(N), mW1p(N);
+    // pilot: integrals, Q = max 2 Iv (estimateControlVariate :504-516)
+    std::vector<Vec> Iv(rows, Vec(Mc)), Is(rows, Vec(Mc));
+    for (int i = 0; i < rows; ++i) Q[i] = 0.0;
+    for (long m = 0; m < Mc; ++m) {
+        const double* W1 = Zc + 2 * m * N;
+        for (int i = 0; i < rows; ++i) {
+            row_integrals(W1, W1 + N, xi[i], H[i], eta[i], T[i], N, comp_mode, Wt, v, Iv[i][m], Is[i][m]);
+            Q[i] = std::max(Q[i], 2.0 * Iv[i][m]);
+        }
+    }
+    // X, Y and alpha = -cov(X, Y) / cov(Y, Y)  (:520-534)
+    for (int i = 0; i < rows; ++i) {
+        Vec X(Mc), Y(Mc);
+        const double r2 = rho[i] * rho[i];
+        for (long m = 0; m < Mc; ++m) {
+            const double S = std::exp(-0.5 * r2 * Iv[i][m] + rho[i] * Is[i][m]);
+            X[m] = orc_bs_call(S, K[i], 1.0, std::sqrt((1.0 - r2) * Iv[i][m]));
+            Y[m] = orc_bs_call(S, K[i], 1.0, std::sqrt(r2 * (Q[i] - Iv[i][m])));
+        }
+        const double vy = sample_cov(Y, Y);
+        alpha[i] = vy > 0.0 ? -sample_cov(X, Y) / vy : 0.0;
+    }
+    // main loop with antithetics (:245-266) and the control-variate payoff (:478-491)
+    std::vector<double> s1(rows, 0.0), s2(rows, 0.0);
+    for (int i = 0; i < rows; ++i) exceptions[i] = 0;
+    for (long m = 0; m < M; ++m) {
+        const double* W1 = Z + 2 * m * N;
+        for (int k = 0; k < N; ++k) {
+            mW1[k] = -W1[k];
+            mW1p[k] = -W1[N + k];
+        }
+        for (int i = 0; i < rows; ++i) {
+            const double r2 = rho[i] * rho[i];
+            double pay = 0.0;
+            for (int side = 0; side < 2; ++side) {
+                double iv_, is_;
+                if (side == 0)
+                    row_integrals(W1, W1 + N, xi[i], H[i], eta[i], T[i], N, comp_mode, Wt, v, iv_, is_);
+                else
+                    row_integrals(mW1.data(), mW1p.data(), xi[i], H[i], eta[i], T[i], N, comp_mode, Wt, v,
+                                  iv_, is_);
+                const double S = std::exp(-0.5 * r2 * iv_ + rho[i] * is_);
+                const double X = orc_bs_call(S, K[i], 1.0, std::sqrt((1.0 - r2) * iv_));
+                const bool exc = !(Q[i] > iv_);
+                if (exc) ++exceptions[i];
+                const double Y = orc_bs_call(S, K[i], 1.0, std::sqrt(r2 * ((exc ? qmax : Q[i]) - iv_)));
+                pay += 0.5 * (X + alpha[i] * Y);
+            }
+            s1[i] += pay;
+            s2[i] += pay * pay;
+        }
+    }
+    for (int i = 0; i < rows; ++i) {
+        const double mean = s1[i] / double(M);
+        const double var = s2[i] / double(M) - mean * mean;
+        stat[i] = std::sqrt(var) / std::sqrt(double(M));
+        const double pq = double(exceptions[i]) / double(M);
+        const double r2 = rho[i] * rho[i];
+        const double adj = (1.0 - pq) * orc_bs_call(1.0, K[i], 1.0, std::sqrt(r2 * Q[i])) +
+                           pq * orc_bs_call(1.0, K[i], 1.0, std::sqrt(r2 * qmax));
+        price[i] = mean - alpha[i] * adj;
+    }
+}
+
 void orc_payoff_mc_samples(double xi, const double* H, const double* eta, const double* rho,
                            int par_size, const double* T, const double* K, int op_size,
                            int N, long M, const double* Z, int comp_mode, double* payoff) {

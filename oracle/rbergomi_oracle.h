@@ -25,6 +25,9 @@
  *   orc_iv_call          round trip IV(BS(sigma)) = sigma; put-call parity branch     (test_oracle_math.py)
  *   orc_philox4x32_10    Random123 known-answer vectors                                (test_oracle_math.py)
  *   orc_normals_philox   distribution moments; equals Box-Muller on orc_philox4x32_10 outputs
+ *   orc_turbo_rows       unbiased against the plain RT oracle (within 4.5 combined SE); eta = 0
+ *                        closed form C_BS(1, K, T, sqrt(xi)); the control-variate mean equals its
+ *                        closed-form adjustment; variance below the plain estimator's   (test_oracle_rows.py)
  *   orc_payoff_rows_samples  equals orc_payoff_rt_samples on the zipped tensor laid out as rows;
  *                        per-row xi equals separate scalar-xi runs; row-order independent
  *                        (test_oracle_rows.py)
@@ -116,6 +119,22 @@ void orc_payoff_mc_samples(double xi, const double* H, const double* eta, const 
 void orc_payoff_rows_samples(const double* xi, const double* H, const double* eta,
                              const double* rho, const double* T, const double* K, int rows,
                              int N, long M, const double* Z, int comp_mode, double* payoff);
+
+/* Turbo-charged Romano-Touzi estimator of unordered rows (ComputePriceRTVarRed,
+ * ML_console_ST/RBergomi.cpp:219-288; estimateControlVariate :493-537;
+ * updatePayoffControlVariate :478-491; covariance ML_console_ST/aux.h:207-220):
+ *   pilot (Mc paths of Zc): Q_i = max 2 Iv; X = C_BS(S', K, 1, sqrt((1-rho^2) Iv)),
+ *   Y = C_BS(S', K, 1, sqrt(rho^2 (Q - Iv))); alpha_i = -cov(X, Y) / cov(Y, Y) (two-pass sample
+ *   covariances; alpha = 0 when cov(Y, Y) = 0, where the reference divides by zero);
+ *   main (M paths of Z): payoff = (X + alpha Y)(W)/2 + (X + alpha Y)(-W)/2 (antithetic), with
+ *   qmax in place of Q when Q <= Iv, counted per evaluation; price = mean - alpha [(1 - p)
+ *   C_BS(1, K, 1, sqrt(rho^2 Q)) + p C_BS(1, K, 1, sqrt(rho^2 qmax))], p = count / M as in the
+ *   reference; stat from the pair-averaged payoffs.  Z layouts as orc_payoff_rt_samples.
+ *   exceptions[i] = count. */
+void orc_turbo_rows(const double* xi, const double* H, const double* eta, const double* rho,
+                    const double* T, const double* K, int rows, int N, long Mc, const double* Zc,
+                    long M, const double* Z, double qmax, int comp_mode, double* price,
+                    double* stat, double* alpha, double* Q, long* exceptions);
 
 /* price/stat from a payoff matrix [M][out_size]: mean and sqrt(E[X^2]-mean^2)/sqrt(M),
  * rBergLib/rBergomiMT.cpp:404-409 */

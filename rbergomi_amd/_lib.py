@@ -61,6 +61,8 @@ def load():
     L.rb_price_rows.argtypes = [dp, dp, dp, dp, dp, dp, i, i, l, u64p, i, cfgp, dp, dp, dp]
     L.rb_plan_create_rows.argtypes = [dp, dp, dp, dp, dp, dp, i, i, l, u64p, i, cfgp, vp,
                                       ctypes.POINTER(vp)]
+    L.rb_price_rows_turbo.argtypes = [dp, dp, dp, dp, dp, dp, i, i, l, u64p, i, l, d, dp, dp, dp,
+                                      dp, dp, lp]
     L.rb_payoff_rows_samples.argtypes = [dp, dp, dp, dp, dp, dp, i, i, l, dp, cfgp, dp, dp, dp]
     L.rb_plan_run.argtypes = [vp]
     L.rb_plan_results.argtypes = [vp, dp, dp, dp]

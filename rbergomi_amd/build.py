@@ -11,7 +11,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "librbergomi_mi355x.so")
 
-SOURCES = ["rb_staged.hip", "rb_fused.hip", "rb_fused_mc.hip", "rb_fused_mc8.hip", "rb_fft.hip", "rb_probe.hip", "rb_engine.cpp",
+SOURCES = ["rb_staged.hip", "rb_fused.hip", "rb_fused_mc.hip", "rb_fused_mc8.hip", "rb_fft.hip",
+           "rb_turbo.hip", "rb_probe.hip", "rb_engine.cpp",
            "rb_fused_host.cpp", "rb_capi.cpp", "rb_server.cpp"]
 # Packed FP32 instructions are disabled for all device code (see the package notes): the
 # compiler would otherwise form v_pk_* from ordinary float code.

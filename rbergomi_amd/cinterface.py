@@ -132,6 +132,26 @@ def price_rows(xi, H, eta, rho, T, K, N, M, seed, precision=_lib.PREC_FP64,
     return price, iv, stat
 
 
+def price_rows_turbo(xi, H, eta, rho, T, K, N, M, seed, m_control=10000, q_max=10000.0):
+    """Turbo-charged RT estimator of unordered rows (rb_price_rows_turbo): control variate plus
+    antithetic pairs, FP64 -> dict(price, iv, stat, alpha, Q, exceptions)."""
+    n = len(H)
+    xi = np.full(n, float(xi)) if np.ndim(xi) == 0 else xi
+    arrs = [_arr(x) for x in (xi, H, eta, rho, T, K)]
+    if any(len(a) != n for a, _ in arrs):
+        raise ValueError("every row array needs one entry per row")
+    s, ps = _seed(seed)
+    out = {k: np.zeros(n) for k in ("price", "iv", "stat", "alpha", "Q")}
+    exc = np.zeros(n, dtype=np.int64)
+    P = lambda k: out[k].ctypes.data_as(dp)
+    check(_lib.load().rb_price_rows_turbo(*[p for _, p in arrs], n, int(N), int(M), ps, len(s),
+                                          int(m_control), float(q_max), P("price"), P("iv"),
+                                          P("stat"), P("alpha"), P("Q"),
+                                          exc.ctypes.data_as(ctypes.POINTER(ctypes.c_long))))
+    out["exceptions"] = exc
+    return out
+
+
 def payoff_rows_samples(xi, H, eta, rho, T, K, N, Z, compensator=_lib.COMP_REF,
                         method=_lib.METHOD_RT):
     """Per-path payoffs of unordered rows from caller Gaussians Z[M][streams][N] (FP64 path)

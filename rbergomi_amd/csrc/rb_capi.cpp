@@ -274,6 +274,29 @@ int rb_price_rows(const double* xi, const double* H, const double* eta, const do
     return rc;
 }
 
+int rb_price_rows_turbo(const double* xi, const double* H, const double* eta, const double* rho,
+                        const double* T, const double* K, int rows, int N, long M,
+                        const uint64_t* seed, int seed_size, long m_control, double q_max,
+                        double* price, double* iv, double* stat, double* alpha, double* Q,
+                        long* exceptions) {
+    set_last_error("");
+    int rc = check_rows(xi, H, eta, rho, T, K, rows);
+    if (rc) return rc;
+    if (!price || !stat) return fail(RB_ERR_ARG, "null output array");
+    Request r = make_rows_request(xi, H, eta, rho, T, K, rows, N, M, seed, seed_size, nullptr);
+    r.turbo = true;
+    r.m_control = m_control;
+    r.q_max = q_max;
+    std::string err;
+    Plan p;
+    rc = p.init(r, nullptr, false, err);
+    if (!rc) rc = p.run(err);
+    if (!rc) rc = p.results(price, iv, stat, err);
+    if (!rc && (alpha || Q || exceptions)) rc = p.turbo_results(alpha, Q, exceptions, err);
+    if (rc) return fail(rc, err);
+    return RB_OK;
+}
+
 int rb_payoff_rows_samples(const double* xi, const double* H, const double* eta, const double* rho,
                            const double* T, const double* K, int rows, int N, long M,
                            const double* Z, const rb_config* cfg, double* payoff, double* price,

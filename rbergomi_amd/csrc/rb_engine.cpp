@@ -217,8 +217,9 @@ Plan::~Plan() {
     if (graph) (void)hipGraphDestroy(graph);
     fused_batch.reset();  // stream-ordered frees on this plan's stream
     if (stream) (void)hipStreamSynchronize(stream);
-    void* bufs[] = {d_units, d_gamma, d_groups, d_comp, d_opts, d_Z,    d_Wt,  d_ints,
-                    d_partial, d_price, d_stat, d_iv, d_payoff, d_tw, d_ghat};
+    void* bufs[] = {d_units, d_gamma, d_groups, d_comp, d_opts,  d_Z,     d_Wt,   d_ints,
+                    d_partial, d_price, d_stat, d_iv, d_payoff, d_tw, d_ghat, d_ints2,
+                    d_Qg, d_alpha, d_exc, d_count};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
@@ -252,6 +253,10 @@ int Plan::init(const Request& r, hipStream_t st, bool quadrature, std::string& e
         own_stream = true;
     }
     const bool split = r.precision == PREC_SPLIT_FP16 || r.precision == PREC_SPLIT_BF16;
+    if (r.turbo && (r.precision != PREC_FP64 || r.method != METHOD_RT || quadrature)) {
+        err = "the turbo-charged estimator runs the Romano-Touzi FP64 path on device Gaussians";
+        return ST_ERR_ARG;
+    }
     if (r.precision == PREC_FP32 || (split && (r.N > 1024 || quadrature))) {
         // staged pipeline with the FP32 FFT convolution (long paths, user Gaussians)
         conv_fft = true;
@@ -273,7 +278,17 @@ int Plan::init(const Request& r, hipStream_t st, bool quadrature, std::string& e
     const int nopt = int(lay.options.size());
     const long nunits = long(lay.units.size()), ngroups = long(lay.groups.size());
     // chunk the paths so that the staged buffers stay within ~6 GiB
-    const double per_path = 8.0 * N * (nstreams + nunits) + 16.0 * ngroups +
+    if (r.turbo) {
+        if (r.precision != PREC_FP64 || r.method != METHOD_RT || quadrature) {
+            err = "the turbo-charged estimator runs the Romano-Touzi FP64 path on device Gaussians";
+            return ST_ERR_ARG;
+        }
+        if (r.m_control < 2 || !(r.q_max > 0.0)) {
+            err = "turbo-charged estimator: m_control >= 2 and q_max > 0 required";
+            return ST_ERR_ARG;
+        }
+    }
+    const double per_path = 8.0 * N * (nstreams + nunits) + 16.0 * ngroups * (r.turbo ? 2 : 1) +
                             (quadrature ? 8.0 * nopt : 0.0);
     long ch = long(6.0e9 / per_path);
     // RBERGOMI_MAX_CHUNK caps the paths per chunk (used to test chunk boundaries)
@@ -284,6 +299,15 @@ int Plan::init(const Request& r, hipStream_t st, bool quadrature, std::string& e
     ch = std::max<long>(256, (ch / 256) * 256);
     const long Mr = ((r.M + 255) / 256) * 256;
     chunk = std::min(ch, Mr);
+    if (r.turbo) {
+        // the pilot paths are priced as one chunk (two-pass covariances over all of them)
+        const long mc = ((r.m_control + 255) / 256) * 256;
+        if (mc > ch) {
+            err = "m_control above the paths one chunk holds (" + std::to_string(ch) + ")";
+            return ST_ERR_LIMIT;
+        }
+        chunk = std::max(chunk, mc);
+    }
     nblk = (r.M + 255) / 256;
     RB_TRY(upload(&d_units, lay.units, stream));
     RB_TRY(upload(&d_gamma, lay.gamma, stream));
@@ -302,6 +326,13 @@ int Plan::init(const Request& r, hipStream_t st, bool quadrature, std::string& e
     RB_TRY(dalloc(&d_stat, nopt));
     RB_TRY(dalloc(&d_iv, nopt));
     if (quadrature) RB_TRY(dalloc(&d_payoff, size_t(chunk) * nopt));
+    if (r.turbo) {
+        RB_TRY(dalloc(&d_ints2, size_t(ngroups) * chunk * 2));
+        RB_TRY(dalloc(&d_Qg, size_t(ngroups)));
+        RB_TRY(dalloc(&d_alpha, size_t(nopt)));
+        RB_TRY(dalloc(&d_exc, size_t(nopt) * nblk));
+        RB_TRY(dalloc(&d_count, size_t(nopt)));
+    }
     RB_TRY(hipStreamSynchronize(stream));
     return ST_OK;
 }
@@ -358,16 +389,37 @@ int Plan::enqueue(std::string& err) {
     const int N = req.N;
     const int nopt = int(lay.options.size());
     const PhiloxKey key = philox_key(req.seed.data(), int(req.seed.size()));
+    const int ngroups = int(lay.groups.size());
+    if (req.turbo) {
+        // pilot paths (counters TURBO_PILOT_BASE + m, disjoint from the main paths): Q and alpha
+        const long mc = req.m_control;
+        RB_LAUNCH(K_NORMALS, launch_normals_f64(key, TURBO_PILOT_BASE, mc, nstreams, N, d_Z, stream));
+        RB_LAUNCH(K_VOLTERRA, conv(mc, long(nstreams) * N));
+        RB_LAUNCH(K_SCAN, launch_scan_f64(METHOD_RT, d_Z, long(nstreams) * N, d_Wt, mc, N, d_groups,
+                                          ngroups, d_comp, d_ints, stream, 1.0));
+        RB_LAUNCH(K_PAYOFF, launch_turbo_q(d_ints, mc, ngroups, d_Qg, stream));
+        RB_LAUNCH(K_PAYOFF, launch_turbo_alpha(d_ints, mc, d_opts, nopt, d_Qg, d_alpha, stream));
+    }
     for (long p0 = 0; p0 < req.M; p0 += chunk) {
         const long np = std::min(chunk, req.M - p0);
         RB_LAUNCH(K_NORMALS, launch_normals_f64(key, p0, np, nstreams, N, d_Z, stream));
         RB_LAUNCH(K_VOLTERRA, conv(np, long(nstreams) * N));
         RB_LAUNCH(K_SCAN, launch_scan_f64(req.method, d_Z, long(nstreams) * N, d_Wt, np, N, d_groups,
-                                          int(lay.groups.size()), d_comp, d_ints, stream));
-        RB_LAUNCH(K_PAYOFF, launch_payoff_f64(d_ints, np, p0, d_opts, nopt, req.method, p0 / 256, nblk,
-                                              d_partial, nullptr, stream));
+                                          ngroups, d_comp, d_ints, stream));
+        if (req.turbo) {
+            RB_LAUNCH(K_SCAN, launch_scan_f64(METHOD_RT, d_Z, long(nstreams) * N, d_Wt, np, N, d_groups,
+                                              ngroups, d_comp, d_ints2, stream, -1.0));
+            RB_LAUNCH(K_PAYOFF, launch_turbo_payoff(d_ints, d_ints2, np, d_opts, nopt, d_Qg, d_alpha,
+                                                    req.q_max, p0 / 256, nblk, d_partial, d_exc, stream));
+        } else {
+            RB_LAUNCH(K_PAYOFF, launch_payoff_f64(d_ints, np, p0, d_opts, nopt, req.method, p0 / 256,
+                                                  nblk, d_partial, nullptr, stream));
+        }
     }
     RB_LAUNCH(K_REDUCE, launch_reduce_f64(d_partial, nblk, req.M, nopt, d_price, d_stat, stream));
+    if (req.turbo)
+        RB_LAUNCH(K_REDUCE, launch_turbo_adjust(d_price, d_opts, nopt, d_Qg, d_alpha, req.q_max, d_exc,
+                                                nblk, req.M, d_count, stream));
     RB_LAUNCH(K_IV, launch_iv_f64(d_price, d_opts, nopt, d_iv, stream));
     return ST_OK;
 }
@@ -490,6 +542,23 @@ hipError_t Plan::conv(long np, long zstride) {
         return launch_volterra_fft(d_Z, zstride, np, req.N, log2L, int(lay.units.size()), d_units, d_ghat,
                                    d_tw, d_Wt, stream);
     return launch_volterra_f64(d_Z, zstride, np, req.N, int(lay.units.size()), d_units, d_gamma, d_Wt, stream);
+}
+
+int Plan::turbo_results(double* alpha, double* Q, long* exceptions, std::string& err) {
+    if (!req.turbo) {
+        err = "not a turbo-charged plan";
+        return ST_ERR_ARG;
+    }
+    const int nopt = int(lay.options.size());
+    if (alpha) RB_TRY(hipMemcpyAsync(alpha, d_alpha, nopt * sizeof(double), hipMemcpyDeviceToHost, stream));
+    if (exceptions)
+        RB_TRY(hipMemcpyAsync(exceptions, d_count, nopt * sizeof(long), hipMemcpyDeviceToHost, stream));
+    std::vector<double> qg(lay.groups.size());
+    RB_TRY(hipMemcpyAsync(qg.data(), d_Qg, qg.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+    RB_TRY(hipStreamSynchronize(stream));
+    if (Q)
+        for (int i = 0; i < nopt; ++i) Q[i] = qg[lay.options[i].group];
+    return ST_OK;
 }
 
 int Plan::results(double* price, double* iv, double* stat, std::string& err) {

@@ -40,6 +40,11 @@ struct Request {
     int method = METHOD_RT;
     int precision = PREC_FP64;
     int comp = COMP_REF;
+    // turbo-charged RT estimator (control variate + antithetics, ML_console_ST/RBergomi.cpp:219-288):
+    // m_control pilot paths estimate Q and alpha; q_max replaces Q where Q <= Iv
+    bool turbo = false;
+    long m_control = 10000;
+    double q_max = 10000.0;
     int out_size() const { return rows ? int(H.size()) : int(H.size() * T.size()); }
 };
 
@@ -52,6 +57,9 @@ struct Layout {
     std::vector<double> comp;   // [groups][N]: c_i = eta^2/2 ((i-1) dt)^(2H), c_0 = 0
     std::vector<Option> options;
 };
+
+// Philox path counters of the turbo-charged estimator's pilot paths (disjoint from main paths)
+constexpr long TURBO_PILOT_BASE = 1L << 62;
 
 int validate(const Request& r, std::string& err);
 Layout make_layout(const Request& r);
@@ -90,6 +98,12 @@ struct Plan {
     double* d_stat = nullptr;
     double* d_iv = nullptr;
     double* d_payoff = nullptr;   // [chunk][options], quadrature mode only
+    // turbo-charged estimator
+    double* d_ints2 = nullptr;    // [groups][chunk][2] integrals of the antithetic paths
+    double* d_Qg = nullptr;       // [groups] Q = max over the pilot paths of 2 Iv
+    double* d_alpha = nullptr;    // [options] control-variate coefficients
+    int* d_exc = nullptr;         // [options][nblk] Q <= Iv evaluations per path block
+    long* d_count = nullptr;      // [options] their totals
     // long-path FP32 FFT convolution (replaces k_volterra_f64 in the staged pipeline)
     bool conv_fft = false;
     int log2L = 0;
@@ -117,6 +131,8 @@ struct Plan {
     // user Gaussians (host buffer, quadrature layout), per-path payoffs back to the host
     int run_samples(const double* Z_host, double* payoff_host, std::string& err);
     int results(double* price, double* iv, double* stat, std::string& err);
+    // turbo-charged estimator: per-option alpha, Q and count of Q <= Iv evaluations (each may be null)
+    int turbo_results(double* alpha, double* Q, long* exceptions, std::string& err);
     int init_fused(std::string& err);
     int init_fft(std::string& err);
     hipError_t conv(long np, long zstride);

@@ -20,7 +20,7 @@ hipError_t launch_volterra_f64(const double* Z, long zstride, long np, int N, in
                                hipStream_t st);
 hipError_t launch_scan_f64(int method, const double* Z, long zstride, const double* Wt, long np,
                            int N, const Group* groups, int ngroups, const double* comp,
-                           double* out, hipStream_t st);
+                           double* out, hipStream_t st, double sign = 1.0);
 hipError_t launch_payoff_f64(const double* ints, long np, long p0, const Option* opts,
                              int out_size, int method, long blk0, long nblk_total,
                              double* partial, double* payoff, hipStream_t st);
@@ -46,5 +46,17 @@ size_t volterra_fft_lds_bytes(int log2L);
 hipError_t launch_volterra_fft(const double* Z, long zstride, long np, int N, int log2L, int nunits,
                                const Unit* units, const float2* ghat, const float2* tw, double* Wt,
                                hipStream_t st);
+
+// turbo-charged RT estimator (control variate + antithetics)
+hipError_t launch_turbo_q(const double* ints, long np, int ngroups, double* Qg, hipStream_t st);
+hipError_t launch_turbo_alpha(const double* ints, long np, const Option* opts, int nopt,
+                              const double* Qg, double* alpha, hipStream_t st);
+hipError_t launch_turbo_payoff(const double* ints_p, const double* ints_m, long np,
+                               const Option* opts, int nopt, const double* Qg,
+                               const double* alpha, double qmax, long blk0, long nblk_total,
+                               double* partial, int* exc, hipStream_t st);
+hipError_t launch_turbo_adjust(double* price, const Option* opts, int nopt, const double* Qg,
+                               const double* alpha, double qmax, const int* exc, long nblk,
+                               long M, long* count, hipStream_t st);
 
 }  // namespace rb

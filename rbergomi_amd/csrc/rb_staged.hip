@@ -142,7 +142,7 @@ __device__ __forceinline__ double wave_sum_f64(double v) {
 __global__ void __launch_bounds__(256)
 k_scan_rt_f64(const double* __restrict__ Z, long zstride, const double* __restrict__ Wt,
               long np, int N, const Group* __restrict__ groups, int ngroups,
-              const double* __restrict__ comp, double* __restrict__ out) {
+              const double* __restrict__ comp, double* __restrict__ out, double sign) {
     const long wid = (long(blockIdx.x) * blockDim.x + threadIdx.x) / 64;
     const int lane = threadIdx.x & 63;
     if (wid >= np * ngroups) return;
@@ -157,9 +157,11 @@ k_scan_rt_f64(const double* __restrict__ Z, long zstride, const double* __restri
     double sv = 0.0, ss = 0.0;
     for (int i = lane; i < N; i += 64) {
         // sqrt(v_i) = sqrt(xi) exp((eta T^H Wtilde_{i-1} - c_i)/2), v_0 = xi
-        const double r = i == 0 ? sxi : sxi * exp(e2 * wt[i - 1] - 0.5 * c[i]);
+        // sign = -1: the antithetic path (-W1, -W1perp), whose Volterra process is -Wtilde
+        // exactly (the convolution is linear and negation is exact)
+        const double r = i == 0 ? sxi : sxi * exp(e2 * (sign * wt[i - 1]) - 0.5 * c[i]);
         sv = fma(r, r, sv);
-        ss = fma(r, W1[i], ss);
+        ss = fma(r, sign * W1[i], ss);
     }
     sv = wave_sum_f64(sv);
     ss = wave_sum_f64(ss);
@@ -357,12 +359,12 @@ hipError_t launch_volterra_f64(const double* Z, long zstride, long np, int N, in
 
 hipError_t launch_scan_f64(int method, const double* Z, long zstride, const double* Wt, long np,
                            int N, const Group* groups, int ngroups, const double* comp,
-                           double* out, hipStream_t st) {
+                           double* out, hipStream_t st, double sign) {
     const long waves = np * ngroups;
     dim3 grid(unsigned((waves + 3) / 4));
     if (method == METHOD_RT)
         hipLaunchKernelGGL(k_scan_rt_f64, grid, dim3(256), 0, st, Z, zstride, Wt, np, N, groups,
-                           ngroups, comp, out);
+                           ngroups, comp, out, sign);
     else
         hipLaunchKernelGGL(k_scan_mc_f64, grid, dim3(256), 0, st, Z, zstride, Wt, np, N, groups,
                            ngroups, comp, out);

@@ -93,3 +93,57 @@ def test_row_argument_errors(rb):
                          seed.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), 1, None,
                          out.ctypes.data_as(dp), out.ctypes.data_as(dp), out.ctypes.data_as(dp))
     assert rc == 1
+
+
+TURBO_PILOT_BASE = 1 << 62   # Philox path counters of the pilot paths
+
+
+def turbo_case():
+    K = np.array([0.8, 0.95, 1.0, 1.1, 1.3, 1.0])
+    n = len(K)
+    H = np.array([0.1, 0.1, 0.1, 0.1, 0.1, 0.3])
+    eta = np.array([1.5, 1.5, 1.5, 1.5, 1.5, 0.9])
+    rho = np.array([-0.7, -0.7, -0.7, -0.7, -0.7, -0.3])
+    T = np.array([1.0, 1.0, 1.0, 1.0, 1.0, 0.5])
+    xi = np.array([0.04, 0.04, 0.04, 0.04, 0.04, 0.09])
+    return xi, H, eta, rho, T, K
+
+
+@pytest.mark.parametrize("mc,qmax", [(3000, 10000.0), (3, 50.0)])
+def test_turbo_matches_oracle_on_same_gaussians(rb, orc, mc, qmax):
+    """The device estimator against the oracle's restatement of ComputePriceRTVarRed on the same
+    pilot and main Gaussians (the small pilot forces Q <= Iv exceptions)."""
+    xi, H, eta, rho, T, K = turbo_case()
+    N, M, seed = 32, 4000, [21, 5]
+    print("turbo mc=%d" % mc, flush=True)
+    got = rb.price_rows_turbo(xi, H, eta, rho, T, K, N, M, seed, m_control=mc, q_max=qmax)
+    Zc = orc.normals_philox(seed, TURBO_PILOT_BASE, mc, 2, N)
+    Z = orc.normals_philox(seed, 0, M, 2, N)
+    ref = orc.turbo_rows(xi, H, eta, rho, T, K, N, Zc, Z, qmax=qmax)
+    assert np.array_equal(got["exceptions"], ref["exceptions"])
+    if mc == 3:
+        assert np.all(got["exceptions"] > 0)
+    assert np.max(np.abs(got["Q"] / ref["Q"] - 1)) < 1e-12
+    assert np.max(np.abs(got["alpha"] - ref["alpha"])) < 1e-9 * np.max(np.abs(ref["alpha"]))
+    assert np.max(np.abs(got["price"] - ref["price"])) < 1e-11
+    assert np.max(np.abs(got["stat"] / ref["stat"] - 1)) < 1e-9
+
+
+def test_turbo_agrees_with_plain_rt_and_reduces_variance(rb):
+    xi, H, eta, rho, T, K = turbo_case()
+    N, M = 64, 200000
+    tb = rb.price_rows_turbo(xi, H, eta, rho, T, K, N, M, [1, 1])
+    p, iv, s = rb.price_rows(xi, H, eta, rho, T, K, N, M, [2, 2])
+    assert np.all(np.abs(tb["price"] - p) <= 4.5 * np.sqrt(tb["stat"] ** 2 + s ** 2))
+    assert np.all(tb["stat"] < s)
+    assert np.all(np.isfinite(tb["iv"]))
+
+
+def test_turbo_argument_errors(rb):
+    xi, H, eta, rho, T, K = turbo_case()
+    with pytest.raises(rb.RbError) as e:
+        rb.price_rows_turbo(xi, H, eta, rho, T, K, 32, 1000, [1], m_control=1)
+    assert e.value.code == 1
+    with pytest.raises(rb.RbError) as e:
+        rb.price_rows_turbo(xi, H, eta, rho, T, K, 32, 1000, [1], q_max=-1.0)
+    assert e.value.code == 1

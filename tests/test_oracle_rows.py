@@ -52,3 +52,60 @@ def test_rows_are_independent_of_order(orc):
     perm = rng.permutation(n)
     b = orc.payoff_rows_samples(xi[perm], H[perm], eta[perm], rho[perm], T[perm], K[perm], N, Z)
     assert np.array_equal(a[:, perm], b)
+
+
+def _turbo_rows_case():
+    K = [0.8, 0.95, 1.0, 1.1, 1.3]
+    n = len(K)
+    return (np.full(n, 0.04), np.full(n, 0.1), np.full(n, 1.5), np.full(n, -0.7),
+            np.full(n, 1.0), np.array(K))
+
+
+def test_turbo_is_unbiased_against_plain_rt(orc):
+    """The turbo-charged estimator (control variate + antithetics, ML_console_ST/RBergomi.cpp:
+    219-288) and the plain RT estimator estimate the same price: within 4.5 combined SE."""
+    xi, H, eta, rho, T, K = _turbo_rows_case()
+    N = 24
+    rng = np.random.default_rng(11)
+    Zc = rng.standard_normal((1000, 2, N))
+    Z = rng.standard_normal((3000, 2, N))
+    tb = orc.turbo_rows(xi, H, eta, rho, T, K, N, Zc, Z)
+    Zp = rng.standard_normal((20000, 2, N))
+    pp, sp = orc.price_stat(orc.payoff_rows_samples(xi, H, eta, rho, T, K, N, Zp))
+    assert np.all(np.abs(tb["price"] - pp) <= 4.5 * np.sqrt(tb["stat"] ** 2 + sp ** 2))
+    # the variance reduction is real: same path count, smaller standard error
+    _, s_same = orc.price_stat(orc.payoff_rows_samples(xi, H, eta, rho, T, K, N, Z))
+    assert np.all(tb["stat"] < s_same)
+    assert np.all(tb["Q"] > 0) and np.all(np.isfinite(tb["alpha"]))
+
+
+def test_turbo_eta_zero_closed_form(orc, ):
+    """eta = 0: v = xi, Iv = xi T on every path, so E[X] = C_BS(1, K, T, sqrt(xi)), Q = 2 xi T
+    exactly, and X, Y are functions of one lognormal S' (the control variate removes most of
+    the variance)."""
+    from scipy.stats import norm
+    xi, H, eta, rho, T, K = _turbo_rows_case()
+    eta = np.zeros_like(eta)
+    N = 16
+    rng = np.random.default_rng(12)
+    tb = orc.turbo_rows(xi, H, eta, rho, T, K, N, rng.standard_normal((500, 2, N)),
+                        rng.standard_normal((4000, 2, N)))
+    s = np.sqrt(xi * T)
+    d1 = -np.log(K) / s + 0.5 * s
+    exact = norm.cdf(d1) - K * norm.cdf(d1 - s)
+    assert np.allclose(tb["Q"], 2 * xi * T, rtol=1e-12)
+    assert np.all(np.abs(tb["price"] - exact) <= 4.5 * tb["stat"] + 1e-12)
+    assert np.all(tb["exceptions"] == 0)
+
+
+def test_turbo_counts_q_exceptions(orc):
+    """With a tiny pilot, main paths can exceed Q = max 2 Iv; those evaluations use qmax and are
+    counted (ML_console_ST/RBergomi.cpp:486-488), and the price stays finite."""
+    xi, H, eta, rho, T, K = _turbo_rows_case()
+    eta = np.full_like(eta, 3.0)
+    N = 16
+    rng = np.random.default_rng(13)
+    tb = orc.turbo_rows(xi, H, eta, rho, T, K, N, rng.standard_normal((3, 2, N)),
+                        rng.standard_normal((2000, 2, N)), qmax=50.0)
+    assert np.all(tb["exceptions"] > 0)
+    assert np.all(np.isfinite(tb["price"])) and np.all(np.isfinite(tb["stat"]))
