@@ -69,7 +69,7 @@ enum rb_status {
 };
 enum rb_precision {
     RB_PREC_FP64 = 0,        /* every stage in FP64 (the parity path)               */
-    RB_PREC_FP32 = 1,        /* Toeplitz GEMM on FP32 MFMA                          */
+    RB_PREC_FP32 = 1,        /* Volterra convolution by FP32 FFT in LDS, the rest FP64 */
     RB_PREC_SPLIT_FP16 = 2,  /* Toeplitz GEMM as FP16 hi+lo products, FP32 accumulate */
     RB_PREC_SPLIT_BF16 = 3   /* Toeplitz GEMM as BF16 hi+lo products, FP32 accumulate */
 };
@@ -78,12 +78,18 @@ enum rb_compensator {
     RB_COMP_EXACT = 1  /* 0.5 eta^2 (i dt)^(2H), clean_Rfiles/hybridScheme.R:110-113 */
 };
 enum rb_method { RB_METHOD_RT = 0, RB_METHOD_MC = 1 };
+/* How Wtilde is sampled.  HYBRID: the reference's hybrid scheme (kappa = 1).  CHOLESKY: exactly,
+ * by the Cholesky factor of the joint covariance of (W1, Wtilde) (RfBm, PremiaRBergomi/RfBm.cpp:
+ * 18-95), FP64 path only, H in (0, 1/2), N <= 2048; the path's two Gaussian blocks are the
+ * factor's inputs (the first is W1).  It has no discretisation bias in Wtilde, so it measures the
+ * hybrid scheme's bias against N. */
+enum rb_scheme { RB_SCHEME_HYBRID = 0, RB_SCHEME_CHOLESKY = 1 };
 
 typedef struct rb_config {
     int precision;    /* rb_precision   */
     int compensator;  /* rb_compensator */
     int method;       /* rb_method      */
-    int reserved;
+    int scheme;       /* rb_scheme (0 in older callers' zero-initialised configs: hybrid) */
 } rb_config;
 
 /* Message of the last failed call on this thread ("" if none). */
@@ -221,6 +227,10 @@ int rb_layout(const double* H, const double* eta, const double* rho, int par_siz
               int* nunits, int* ngroups, int* group_of_option, int* unit_of_group);
 
 /* ---- probes used by the parity tests (device work, FP64) ---- */
+/* Host only: the exact scheme's covariance blocks A21[i][j] = Cov(Wtilde_i, W1_j),
+ * A22[i][k] = Cov(Wtilde_i, Wtilde_k) (N x N each) and its factor L (N rows of 2N); any may be
+ * NULL. */
+int rb_probe_chol(double H, int N, double* A21, double* A22, double* L);
 /* Philox4x32-10 on the device: ctr[4n], key[2] -> out[4n] */
 int rb_probe_philox(const uint32_t* ctr, const uint32_t* key, long n, uint32_t* out);
 /* Device Gaussians of paths [p0, p0+np): out[((p-p0)*nstreams + s)*N + j] */

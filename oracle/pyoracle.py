@@ -58,6 +58,11 @@ def lib():
         L.orc_integrals_rt_samples.argtypes = samp
         L.orc_payoff_mc_samples.argtypes = samp
         L.orc_payoff_rows_samples.argtypes = [_dp, _dp, _dp, _dp, _dp, _dp, i, i, l, _dp, i, _dp]
+        L.orc_hyp2f1.argtypes = [d, d, d, d]
+        L.orc_hyp2f1.restype = d
+        L.orc_chol_cov.argtypes = [d, i, _dp]
+        L.orc_chol_factor.argtypes = [d, i, _dp]
+        L.orc_payoff_chol_samples.argtypes = [d, _dp, _dp, _dp, i, _dp, _dp, i, i, l, _dp, i, i, _dp]
         L.orc_turbo_rows.argtypes = [_dp, _dp, _dp, _dp, _dp, _dp, i, i, l, _dp, l, _dp, d, i,
                                      _dp, _dp, _dp, _dp, ctypes.POINTER(ctypes.c_long)]
         L.orc_price_stat.argtypes = [_dp, l, i, _dp, _dp]
@@ -194,6 +199,36 @@ def turbo_rows(xi, H, eta, rho, T, K, N, Zc, Z, qmax=10000.0, comp=COMP_REF):
                          exc.ctypes.data_as(ctypes.POINTER(ctypes.c_long)))
     out["exceptions"] = exc
     return out
+
+
+def hyp2f1(a, b, c, z):
+    return lib().orc_hyp2f1(a, b, c, z)
+
+
+def chol_cov(H, n):
+    A = np.zeros((2 * n, 2 * n))
+    lib().orc_chol_cov(H, n, A.ctypes.data_as(_dp))
+    return A
+
+
+def chol_factor(H, n):
+    L = np.zeros((2 * n, 2 * n))
+    if lib().orc_chol_factor(H, n, L.ctypes.data_as(_dp)) != 0:
+        raise ValueError("covariance not positive definite")
+    return L
+
+
+def payoff_chol_samples(xi, H, eta, rho, T, K, N, Z, comp=COMP_REF, method=0):
+    """per-path payoffs with the exact (Cholesky) Wtilde; Z[M][2 or 3][N]"""
+    keep, args = _params(H, eta, rho, T, K)
+    Z = np.ascontiguousarray(Z, dtype=np.float64)
+    M = Z.shape[0]
+    pay = np.zeros((M, args[3] * args[6]))
+    rc = lib().orc_payoff_chol_samples(xi, *args, N, M, Z.ctypes.data_as(_dp), comp, method,
+                                       pay.ctypes.data_as(_dp))
+    if rc != 0:
+        raise ValueError("covariance not positive definite")
+    return pay
 
 
 def price_stat(payoff):

@@ -443,6 +443,119 @@ void orc_turbo_rows(const double* xi, const double* H, const double* eta, const 
     }
 }
 
+static double orc_gauss_series(double a, double b, double c, double z) {
+    double term = 1.0, sum = 1.0;
+    for (int k = 0; k < 2000; ++k) {
+        term *= (a + k) * (b + k) / ((c + k) * (k + 1.0)) * z;
+        sum += term;
+        if (std::fabs(term) < 1e-18 * std::fabs(sum)) break;
+    }
+    return sum;
+}
+
+double orc_hyp2f1(double a, double b, double c, double z) {
+    if (z <= 0.5) return orc_gauss_series(a, b, c, z);
+    const double w = 1.0 - z, s = c - a - b;
+    return std::tgamma(c) * std::tgamma(s) / (std::tgamma(c - a) * std::tgamma(c - b)) *
+               orc_gauss_series(a, b, 1.0 - s, w) +
+           std::pow(w, s) * std::tgamma(c) * std::tgamma(-s) / (std::tgamma(a) * std::tgamma(b)) *
+               orc_gauss_series(c - a, c - b, 1.0 + s, w);
+}
+
+// RfBm::A / G / cBm / cfBm / cMixed, PremiaRBergomi/RfBm.cpp:18-58
+static double rfbm_G(double H, double x) {
+    const double g = 0.5 - H;
+    if (g == 0.0) return 1.0;
+    return 2.0 * H * (std::pow(x, -g) / (1.0 - g) +
+                      g * std::pow(x, -(1.0 + g)) * orc_hyp2f1(1.0, 1.0 + g, 3.0 - g, 1.0 / x) /
+                          ((1.0 - g) * (2.0 - g)));
+}
+
+static double rfbm_A(double H, int n, int i, int j) {
+    if (i < n && j < n) return i == j ? 1.0 : 0.0;
+    if (i >= n && j >= n) {
+        const double t = double(std::min(i, j) - n + 1) / n, s = double(std::max(i, j) - n + 1) / n;
+        return std::pow(t, 2 * H) * rfbm_G(H, s / t);
+    }
+    const int b = i < n ? i : j, f = i < n ? j - n : i - n;  // Bm index b, fBm index f
+    if (b > f) return 0.0;
+    const double u = double(b) / n, v = double(b + 1) / n, t = double(f + 1) / n;
+    return 2.0 * std::sqrt(2.0 * H * n) * (std::pow(t - u, 0.5 + H) - std::pow(t - v, 0.5 + H)) /
+           (1.0 + 2.0 * H);
+}
+
+void orc_chol_cov(double H, int n, double* A) {
+    for (int i = 0; i < 2 * n; ++i)
+        for (int j = 0; j < 2 * n; ++j) A[size_t(i) * 2 * n + j] = rfbm_A(H, n, i, j);
+}
+
+int orc_chol_factor(double H, int n, double* L) {
+    const int m = 2 * n;
+    std::vector<double> A(size_t(m) * m);
+    orc_chol_cov(H, n, A.data());
+    for (size_t k = 0; k < size_t(m) * m; ++k) L[k] = 0.0;
+    for (int k = 0; k < m; ++k) {  // column-by-column as the RfBm constructor
+        double temp = 0.0;
+        for (int j = 0; j < k; ++j) temp += L[size_t(k) * m + j] * L[size_t(k) * m + j];
+        const double d = A[size_t(k) * m + k] - temp;
+        if (!(d > 0.0)) return -1;
+        L[size_t(k) * m + k] = std::sqrt(d);
+        for (int i = k + 1; i < m; ++i) {
+            double t2 = 0.0;
+            for (int j = 0; j < k; ++j) t2 += L[size_t(i) * m + j] * L[size_t(k) * m + j];
+            L[size_t(i) * m + k] = (A[size_t(i) * m + k] - t2) / L[size_t(k) * m + k];
+        }
+    }
+    return 0;
+}
+
+int orc_payoff_chol_samples(double xi, const double* H, const double* eta, const double* rho,
+                            int par_size, const double* T, const double* K, int op_size, int N,
+                            long M, const double* Z, int comp_mode, int method, double* payoff) {
+    const int ns = method == 1 ? 3 : 2;
+    const long out = long(par_size) * op_size;
+    // one factor per parameter set (the reference builds one RfBm per H)
+    std::vector<std::vector<double>> Ls(par_size, std::vector<double>(size_t(4) * N * N));
+    for (int h = 0; h < par_size; ++h)
+        if (orc_chol_factor(H[h], N, Ls[h].data())) return -1;
+#pragma omp parallel
+    {
+        Vec Wt(N), v(N);
+#pragma omp for schedule(static)
+        for (long m = 0; m < M; ++m) {
+            const double* x = Z + size_t(m) * ns * N;  // x1 = W1, x2, (Wperp)
+            for (int h = 0; h < par_size; ++h) {
+                const double* L = Ls[h].data();
+                for (int i = 0; i < N; ++i) {  // Wtilde_i = row n+i of L times (x1, x2)
+                    double w = 0.0;
+                    for (int j = 0; j <= N + i; ++j) w += L[size_t(N + i) * 2 * N + j] * x[j];
+                    Wt[i] = w;
+                }
+                for (int t = 0; t < op_size; ++t) {
+                    orc_update_v(Wt.data(), xi, H[h], eta[h], T[t], N, comp_mode, v.data());
+                    const double dt = T[t] / N, sdt = std::sqrt(dt);
+                    double pay;
+                    if (method == 1) {
+                        // log-Euler with Z = rho W1 + sqrt(1 - rho^2) Wperp (updateZ/updateS)
+                        const double c = std::sqrt(1.0 - rho[h] * rho[h]);
+                        double lnS = 0.0;
+                        for (int i = 0; i < N; ++i)
+                            lnS += std::sqrt(v[i]) * sdt * (rho[h] * x[i] + c * x[2 * N + i]) - 0.5 * v[i] * dt;
+                        const double S = std::exp(lnS);
+                        pay = S > K[t] ? S - K[t] : 0.0;
+                    } else {
+                        const double Iv = orc_int_vdt(v.data(), N, dt);
+                        const double Is = orc_int_rootvdw(v.data(), x, N, sdt);
+                        pay = orc_rt_payoff(Iv, Is, rho[h], K[t]);
+                    }
+                    payoff[m * out + long(h) * op_size + t] = pay;
+                }
+            }
+        }
+    }
+    return 0;
+}
+
 void orc_payoff_mc_samples(double xi, const double* H, const double* eta, const double* rho,
                            int par_size, const double* T, const double* K, int op_size,
                            int N, long M, const double* Z, int comp_mode, double* payoff) {

@@ -28,6 +28,10 @@
  *   orc_turbo_rows       unbiased against the plain RT oracle (within 4.5 combined SE); eta = 0
  *                        closed form C_BS(1, K, T, sqrt(xi)); the control-variate mean equals its
  *                        closed-form adjustment; variance below the plain estimator's   (test_oracle_rows.py)
+ *   orc_hyp2f1           scipy.special.hyp2f1 over the (a, b, c, z) the covariance uses    (test_chol_cpu.py)
+ *   orc_chol_cov         Var Wtilde_t = t^(2H); H = 1/2 gives Brownian motion (min(s, t));
+ *                        quadrature of 2H int (t-u)^(H-1/2) (s-u)^(H-1/2) du             (test_chol_cpu.py)
+ *   orc_chol_factor      L L^T = A                                                      (test_chol_cpu.py)
  *   orc_payoff_rows_samples  equals orc_payoff_rt_samples on the zipped tensor laid out as rows;
  *                        per-row xi equals separate scalar-xi runs; row-order independent
  *                        (test_oracle_rows.py)
@@ -135,6 +139,26 @@ void orc_turbo_rows(const double* xi, const double* H, const double* eta, const 
                     const double* T, const double* K, int rows, int N, long Mc, const double* Zc,
                     long M, const double* Z, double qmax, int comp_mode, double* price,
                     double* stat, double* alpha, double* Q, long* exceptions);
+
+/* Gauss hypergeometric 2F1(a, b; c; z) for 0 <= z <= 1 with c - a - b > 0 non-integer: the Gauss
+ * series, and for z > 1/2 the transformation to 1 - z (Abramowitz & Stegun 15.3.6).  Stands in
+ * for gsl_sf_hyperg_2F1 (GSL is absent), used by RfBm::G, PremiaRBergomi/RfBm.cpp:31-37. */
+double orc_hyp2f1(double a, double b, double c, double z);
+
+/* The exact joint covariance of (W1_0..W1_{n-1}, Wtilde_0..Wtilde_{n-1}) as RfBm::A
+ * (PremiaRBergomi/RfBm.cpp:18-58), A[2n][2n] row-major, and its full 2n x 2n Cholesky factor as
+ * the RfBm constructor computes it (:63-80), L[2n][2n].  Returns 0, or -1 if A is not positive
+ * definite. */
+void orc_chol_cov(double H, int n, double* A);
+int orc_chol_factor(double H, int n, double* L);
+
+/* Per-path payoffs with Wtilde sampled exactly (RfBm::generate, :82-95): path m's first two
+ * blocks x1, x2 of Z give W1 = x1 and Wtilde = L (x1, x2) (rows n..2n-1 of L); the rest as
+ * orc_payoff_rt_samples (method 0) or orc_payoff_mc_samples (method 1, third block Wperp).
+ * updatePayoff_cholesky, PremiaRBergomi/rBergomi.cpp:190-204, for the RT payoff. */
+int orc_payoff_chol_samples(double xi, const double* H, const double* eta, const double* rho,
+                            int par_size, const double* T, const double* K, int op_size, int N,
+                            long M, const double* Z, int comp_mode, int method, double* payoff);
 
 /* price/stat from a payoff matrix [M][out_size]: mean and sqrt(E[X^2]-mean^2)/sqrt(M),
  * rBergLib/rBergomiMT.cpp:404-409 */

@@ -5,7 +5,8 @@ include/interface.h); this package builds it and mirrors the reference's pricing
 in Python through ctypes.
 """
 from ._lib import (COMP_EXACT, COMP_REF, METHOD_MC, METHOD_RT, PREC_FP32, PREC_FP64,
-                   PREC_SPLIT_BF16, PREC_SPLIT_FP16, RbError, load)
+                   PREC_SPLIT_BF16, PREC_SPLIT_FP16, SCHEME_CHOLESKY, SCHEME_HYBRID, RbError,
+                   load)
 from .cinterface import (Plan, computeIV, computeIV_RT, computeIVRT_tensor_quadrature, layout,
                          payoff_rows_samples, payoff_samples_v2, price_rows,
                          price_rows_turbo, price_v2)
@@ -13,4 +14,4 @@ from .cinterface import (Plan, computeIV, computeIV_RT, computeIVRT_tensor_quadr
 __all__ = ["computeIV", "computeIV_RT", "computeIVRT_tensor_quadrature", "price_v2",
            "payoff_samples_v2", "price_rows", "price_rows_turbo", "payoff_rows_samples", "Plan", "layout", "load", "RbError", "PREC_FP64", "PREC_FP32",
            "PREC_SPLIT_FP16", "PREC_SPLIT_BF16", "COMP_REF", "COMP_EXACT", "METHOD_RT",
-           "METHOD_MC"]
+           "METHOD_MC", "SCHEME_HYBRID", "SCHEME_CHOLESKY"]

@@ -17,12 +17,13 @@ u64p = ctypes.POINTER(ctypes.c_uint64)
 
 class RbConfig(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("compensator", ctypes.c_int),
-                ("method", ctypes.c_int), ("reserved", ctypes.c_int)]
+                ("method", ctypes.c_int), ("scheme", ctypes.c_int)]
 
 
 PREC_FP64, PREC_FP32, PREC_SPLIT_FP16, PREC_SPLIT_BF16 = 0, 1, 2, 3
 COMP_REF, COMP_EXACT = 0, 1
 METHOD_RT, METHOD_MC = 0, 1
+SCHEME_HYBRID, SCHEME_CHOLESKY = 0, 1
 STATUS = {0: "RB_OK", 1: "RB_ERR_ARG", 2: "RB_ERR_SIZE", 3: "RB_ERR_DEVICE", 4: "RB_ERR_NOMEM",
           5: "RB_ERR_LIMIT"}
 
@@ -80,6 +81,7 @@ def load():
     L.rb_probe_mfma_layout.argtypes = [fp, fp, i, fp]
     L.rb_probe_payoff.argtypes = [dp, dp, dp, i, dp, fp, fp]
     L.rb_probe_copy.argtypes = [l, i, dp]
+    L.rb_probe_chol.argtypes = [d, i, dp, dp, dp]
     L.rb_probe_integrals.argtypes = [d, dp, dp, dp, i, dp, dp, i, i, l, dp, cfgp, dp]
     _lib = L
     return _lib

@@ -71,12 +71,12 @@ def computeIVRT_tensor_quadrature(xi, H, eta, rho, T, K, N, Z, num_threads=1):
     return out
 
 
-def _cfg(precision=_lib.PREC_FP64, compensator=_lib.COMP_REF, method=_lib.METHOD_RT):
-    return RbConfig(precision, compensator, method, 0)
+def _cfg(precision=_lib.PREC_FP64, compensator=_lib.COMP_REF, method=_lib.METHOD_RT, scheme=0):
+    return RbConfig(precision, compensator, method, scheme)
 
 
 def price_v2(xi, H, eta, rho, T, K, N, M, seed, precision=_lib.PREC_FP64,
-             compensator=_lib.COMP_REF, method=_lib.METHOD_RT):
+             compensator=_lib.COMP_REF, method=_lib.METHOD_RT, scheme=0):
     """Status-checked pricing (rb_price_v2) -> (price, iv, stat); raises RbError on failure."""
     H, pH = _arr(H)
     eta, pe = _arr(eta)
@@ -86,7 +86,7 @@ def price_v2(xi, H, eta, rho, T, K, N, M, seed, precision=_lib.PREC_FP64,
     s, ps = _seed(seed)
     out = len(H) * len(T)
     price, iv, stat = np.zeros(out), np.zeros(out), np.zeros(out)
-    cfg = _cfg(precision, compensator, method)
+    cfg = _cfg(precision, compensator, method, scheme)
     check(_lib.load().rb_price_v2(float(xi), pH, pe, pr, len(H), pT, pK, len(T), int(N), int(M), ps,
                                   len(s), ctypes.byref(cfg), price.ctypes.data_as(dp),
                                   iv.ctypes.data_as(dp), stat.ctypes.data_as(dp), out))
@@ -94,7 +94,7 @@ def price_v2(xi, H, eta, rho, T, K, N, M, seed, precision=_lib.PREC_FP64,
 
 
 def payoff_samples_v2(xi, H, eta, rho, T, K, N, Z, method=_lib.METHOD_RT,
-                      compensator=_lib.COMP_REF):
+                      compensator=_lib.COMP_REF, scheme=0):
     """Per-path payoffs from user Gaussians -> (payoff [M, out], price, stat)"""
     H, pH = _arr(H)
     eta, pe = _arr(eta)
@@ -107,7 +107,7 @@ def payoff_samples_v2(xi, H, eta, rho, T, K, N, Z, method=_lib.METHOD_RT,
     out = len(H) * len(T)
     payoff = np.zeros((M, out))
     price, stat = np.zeros(out), np.zeros(out)
-    cfg = _cfg(_lib.PREC_FP64, compensator, method)
+    cfg = _cfg(_lib.PREC_FP64, compensator, method, scheme)
     check(_lib.load().rb_payoff_samples_v2(float(xi), pH, pe, pr, len(H), pT, pK, len(T), int(N),
                                            int(M), pZ, ctypes.byref(cfg), payoff.ctypes.data_as(dp),
                                            price.ctypes.data_as(dp), stat.ctypes.data_as(dp)))
@@ -115,7 +115,7 @@ def payoff_samples_v2(xi, H, eta, rho, T, K, N, Z, method=_lib.METHOD_RT,
 
 
 def price_rows(xi, H, eta, rho, T, K, N, M, seed, precision=_lib.PREC_FP64,
-               compensator=_lib.COMP_REF, method=_lib.METHOD_RT):
+               compensator=_lib.COMP_REF, method=_lib.METHOD_RT, scheme=0):
     """Unordered rows (rb_price_rows): row i prices (T[i], K[i]) under (xi[i], H[i], eta[i],
     rho[i]) on shared paths; xi may be a scalar.  -> (price, iv, stat), one entry per row."""
     n = len(H)
@@ -125,7 +125,7 @@ def price_rows(xi, H, eta, rho, T, K, N, M, seed, precision=_lib.PREC_FP64,
         raise ValueError("every row array needs one entry per row")
     s, ps = _seed(seed)
     price, iv, stat = np.zeros(n), np.zeros(n), np.zeros(n)
-    cfg = _cfg(precision, compensator, method)
+    cfg = _cfg(precision, compensator, method, scheme)
     check(_lib.load().rb_price_rows(*[p for _, p in arrs], n, int(N), int(M), ps, len(s),
                                     ctypes.byref(cfg), price.ctypes.data_as(dp),
                                     iv.ctypes.data_as(dp), stat.ctypes.data_as(dp)))
@@ -153,7 +153,7 @@ def price_rows_turbo(xi, H, eta, rho, T, K, N, M, seed, m_control=10000, q_max=1
 
 
 def payoff_rows_samples(xi, H, eta, rho, T, K, N, Z, compensator=_lib.COMP_REF,
-                        method=_lib.METHOD_RT):
+                        method=_lib.METHOD_RT, scheme=0):
     """Per-path payoffs of unordered rows from caller Gaussians Z[M][streams][N] (FP64 path)
     -> (payoff[M][rows], price, stat)."""
     n = len(H)
@@ -163,7 +163,7 @@ def payoff_rows_samples(xi, H, eta, rho, T, K, N, Z, compensator=_lib.COMP_REF,
     M = Z.shape[0]
     payoff = np.zeros((M, n))
     price, stat = np.zeros(n), np.zeros(n)
-    cfg = _cfg(_lib.PREC_FP64, compensator, method)
+    cfg = _cfg(_lib.PREC_FP64, compensator, method, scheme)
     check(_lib.load().rb_payoff_rows_samples(*[p for _, p in arrs], n, int(N), int(M),
                                              Z.ctypes.data_as(dp), ctypes.byref(cfg),
                                              payoff.ctypes.data_as(dp), price.ctypes.data_as(dp),
@@ -181,11 +181,11 @@ class Plan:
 
     def __init__(self, xi, H, eta, rho, T, K, N, M, seed, precision=_lib.PREC_FP64,
                  compensator=_lib.COMP_REF, method=_lib.METHOD_RT, profile=False, graph=False,
-                 stream=None):
+                 stream=None, scheme=0):
         self._keep = [_arr(H), _arr(eta), _arr(rho), _arr(T), _arr(K), _seed(seed)]
         (H, pH), (eta, pe), (rho, pr), (T, pT), (K, pK), (s, ps) = self._keep
         self.out_size = len(H) * len(T)
-        self.cfg = _cfg(precision, compensator, method)
+        self.cfg = _cfg(precision, compensator, method, scheme)
         self.ptr = ctypes.c_void_p()
         L = _lib.load()
         check(L.rb_plan_create(float(xi), pH, pe, pr, len(H), pT, pK, len(T), int(N), int(M), ps,

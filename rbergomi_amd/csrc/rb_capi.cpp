@@ -4,6 +4,7 @@
 // (rBergLib/interface.cpp:11-52), and the status-returning v2 / plan / probe API.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -46,6 +47,7 @@ Request make_request(double xi, const double* H, const double* eta, const double
         r.precision = cfg->precision;
         r.comp = cfg->compensator;
         r.method = cfg->method;
+        r.scheme = cfg->scheme;
     }
     return r;
 }
@@ -397,6 +399,23 @@ int rb_layout(const double* H, const double* eta, const double* rho, int par_siz
             goto done;                                                             \
         }                                                                          \
     } while (0)
+
+int rb_probe_chol(double H, int N, double* A21, double* A22, double* L) {
+    set_last_error("");
+    if (N < 1 || N > 2048) return fail(RB_ERR_ARG, "N in [1, 2048]");
+    std::vector<double> a21(size_t(N) * N), a22(size_t(N) * N);
+    chol_covariance(H, N, a21.data(), a22.data());
+    if (A21) std::copy(a21.begin(), a21.end(), A21);
+    if (A22) std::copy(a22.begin(), a22.end(), A22);
+    if (L) {
+        std::vector<double> f;
+        std::string err;
+        const int rc = chol_factor(H, N, f, err);
+        if (rc) return fail(rc, err);
+        std::copy(f.begin(), f.end(), L);
+    }
+    return RB_OK;
+}
 
 int rb_probe_philox(const uint32_t* ctr, const uint32_t* key, long n, uint32_t* out) {
     int rc = RB_OK;

@@ -48,6 +48,10 @@ int validate(const Request& r, std::string& err) {
         err = "outputs above the supported maximum of 65535 per call";
         return ST_ERR_LIMIT;
     }
+    if (r.scheme != SCHEME_HYBRID && r.scheme != SCHEME_CHOLESKY) {
+        err = "unknown scheme";
+        return ST_ERR_ARG;
+    }
     if (r.N < 1 || r.M < 1) {
         err = "N and M must be positive";
         return ST_ERR_ARG;
@@ -219,7 +223,7 @@ Plan::~Plan() {
     if (stream) (void)hipStreamSynchronize(stream);
     void* bufs[] = {d_units, d_gamma, d_groups, d_comp, d_opts,  d_Z,     d_Wt,   d_ints,
                     d_partial, d_price, d_stat, d_iv, d_payoff, d_tw, d_ghat, d_ints2,
-                    d_Qg, d_alpha, d_exc, d_count};
+                    d_Qg, d_alpha, d_exc, d_count, d_chol};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
@@ -256,6 +260,14 @@ int Plan::init(const Request& r, hipStream_t st, bool quadrature, std::string& e
     if (r.turbo && (r.precision != PREC_FP64 || r.method != METHOD_RT || quadrature)) {
         err = "the turbo-charged estimator runs the Romano-Touzi FP64 path on device Gaussians";
         return ST_ERR_ARG;
+    }
+    if (r.scheme == SCHEME_CHOLESKY && r.precision != PREC_FP64) {
+        err = "the Cholesky scheme runs on the FP64 path";
+        return ST_ERR_ARG;
+    }
+    if (r.scheme == SCHEME_CHOLESKY && r.N > 2048) {
+        err = "the Cholesky scheme covers N <= 2048";
+        return ST_ERR_LIMIT;
     }
     if (r.precision == PREC_FP32 || (split && (r.N > 1024 || quadrature))) {
         // staged pipeline with the FP32 FFT convolution (long paths, user Gaussians)
@@ -317,6 +329,16 @@ int Plan::init(const Request& r, hipStream_t st, bool quadrature, std::string& e
     if (conv_fft) {
         rc = init_fft(err);
         if (rc) return rc;
+    }
+    if (r.scheme == SCHEME_CHOLESKY) {
+        // host factor per unit (distinct H), then one upload
+        std::vector<double> all, L;
+        for (const Unit& un : lay.units) {
+            rc = chol_factor(un.H, N, L, err);
+            if (rc) return rc;
+            all.insert(all.end(), L.begin(), L.end());
+        }
+        RB_TRY(upload(&d_chol, all, stream));
     }
     RB_TRY(dalloc(&d_Z, size_t(chunk) * nstreams * N));
     RB_TRY(dalloc(&d_Wt, size_t(nunits) * chunk * N));
@@ -538,6 +560,8 @@ int Plan::init_fft(std::string& err) {
 }
 
 hipError_t Plan::conv(long np, long zstride) {
+    if (req.scheme == SCHEME_CHOLESKY)
+        return launch_volterra_chol(d_Z, zstride, np, req.N, int(lay.units.size()), d_chol, d_Wt, stream);
     if (conv_fft)
         return launch_volterra_fft(d_Z, zstride, np, req.N, log2L, int(lay.units.size()), d_units, d_ghat,
                                    d_tw, d_Wt, stream);

@@ -42,6 +42,7 @@ struct Request {
     int comp = COMP_REF;
     // turbo-charged RT estimator (control variate + antithetics, ML_console_ST/RBergomi.cpp:219-288):
     // m_control pilot paths estimate Q and alpha; q_max replaces Q where Q <= Iv
+    int scheme = 0;              // 0 hybrid scheme, 1 exact Cholesky sampling of Wtilde
     bool turbo = false;
     long m_control = 10000;
     double q_max = 10000.0;
@@ -57,6 +58,14 @@ struct Layout {
     std::vector<double> comp;   // [groups][N]: c_i = eta^2/2 ((i-1) dt)^(2H), c_0 = 0
     std::vector<Option> options;
 };
+
+enum Scheme { SCHEME_HYBRID = 0, SCHEME_CHOLESKY = 1 };
+// exact sampling: Gauss hypergeometric 2F1 for 0 <= z <= 1 (c - a - b non-integer), the joint
+// covariance blocks A21 = Cov(Wtilde, W1), A22 = Cov(Wtilde, Wtilde) (n x n each), and the factor
+// L = [A21 | chol(A22 - A21 A21^T)] (n rows of 2n)
+double hyp2f1(double a, double b, double c, double z);
+void chol_covariance(double H, int n, double* A21, double* A22);
+int chol_factor(double H, int n, std::vector<double>& L, std::string& err);
 
 // Philox path counters of the turbo-charged estimator's pilot paths (disjoint from main paths)
 constexpr long TURBO_PILOT_BASE = 1L << 62;
@@ -101,6 +110,7 @@ struct Plan {
     // turbo-charged estimator
     double* d_ints2 = nullptr;    // [groups][chunk][2] integrals of the antithetic paths
     double* d_Qg = nullptr;       // [groups] Q = max over the pilot paths of 2 Iv
+    double* d_chol = nullptr;     // [units][N][2N] Cholesky factors (exact scheme)
     double* d_alpha = nullptr;    // [options] control-variate coefficients
     int* d_exc = nullptr;         // [options][nblk] Q <= Iv evaluations per path block
     long* d_count = nullptr;      // [options] their totals

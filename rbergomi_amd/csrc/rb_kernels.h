@@ -47,6 +47,9 @@ hipError_t launch_volterra_fft(const double* Z, long zstride, long np, int N, in
                                const Unit* units, const float2* ghat, const float2* tw, double* Wt,
                                hipStream_t st);
 
+// exact scheme: Wtilde = [L21 | L22] (x1, x2) per unit
+hipError_t launch_volterra_chol(const double* Z, long zstride, long np, int n, int nunits,
+                                const double* Lc, double* Wt, hipStream_t st);
 // turbo-charged RT estimator (control variate + antithetics)
 hipError_t launch_turbo_q(const double* ints, long np, int ngroups, double* Qg, hipStream_t st);
 hipError_t launch_turbo_alpha(const double* ints, long np, const Option* opts, int nopt,
