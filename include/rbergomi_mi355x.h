@@ -133,6 +133,17 @@ int rb_plan_kernel_times(rb_plan* plan, char* names, double* total_ms, int* laun
 int rb_plan_info(rb_plan* plan, int* nunits, int* ngroups, int* noptions, long* chunk);
 void rb_plan_destroy(rb_plan* plan);
 
+/* Quasi-Monte Carlo: path m (0-based) uses Sobol' point m + 1 in 2N dimensions (3N for plain
+ * MC), mapped by the inverse normal CDF; dimension 2j goes to W1_j and 2j+1 to W1perp_j (breakZ),
+ * as ComputePriceRTMT_sobol / ComputeIVRTMT_sobol (rBergomi/rBergomiMT.cpp:610-716) with
+ * normalQMC_sample (rBergomi/qmc.cpp:26-41).  The direction numbers are this library's own
+ * (primitive polynomials by degree, seeded odd initial numbers; the reference's table is not
+ * reproduced), so the points differ from the reference's beyond the first dimension.  No seed;
+ * M <= 2^32 - 2.  cfg as for rb_price_v2 (split precisions take the FP32-FFT staged path). */
+int rb_price_sobol(double xi, const double* H, const double* eta, const double* rho, int par_size,
+                   const double* T, const double* K, int op_size, int N, long M,
+                   const rb_config* cfg, double* price, double* iv, double* stat, int out_size);
+
 /* ---- unordered rows ----
  * Row i prices the call (T[i], K[i]) under (xi[i], H[i], eta[i], rho[i]); every row uses the same
  * Gaussians of each path, and outputs are indexed by row.  Replaces ComputePriceRTMTunstructured /
@@ -227,6 +238,11 @@ int rb_layout(const double* H, const double* eta, const double* rho, int par_siz
               int* nunits, int* ngroups, int* group_of_option, int* unit_of_group);
 
 /* ---- probes used by the parity tests (device work, FP64) ---- */
+/* Host only: Sobol' points i0 .. i0+n-1 of the quasi-Monte Carlo source in dims dimensions,
+ * u[p*dims + d] (the generator behind rb_price_sobol). */
+int rb_probe_sobol(int dims, long i0, long n, double* u);
+/* The device inverse normal CDF of the quasi-Monte Carlo source: z[i] = Phi^-1(u[i]). */
+int rb_probe_norm_inv(const double* u, long n, double* z);
 /* Host only: the exact scheme's covariance blocks A21[i][j] = Cov(Wtilde_i, W1_j),
  * A22[i][k] = Cov(Wtilde_i, Wtilde_k) (N x N each) and its factor L (N rows of 2N); any may be
  * NULL. */

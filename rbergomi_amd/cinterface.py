@@ -114,6 +114,30 @@ def payoff_samples_v2(xi, H, eta, rho, T, K, N, Z, method=_lib.METHOD_RT,
     return payoff, price, stat
 
 
+def price_sobol(xi, H, eta, rho, T, K, N, M, precision=_lib.PREC_FP64, compensator=_lib.COMP_REF,
+                method=_lib.METHOD_RT, scheme=0):
+    """Quasi-Monte Carlo pricing on Sobol' points (rb_price_sobol) -> (price, iv, stat)."""
+    H, pH = _arr(H)
+    eta, pe = _arr(eta)
+    rho, pr = _arr(rho)
+    T, pT = _arr(T)
+    K, pK = _arr(K)
+    out = len(H) * len(T)
+    price, iv, stat = np.zeros(out), np.zeros(out), np.zeros(out)
+    cfg = _cfg(precision, compensator, method, scheme)
+    check(_lib.load().rb_price_sobol(float(xi), pH, pe, pr, len(H), pT, pK, len(T), int(N), int(M),
+                                     ctypes.byref(cfg), price.ctypes.data_as(dp), iv.ctypes.data_as(dp),
+                                     stat.ctypes.data_as(dp), out))
+    return price, iv, stat
+
+
+def sobol_points(dims, i0, n):
+    """Host Sobol' points i0 .. i0+n-1 of the quasi-Monte Carlo source: u[n][dims]."""
+    u = np.zeros((n, dims))
+    check(_lib.load().rb_probe_sobol(int(dims), int(i0), int(n), u.ctypes.data_as(dp)))
+    return u
+
+
 def price_rows(xi, H, eta, rho, T, K, N, M, seed, precision=_lib.PREC_FP64,
                compensator=_lib.COMP_REF, method=_lib.METHOD_RT, scheme=0):
     """Unordered rows (rb_price_rows): row i prices (T[i], K[i]) under (xi[i], H[i], eta[i],

@@ -209,6 +209,56 @@ int rb_price_v2(double xi, const double* H, const double* eta, const double* rho
     return rc;
 }
 
+int rb_price_sobol(double xi, const double* H, const double* eta, const double* rho, int par_size,
+                   const double* T, const double* K, int op_size, int N, long M,
+                   const rb_config* cfg, double* price, double* iv, double* stat, int out_size) {
+    set_last_error("");
+    if (out_size != par_size * op_size) return fail(RB_ERR_SIZE, "out_size != par_size*op_size");
+    if (par_size < 1 || op_size < 1) return fail(RB_ERR_ARG, "par_size and op_size must be positive");
+    if (!H || !eta || !rho || !T || !K || !price || !stat) return fail(RB_ERR_ARG, "null array");
+    Request r = make_request(xi, H, eta, rho, par_size, T, K, op_size, N, M, nullptr, 0, cfg);
+    r.gauss = 1;
+    std::string err;
+    Plan p;
+    int rc = p.init(r, nullptr, false, err);
+    if (!rc) rc = p.run(err);
+    if (!rc) rc = p.results(price, iv, stat, err);
+    if (rc) return fail(rc, err);
+    return RB_OK;
+}
+
+int rb_probe_sobol(int dims, long i0, long n, double* u) {
+    set_last_error("");
+    if (dims < 1 || n < 1 || i0 < 0 || !u) return fail(RB_ERR_ARG, "bad arguments");
+    std::vector<uint32_t> V;
+    sobol_directions(dims, V);
+    std::vector<double> out(size_t(n) * dims);
+    for (long p = 0; p < n; ++p)
+        for (int d = 0; d < dims; ++d) {
+            const uint64_t i = uint64_t(i0 + p);
+            uint32_t x = 0;
+            for (int k = 0; k < 32; ++k)
+                if (i >> k & 1) x ^= V[size_t(d) * 32 + k];
+            out[size_t(p) * dims + d] = double(x) * 2.3283064365386963e-10;
+        }
+    std::copy(out.begin(), out.end(), u);
+    return RB_OK;
+}
+
+int rb_probe_norm_inv(const double* u, long n, double* z) {
+    set_last_error("");
+    if (!u || !z || n < 1) return fail(RB_ERR_ARG, "bad arguments");
+    double *du = nullptr, *dz = nullptr;
+    hipError_t e = hipMalloc(&du, n * sizeof(double));
+    if (!e) e = hipMalloc(&dz, n * sizeof(double));
+    if (!e) e = hipMemcpy(du, u, n * sizeof(double), hipMemcpyHostToDevice);
+    if (!e) e = launch_norm_inv_probe(du, n, dz, nullptr);
+    if (!e) e = hipMemcpy(z, dz, n * sizeof(double), hipMemcpyDeviceToHost);
+    if (du) (void)hipFree(du);
+    if (dz) (void)hipFree(dz);
+    return e ? fail(RB_ERR_DEVICE, hipGetErrorString(e)) : RB_OK;
+}
+
 int rb_payoff_samples_v2(double xi, const double* H, const double* eta, const double* rho,
                          int par_size, const double* T, const double* K, int op_size, int N,
                          long M, const double* Z, const rb_config* cfg, double* payoff,

@@ -85,6 +85,24 @@ __device__ __forceinline__ void normals4_f32(PhiloxKey key, uint64_t p, uint32_t
 // Phi(x) = erfc(-x/sqrt 2)/2, rBergLib/BlackScholes.cpp:16-19
 __device__ __forceinline__ double phi_f64(double x) { return 0.5 * erfc(-x * 0.7071067811865476); }
 
+// Inverse standard normal CDF in FP64 for 0 < u < 1: the rational start of Abramowitz & Stegun
+// 26.2.23 (|error| < 4.5e-4) on q = min(u, 1 - u), then three Halley steps on Phi(x) - q (cubic
+// convergence: 4.5e-4 -> 1e-10 -> machine precision).  Used for Sobol' points as normalQMC_sample
+// does with Phi_inv (rBergomi/qmc.cpp:26-41).
+__device__ inline double norm_inv_f64(double u) {
+    const double q = u < 0.5 ? u : 1.0 - u;
+    const double t = sqrt(-2.0 * log(q));
+    double x = -(t - (2.515517 + t * (0.802853 + t * 0.010328)) /
+                         (1.0 + t * (1.432788 + t * (0.189269 + t * 0.001308))));
+    for (int it = 0; it < 3; ++it) {
+        const double f = phi_f64(x) - q;
+        const double d = 0.3989422804014327 * exp(-0.5 * x * x);
+        const double r = f / d;
+        x -= r / (1.0 + 0.5 * x * r);
+    }
+    return u < 0.5 ? x : -x;
+}
+
 // Romano-Touzi payoff: C_BS(S', K, tau = 1, sigma) with sigma = sqrt((1-rho^2) Iv),
 // ln S' = -rho^2 Iv / 2 + rho Is  (rBergLib/rBergomiMT.cpp:383-388, BlackScholes.cpp:22-26).
 // ln(S'/K) is formed as lnS - lnK, which avoids a log per strike.

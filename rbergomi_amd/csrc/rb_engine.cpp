@@ -223,7 +223,7 @@ Plan::~Plan() {
     if (stream) (void)hipStreamSynchronize(stream);
     void* bufs[] = {d_units, d_gamma, d_groups, d_comp, d_opts,  d_Z,     d_Wt,   d_ints,
                     d_partial, d_price, d_stat, d_iv, d_payoff, d_tw, d_ghat, d_ints2,
-                    d_Qg, d_alpha, d_exc, d_count, d_chol};
+                    d_Qg, d_alpha, d_exc, d_count, d_chol, d_sobol};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
@@ -265,11 +265,19 @@ int Plan::init(const Request& r, hipStream_t st, bool quadrature, std::string& e
         err = "the Cholesky scheme runs on the FP64 path";
         return ST_ERR_ARG;
     }
+    if (r.gauss == 1 && (r.turbo || quadrature)) {
+        err = "Sobol' points are not combined with caller Gaussians or the turbo-charged estimator";
+        return ST_ERR_ARG;
+    }
+    if (r.gauss == 1 && r.M > 4294967294L) {
+        err = "Sobol' points: M above 2^32 - 2";
+        return ST_ERR_LIMIT;
+    }
     if (r.scheme == SCHEME_CHOLESKY && r.N > 2048) {
         err = "the Cholesky scheme covers N <= 2048";
         return ST_ERR_LIMIT;
     }
-    if (r.precision == PREC_FP32 || (split && (r.N > 1024 || quadrature))) {
+    if (r.precision == PREC_FP32 || (split && (r.N > 1024 || quadrature || r.gauss == 1))) {
         // staged pipeline with the FP32 FFT convolution (long paths, user Gaussians)
         conv_fft = true;
     } else if (split) {
@@ -329,6 +337,11 @@ int Plan::init(const Request& r, hipStream_t st, bool quadrature, std::string& e
     if (conv_fft) {
         rc = init_fft(err);
         if (rc) return rc;
+    }
+    if (r.gauss == 1) {
+        std::vector<uint32_t> V;
+        sobol_directions(nstreams * N, V);
+        RB_TRY(upload(&d_sobol, V, stream));
     }
     if (r.scheme == SCHEME_CHOLESKY) {
         // host factor per unit (distinct H), then one upload
@@ -424,7 +437,10 @@ int Plan::enqueue(std::string& err) {
     }
     for (long p0 = 0; p0 < req.M; p0 += chunk) {
         const long np = std::min(chunk, req.M - p0);
-        RB_LAUNCH(K_NORMALS, launch_normals_f64(key, p0, np, nstreams, N, d_Z, stream));
+        if (req.gauss == 1)
+            RB_LAUNCH(K_NORMALS, launch_normals_sobol_f64(d_sobol, p0, np, nstreams, N, d_Z, stream));
+        else
+            RB_LAUNCH(K_NORMALS, launch_normals_f64(key, p0, np, nstreams, N, d_Z, stream));
         RB_LAUNCH(K_VOLTERRA, conv(np, long(nstreams) * N));
         RB_LAUNCH(K_SCAN, launch_scan_f64(req.method, d_Z, long(nstreams) * N, d_Wt, np, N, d_groups,
                                           ngroups, d_comp, d_ints, stream));

@@ -43,6 +43,7 @@ struct Request {
     // turbo-charged RT estimator (control variate + antithetics, ML_console_ST/RBergomi.cpp:219-288):
     // m_control pilot paths estimate Q and alpha; q_max replaces Q where Q <= Iv
     int scheme = 0;              // 0 hybrid scheme, 1 exact Cholesky sampling of Wtilde
+    int gauss = 0;               // Gaussian source: 0 Philox (seeded), 1 Sobol' points (QMC)
     bool turbo = false;
     long m_control = 10000;
     double q_max = 10000.0;
@@ -66,6 +67,10 @@ enum Scheme { SCHEME_HYBRID = 0, SCHEME_CHOLESKY = 1 };
 double hyp2f1(double a, double b, double c, double z);
 void chol_covariance(double H, int n, double* A21, double* A22);
 int chol_factor(double H, int n, std::vector<double>& L, std::string& err);
+
+// Sobol' direction integers V[d*32 + k] of dimensions d < dims (rb_sobol.cpp)
+void sobol_directions(int dims, std::vector<uint32_t>& V);
+std::vector<uint64_t> primitive_polynomials(int count);
 
 // Philox path counters of the turbo-charged estimator's pilot paths (disjoint from main paths)
 constexpr long TURBO_PILOT_BASE = 1L << 62;
@@ -111,6 +116,7 @@ struct Plan {
     double* d_ints2 = nullptr;    // [groups][chunk][2] integrals of the antithetic paths
     double* d_Qg = nullptr;       // [groups] Q = max over the pilot paths of 2 Iv
     double* d_chol = nullptr;     // [units][N][2N] Cholesky factors (exact scheme)
+    uint32_t* d_sobol = nullptr;  // [streams*N][32] Sobol' direction integers (QMC)
     double* d_alpha = nullptr;    // [options] control-variate coefficients
     int* d_exc = nullptr;         // [options][nblk] Q <= Iv evaluations per path block
     long* d_count = nullptr;      // [options] their totals

@@ -334,6 +334,48 @@ hipError_t launch_normals_f64(PhiloxKey key, long p0, long np, int nstreams, int
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------- Sobol' Gaussians
+// Point i = p0 + p + 1 of the Sobol' sequence (dimension d: XOR of the direction integers V[d][k]
+// over the set bits k of i, over 2^32), mapped by the inverse normal CDF.  The point's dimensions
+// are dealt to the streams in turn, dimension nstreams*j + s -> stream s, step j (for two streams:
+// even dimensions to W1 and odd ones to W1perp, as breakZ does, rBergomi/aux.h:186-191).  One
+// thread per (path, dimension).
+__global__ void k_normals_sobol_f64(const uint32_t* __restrict__ V, long p0, long np, int nstreams,
+                                    int N, double* __restrict__ out) {
+    const long dims = long(nstreams) * N;
+    const long total = np * dims;
+    for (long t = blockIdx.x * long(blockDim.x) + threadIdx.x; t < total; t += long(gridDim.x) * blockDim.x) {
+        const long p = t / dims;
+        const int d = int(t - p * dims);
+        const uint64_t i = uint64_t(p0 + p + 1);
+        const uint32_t* v = V + size_t(d) * 32;
+        uint32_t x = 0;
+        for (int k = 0; k < 32; ++k)
+            if (i >> k & 1) x ^= v[k];
+        const double u = double(x) * 2.3283064365386963e-10;
+        const int s = d % nstreams, j = d / nstreams;
+        out[(p * nstreams + s) * N + j] = norm_inv_f64(u);
+    }
+}
+
+hipError_t launch_normals_sobol_f64(const uint32_t* V, long p0, long np, int nstreams, int N,
+                                    double* out, hipStream_t st) {
+    const long total = np * long(nstreams) * N;
+    hipLaunchKernelGGL(k_normals_sobol_f64, dim3(grid_for(total, 256)), dim3(256), 0, st, V, p0, np,
+                       nstreams, N, out);
+    return hipGetLastError();
+}
+
+__global__ void k_norm_inv_probe(const double* __restrict__ u, long n, double* __restrict__ z) {
+    const long i = blockIdx.x * long(blockDim.x) + threadIdx.x;
+    if (i < n) z[i] = norm_inv_f64(u[i]);
+}
+
+hipError_t launch_norm_inv_probe(const double* u, long n, double* z, hipStream_t st) {
+    hipLaunchKernelGGL(k_norm_inv_probe, dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, u, n, z);
+    return hipGetLastError();
+}
+
 size_t volterra_f64_lds_bytes(int N) {
     const int nb = (N + 3) / 4;
     const int tpp = nb < 256 ? nb : 256;
