@@ -95,9 +95,10 @@ def serving_requests(n, seed=5):
     return out
 
 
-def run_serving(rb, reqs, prec, rate_fraction=0.7, max_inflight=2, max_batch=64):
+def run_serving(rb, reqs, prec, rate_fraction=0.7, max_inflight=2, max_batch=64, cap=0.0, rate=None):
     """Closed loop (all submitted at once) gives the capacity; then Poisson arrivals at
-    rate_fraction x capacity give the latency percentiles."""
+    rate_fraction x capacity (or at `rate` requests/s) give the latency percentiles.  cap: the
+    server's work cap per batch (sum of M*N, 0 = none)."""
     import threading
     from rbergomi_amd.serving import Request, Server
 
@@ -105,7 +106,8 @@ def run_serving(rb, reqs, prec, rate_fraction=0.7, max_inflight=2, max_batch=64)
         return Request(d["xi"], d["H"], d["eta"], d["rho"], d["T"], d["K"], d["N"], d["M"], d["seed"])
 
     work = sum(d["M"] * d["N"] for d in reqs)
-    with Server(prec, max_inflight=max_inflight, max_batch_requests=max_batch) as srv:
+    with Server(prec, max_inflight=max_inflight, max_batch_requests=max_batch,
+                max_batch_path_steps=cap) as srv:
         rs = [mk(d) for d in reqs]
         t0 = time.perf_counter()
         for r in rs:
@@ -114,10 +116,12 @@ def run_serving(rb, reqs, prec, rate_fraction=0.7, max_inflight=2, max_batch=64)
             srv.wait(r)
         closed = time.perf_counter() - t0
     capacity = len(reqs) / closed
-    rate = rate_fraction * capacity
+    if rate is None:
+        rate = rate_fraction * capacity
     rng = np.random.default_rng(11)
     gaps = rng.exponential(1.0 / rate, len(reqs))
-    with Server(prec, max_inflight=max_inflight, max_batch_requests=max_batch) as srv:
+    with Server(prec, max_inflight=max_inflight, max_batch_requests=max_batch,
+                max_batch_path_steps=cap) as srv:
         rs = [mk(d) for d in reqs]
         done = []
 

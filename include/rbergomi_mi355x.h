@@ -217,6 +217,9 @@ typedef struct rb_server_config {
     int max_inflight;         /* batches on the device at once (>= 1) */
     int max_batch_requests;   /* requests per batch (>= 1) */
     long max_device_bytes;    /* admission budget for the batches in flight */
+    double max_batch_path_steps;  /* work cap of one batch, sum of M*N (0: none); a request above
+                                   * it runs as a batch of its own.  Short batches bound how long a
+                                   * small request waits behind large ones. */
 } rb_server_config;
 int rb_server_create(const rb_server_config* cfg, rb_server** out);
 /* Copies the inputs; the output arrays must stay valid until rb_server_wait returns for the

@@ -164,11 +164,15 @@ bool rb_server::launch(std::unique_lock<std::mutex>& lk) {
     // admit from the front of the queue while the device budget allows
     std::vector<Job*> take;
     size_t bytes = 0;
+    double work = 0.0;
     while (!pending.empty() && int(take.size()) < cfg.max_batch_requests) {
         Job* j = pending.front();
         if (inflight_bytes + bytes + j->footprint > size_t(cfg.max_device_bytes)) break;
+        const double w = double(j->req.M) * j->req.N;
+        if (!take.empty() && cfg.max_batch_path_steps > 0.0 && work + w > cfg.max_batch_path_steps) break;
         take.push_back(j);
         bytes += j->footprint;
+        work += w;
         pending.pop_front();
     }
     if (take.empty()) return false;

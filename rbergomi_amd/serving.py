@@ -20,7 +20,8 @@ class RbRequest(ctypes.Structure):
 
 class RbServerConfig(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("max_inflight", ctypes.c_int),
-                ("max_batch_requests", ctypes.c_int), ("max_device_bytes", ctypes.c_long)]
+                ("max_batch_requests", ctypes.c_int), ("max_device_bytes", ctypes.c_long),
+                ("max_batch_path_steps", ctypes.c_double)]
 
 
 def _declare(L):
@@ -70,11 +71,12 @@ class Server:
     """Continuous-batching server (rb_server_*)."""
 
     def __init__(self, precision=_lib.PREC_SPLIT_FP16, max_inflight=2, max_batch_requests=64,
-                 max_device_bytes=8 << 30):
+                 max_device_bytes=8 << 30, max_batch_path_steps=0.0):
         L = _lib.load()
         _declare(L)
         self.ptr = ctypes.c_void_p()
-        cfg = RbServerConfig(precision, max_inflight, max_batch_requests, max_device_bytes)
+        cfg = RbServerConfig(precision, max_inflight, max_batch_requests, max_device_bytes,
+                             float(max_batch_path_steps))
         check(L.rb_server_create(ctypes.byref(cfg), ctypes.byref(self.ptr)))
         self.live = {}
 
