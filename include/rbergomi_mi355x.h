@@ -138,11 +138,17 @@ void rb_plan_destroy(rb_plan* plan);
  * as ComputePriceRTMT_sobol / ComputeIVRTMT_sobol (rBergomi/rBergomiMT.cpp:610-716) with
  * normalQMC_sample (rBergomi/qmc.cpp:26-41).  The direction numbers are this library's own
  * (primitive polynomials by degree, seeded odd initial numbers; the reference's table is not
- * reproduced), so the points differ from the reference's beyond the first dimension.  No seed;
- * M <= 2^32 - 2.  cfg as for rb_price_v2 (split precisions take the FP32-FFT staged path). */
+ * reproduced), so the points differ from the reference's beyond the first dimension.  With
+ * seed_size = 0 this is the reference's construction (points 1..M, unscrambled; such an
+ * incomplete net carries a bias of the order of the Monte Carlo error at moderate M).  With a
+ * seed, a random digital shift keyed by the seed is XORed into every dimension and points 0..M-1
+ * are used at cell midpoints: an unbiased randomised QMC estimate whose error is estimated from
+ * independent seeds.  M <= 2^32 - 2.  cfg as for rb_price_v2 (split precisions take the
+ * FP32-FFT staged path). */
 int rb_price_sobol(double xi, const double* H, const double* eta, const double* rho, int par_size,
                    const double* T, const double* K, int op_size, int N, long M,
-                   const rb_config* cfg, double* price, double* iv, double* stat, int out_size);
+                   const uint64_t* seed, int seed_size, const rb_config* cfg, double* price,
+                   double* iv, double* stat, int out_size);
 
 /* ---- unordered rows ----
  * Row i prices the call (T[i], K[i]) under (xi[i], H[i], eta[i], rho[i]); every row uses the same

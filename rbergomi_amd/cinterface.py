@@ -115,8 +115,10 @@ def payoff_samples_v2(xi, H, eta, rho, T, K, N, Z, method=_lib.METHOD_RT,
 
 
 def price_sobol(xi, H, eta, rho, T, K, N, M, precision=_lib.PREC_FP64, compensator=_lib.COMP_REF,
-                method=_lib.METHOD_RT, scheme=0):
-    """Quasi-Monte Carlo pricing on Sobol' points (rb_price_sobol) -> (price, iv, stat)."""
+                method=_lib.METHOD_RT, scheme=0, seed=None):
+    """Quasi-Monte Carlo pricing on Sobol' points (rb_price_sobol) -> (price, iv, stat).  seed:
+    None for the reference's unscrambled points 1..M, else a random digital shift keyed by it."""
+    s, ps = _seed(seed if seed is not None else [])
     H, pH = _arr(H)
     eta, pe = _arr(eta)
     rho, pr = _arr(rho)
@@ -126,7 +128,7 @@ def price_sobol(xi, H, eta, rho, T, K, N, M, precision=_lib.PREC_FP64, compensat
     price, iv, stat = np.zeros(out), np.zeros(out), np.zeros(out)
     cfg = _cfg(precision, compensator, method, scheme)
     check(_lib.load().rb_price_sobol(float(xi), pH, pe, pr, len(H), pT, pK, len(T), int(N), int(M),
-                                     ctypes.byref(cfg), price.ctypes.data_as(dp), iv.ctypes.data_as(dp),
+                                     ps, len(s), ctypes.byref(cfg), price.ctypes.data_as(dp), iv.ctypes.data_as(dp),
                                      stat.ctypes.data_as(dp), out))
     return price, iv, stat
 

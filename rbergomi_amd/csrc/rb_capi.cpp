@@ -211,12 +211,13 @@ int rb_price_v2(double xi, const double* H, const double* eta, const double* rho
 
 int rb_price_sobol(double xi, const double* H, const double* eta, const double* rho, int par_size,
                    const double* T, const double* K, int op_size, int N, long M,
-                   const rb_config* cfg, double* price, double* iv, double* stat, int out_size) {
+                   const uint64_t* seed, int seed_size, const rb_config* cfg, double* price,
+                   double* iv, double* stat, int out_size) {
     set_last_error("");
     if (out_size != par_size * op_size) return fail(RB_ERR_SIZE, "out_size != par_size*op_size");
     if (par_size < 1 || op_size < 1) return fail(RB_ERR_ARG, "par_size and op_size must be positive");
     if (!H || !eta || !rho || !T || !K || !price || !stat) return fail(RB_ERR_ARG, "null array");
-    Request r = make_request(xi, H, eta, rho, par_size, T, K, op_size, N, M, nullptr, 0, cfg);
+    Request r = make_request(xi, H, eta, rho, par_size, T, K, op_size, N, M, seed, seed_size, cfg);
     r.gauss = 1;
     std::string err;
     Plan p;

@@ -17,7 +17,7 @@ struct PhiloxKey {
 
 // Philox4x32 with 10 rounds (Salmon, Moraes, Dror, Shaw, SC'11): multipliers 0xD2511F53,
 // 0xCD9E8D57, Weyl key increments 0x9E3779B9, 0xBB67AE85.
-__device__ __forceinline__ uint4 philox10(uint4 c, PhiloxKey key) {
+__host__ __device__ __forceinline__ uint4 philox10(uint4 c, PhiloxKey key) {
     uint32_t k0 = key.k0, k1 = key.k1;
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
@@ -35,7 +35,7 @@ __device__ __forceinline__ uint4 philox10(uint4 c, PhiloxKey key) {
     return c;
 }
 
-__device__ __forceinline__ uint4 philox_block(PhiloxKey key, uint64_t path, uint32_t stream,
+__host__ __device__ __forceinline__ uint4 philox_block(PhiloxKey key, uint64_t path, uint32_t stream,
                                               uint32_t q) {
     return philox10(make_uint4(q, uint32_t(path), uint32_t(path >> 32), stream), key);
 }

@@ -223,7 +223,7 @@ Plan::~Plan() {
     if (stream) (void)hipStreamSynchronize(stream);
     void* bufs[] = {d_units, d_gamma, d_groups, d_comp, d_opts,  d_Z,     d_Wt,   d_ints,
                     d_partial, d_price, d_stat, d_iv, d_payoff, d_tw, d_ghat, d_ints2,
-                    d_Qg, d_alpha, d_exc, d_count, d_chol, d_sobol};
+                    d_Qg, d_alpha, d_exc, d_count, d_chol, d_sobol, d_shift};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
@@ -342,6 +342,13 @@ int Plan::init(const Request& r, hipStream_t st, bool quadrature, std::string& e
         std::vector<uint32_t> V;
         sobol_directions(nstreams * N, V);
         RB_TRY(upload(&d_sobol, V, stream));
+        if (!r.seed.empty()) {
+            // digital shift: Philox block (0, dimension, 0, stream 7) of the seed's key
+            const PhiloxKey key = philox_key(r.seed.data(), int(r.seed.size()));
+            std::vector<uint32_t> sh(size_t(nstreams) * N);
+            for (size_t d = 0; d < sh.size(); ++d) sh[d] = philox_block(key, uint64_t(d), 7u, 0u).x;
+            RB_TRY(upload(&d_shift, sh, stream));
+        }
     }
     if (r.scheme == SCHEME_CHOLESKY) {
         // host factor per unit (distinct H), then one upload
@@ -438,7 +445,7 @@ int Plan::enqueue(std::string& err) {
     for (long p0 = 0; p0 < req.M; p0 += chunk) {
         const long np = std::min(chunk, req.M - p0);
         if (req.gauss == 1)
-            RB_LAUNCH(K_NORMALS, launch_normals_sobol_f64(d_sobol, p0, np, nstreams, N, d_Z, stream));
+            RB_LAUNCH(K_NORMALS, launch_normals_sobol_f64(d_sobol, d_shift, p0, np, nstreams, N, d_Z, stream));
         else
             RB_LAUNCH(K_NORMALS, launch_normals_f64(key, p0, np, nstreams, N, d_Z, stream));
         RB_LAUNCH(K_VOLTERRA, conv(np, long(nstreams) * N));

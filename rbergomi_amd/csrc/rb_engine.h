@@ -117,6 +117,7 @@ struct Plan {
     double* d_Qg = nullptr;       // [groups] Q = max over the pilot paths of 2 Iv
     double* d_chol = nullptr;     // [units][N][2N] Cholesky factors (exact scheme)
     uint32_t* d_sobol = nullptr;  // [streams*N][32] Sobol' direction integers (QMC)
+    uint32_t* d_shift = nullptr;  // [streams*N] digital shift (seeded QMC), or null
     double* d_alpha = nullptr;    // [options] control-variate coefficients
     int* d_exc = nullptr;         // [options][nblk] Q <= Iv evaluations per path block
     long* d_count = nullptr;      // [options] their totals

@@ -48,8 +48,8 @@ hipError_t launch_volterra_fft(const double* Z, long zstride, long np, int N, in
                                hipStream_t st);
 
 // Sobol' Gaussians (quasi-Monte Carlo), same [path][stream][N] layout as the Philox ones
-hipError_t launch_normals_sobol_f64(const uint32_t* V, long p0, long np, int nstreams, int N,
-                                    double* out, hipStream_t st);
+hipError_t launch_normals_sobol_f64(const uint32_t* V, const uint32_t* shift, long p0, long np,
+                                    int nstreams, int N, double* out, hipStream_t st);
 hipError_t launch_norm_inv_probe(const double* u, long n, double* z, hipStream_t st);
 // exact scheme: Wtilde = [L21 | L22] (x1, x2) per unit
 hipError_t launch_volterra_chol(const double* Z, long zstride, long np, int n, int nunits,

@@ -340,29 +340,32 @@ hipError_t launch_normals_f64(PhiloxKey key, long p0, long np, int nstreams, int
 // are dealt to the streams in turn, dimension nstreams*j + s -> stream s, step j (for two streams:
 // even dimensions to W1 and odd ones to W1perp, as breakZ does, rBergomi/aux.h:186-191).  One
 // thread per (path, dimension).
-__global__ void k_normals_sobol_f64(const uint32_t* __restrict__ V, long p0, long np, int nstreams,
-                                    int N, double* __restrict__ out) {
+// With a digital shift (shift != nullptr, one 32-bit word per dimension XORed into every point,
+// which keeps the net structure and makes each point uniform), point i = p0 + p is used from 0,
+// at the cell midpoint u = (x + 1/2) / 2^32 (never 0 or 1).
+__global__ void k_normals_sobol_f64(const uint32_t* __restrict__ V, const uint32_t* __restrict__ shift,
+                                    long p0, long np, int nstreams, int N, double* __restrict__ out) {
     const long dims = long(nstreams) * N;
     const long total = np * dims;
     for (long t = blockIdx.x * long(blockDim.x) + threadIdx.x; t < total; t += long(gridDim.x) * blockDim.x) {
         const long p = t / dims;
         const int d = int(t - p * dims);
-        const uint64_t i = uint64_t(p0 + p + 1);
+        const uint64_t i = uint64_t(p0 + p + (shift ? 0 : 1));
         const uint32_t* v = V + size_t(d) * 32;
-        uint32_t x = 0;
+        uint32_t x = shift ? shift[d] : 0u;
         for (int k = 0; k < 32; ++k)
             if (i >> k & 1) x ^= v[k];
-        const double u = double(x) * 2.3283064365386963e-10;
+        const double u = (double(x) + (shift ? 0.5 : 0.0)) * 2.3283064365386963e-10;
         const int s = d % nstreams, j = d / nstreams;
         out[(p * nstreams + s) * N + j] = norm_inv_f64(u);
     }
 }
 
-hipError_t launch_normals_sobol_f64(const uint32_t* V, long p0, long np, int nstreams, int N,
-                                    double* out, hipStream_t st) {
+hipError_t launch_normals_sobol_f64(const uint32_t* V, const uint32_t* shift, long p0, long np,
+                                    int nstreams, int N, double* out, hipStream_t st) {
     const long total = np * long(nstreams) * N;
-    hipLaunchKernelGGL(k_normals_sobol_f64, dim3(grid_for(total, 256)), dim3(256), 0, st, V, p0, np,
-                       nstreams, N, out);
+    hipLaunchKernelGGL(k_normals_sobol_f64, dim3(grid_for(total, 256)), dim3(256), 0, st, V, shift, p0,
+                       np, nstreams, N, out);
     return hipGetLastError();
 }
 

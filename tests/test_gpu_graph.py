@@ -24,13 +24,15 @@ def rb():
     return rbergomi_amd
 
 
-# (label, precision, N, M, method)
+# (label, precision, N, M, method, scheme)
 CASES = [
-    ("fused_fp16", 2, 256, 1 << 15, 0),
-    ("fused_bf16", 3, 128, 20000, 0),
-    ("staged_fp64", 0, 100, 1 << 14, 0),
-    ("staged_fp64_mc", 0, 64, 1 << 14, 1),
-    ("fft_fp32", 1, 1100, 4096, 0),
+    ("fused_fp16", 2, 256, 1 << 15, 0, 0),
+    ("fused_bf16", 3, 128, 20000, 0, 0),
+    ("fused_fp16_mc", 2, 96, 20000, 1, 0),
+    ("staged_fp64", 0, 100, 1 << 14, 0, 0),
+    ("staged_fp64_mc", 0, 64, 1 << 14, 1, 0),
+    ("fft_fp32", 1, 1100, 4096, 0, 0),
+    ("cholesky_fp64", 0, 80, 8192, 0, 1),
 ]
 
 
@@ -38,18 +40,20 @@ CASES = [
 def test_graph_replays_match_eager(rb, case):
     import ctypes
     from rbergomi_amd import cinterface as ci
-    label, prec, N, M, method = case
+    label, prec, N, M, method, scheme = case
     H, eta, rho = [0.07, 0.1], [1.9, 1.5], [-0.9, -0.7]
     T = [0.25, 1.0, 1.0, 2.0]
     K = [0.9, 1.0, 1.2, 0.8]
     say("graph case", label, "N", N, "M", M)
-    eager = ci.Plan(0.04, H, eta, rho, T, K, N, M, [11, 5], precision=prec, method=method)
+    eager = ci.Plan(0.04, H, eta, rho, T, K, N, M, [11, 5], precision=prec, method=method,
+                    scheme=scheme)
     eager.run()
     ref = eager.results()
     eager.close()
     assert np.all(np.isfinite(ref[0]))
 
-    g = ci.Plan(0.04, H, eta, rho, T, K, N, M, [11, 5], precision=prec, method=method, graph=True)
+    g = ci.Plan(0.04, H, eta, rho, T, K, N, M, [11, 5], precision=prec, method=method, graph=True,
+                scheme=scheme)
     for rep in range(4):
         g.run()
         got = g.results()
