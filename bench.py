@@ -95,7 +95,13 @@ def serving_requests(n, seed=5):
     return out
 
 
-def run_serving(rb, reqs, prec, rate_fraction=0.7, max_inflight=2, max_batch=64, cap=0.0, rate=None):
+# server settings of the latency measurements: four batches in flight, each capped at 2.5e8
+# paths*steps (about 1.7 ms of work), chosen by a sweep at fixed offered load
+SERVE_INFLIGHT, SERVE_MAX_BATCH, SERVE_CAP = 4, 64, 2.5e8
+
+
+def run_serving(rb, reqs, prec, rate_fraction=0.7, max_inflight=SERVE_INFLIGHT,
+                max_batch=SERVE_MAX_BATCH, cap=SERVE_CAP, rate=None):
     """Closed loop (all submitted at once) gives the capacity; then Poisson arrivals at
     rate_fraction x capacity (or at `rate` requests/s) give the latency percentiles.  cap: the
     server's work cap per batch (sum of M*N, 0 = none)."""
@@ -151,7 +157,9 @@ def run_serving(rb, reqs, prec, rate_fraction=0.7, max_inflight=2, max_batch=64,
             "capacity_req_per_s": capacity, "offered_req_per_s": rate,
             "open_loop_s": open_s, "open_loop_paths_steps_per_s": work / open_s,
             "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-            "server_p50_ms": st["p50_ms"], "server_p99_ms": st["p99_ms"], "batches": st["batches"]}
+            "server_p50_ms": st["p50_ms"], "server_p99_ms": st["p99_ms"], "batches": st["batches"],
+            "server": {"max_inflight": max_inflight, "max_batch_requests": max_batch,
+                       "max_batch_path_steps": cap}}
 
 
 def bench_batched(rb, args):
