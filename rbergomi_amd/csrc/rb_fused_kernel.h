@@ -87,8 +87,11 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
         const FusedReq& A = REQ_IN_LDS ? s_req : Aloc;
         const long item = B.item_local[gi];
         const int NK = A.NK;
-        const int pbw = A.wave_pb[wave];
-        const int nuw = A.wave_nu[wave];
+        // with the per-lane record, the loop-control values are marked wave-uniform (scalar
+        // branches instead of exec masks in the integral loops): 13.3 vs 14.0 ms at N = 512;
+        // with the LDS record it measured slower (11.56 vs 11.27 ms at N = 128)
+        const int pbw = REQ_IN_LDS ? A.wave_pb[wave] : __builtin_amdgcn_readfirstlane(A.wave_pb[wave]);
+        const int nuw = REQ_IN_LDS ? A.wave_nu[wave] : __builtin_amdgcn_readfirstlane(A.wave_nu[wave]);
         double a1 = 0.0, a2 = 0.0;  // sums of option tid (register accumulation)
         if (A.acc_in_lds)
             for (int o = tid; o < 2 * A.out_size; o += 256) accopt[o] = 0.0;
@@ -180,7 +183,8 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
                 }
                 // ---- 3b. integrals of every group of the unit, FUSED_GREG groups per pass
                 for (int g0 = U.g_begin; g0 < U.g_end; g0 += GR) {
-                    const int gn = U.g_end - g0 < GR ? U.g_end - g0 : GR;
+                    const int gn_ = U.g_end - g0 < GR ? U.g_end - g0 : GR;
+                    const int gn = REQ_IN_LDS ? gn_ : __builtin_amdgcn_readfirstlane(gn_);
                     float c1[GR], al[GR], be[GR];
                     double sv[GR], ss[GR], sp[GR];
                     // FP32 partial sums: per 16-row block (Romano-Touzi), or over all row blocks of
