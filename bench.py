@@ -24,8 +24,9 @@ sys.path.insert(0, ROOT)
 
 MATURITIES = [0.1, 0.25, 0.5, 1.0, 2.0]
 LOG_MONEYNESS = np.linspace(-0.4, 0.2, 41)
-PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "fp16": 2500.0, "bf16": 2500.0}
-PREC_CODE = {"fp64": 0, "fp32": 1, "fp16": 2, "bf16": 3}
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "fp16": 2500.0, "bf16": 2500.0, "fp32gemm": 157.3}
+# fp32: FP32 FFT convolution (staged); fp32gemm: Toeplitz GEMM on FP32 MFMA (fused kernel)
+PREC_CODE = {"fp64": 0, "fp32": 1, "fp16": 2, "bf16": 3, "fp32gemm": 4}
 
 
 def workload(paths):
@@ -215,15 +216,16 @@ def bench_serving(rb, args):
 
 def bench_long(rb, args):
     """N in {1024, 2048, 4096}, H in {0.02, 0.05}, eta 2.3, rho -0.9, xi 0.03, T 1, M = 2^18,
-    31 strikes in [0.7, 1.3]: the FP64 path, the FP32 LDS-FFT convolution, and the split-FP16
-    Toeplitz GEMM where it applies (N <= 1024)."""
+    31 strikes in [0.7, 1.3]: the FP64 path, the FP32 LDS-FFT convolution, and the split-FP16 and
+    FP32 Toeplitz GEMMs where they apply (N <= 1024)."""
     K = np.linspace(0.7, 1.3, 31)
     rows = []
     for N in (1024, 2048, 4096):
         for H in (0.02, 0.05):
             row = {"N": N, "H": H}
-            for name, code in (("fp64", rb.PREC_FP64), ("fp16", rb.PREC_SPLIT_FP16), ("fp32fft", rb.PREC_FP32)):
-                if name == "fp16" and N > 1024:
+            for name, code in (("fp64", rb.PREC_FP64), ("fp16", rb.PREC_SPLIT_FP16),
+                               ("fp32gemm", rb.PREC_FP32_GEMM), ("fp32fft", rb.PREC_FP32)):
+                if name in ("fp16", "fp32gemm") and N > 1024:
                     continue
                 pl = rb.Plan(0.03, [H], [2.3], [-0.9], [1.0] * 31, K, N, 2 ** 18, [4, N], precision=code)
                 pl.run()
@@ -335,7 +337,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp64": "f64", "fp32": "f32", "fp16": "f16x2+f32", "bf16": "bf16x2+f32"}[args.precision],
+        "dtype": {"fp64": "f64", "fp32": "f32", "fp16": "f16x2+f32", "bf16": "bf16x2+f32",
+                  "fp32gemm": "f32"}[args.precision],
         "data": "synthetic (device Philox4x32-10 Gaussians)",
         "config": {"workload": "single parameter set at scale: H=0.1 eta=1.5 rho=-0.7 xi=0.04 "
                                "N=512 M=2^22, 5 maturities x 41 strikes"
@@ -358,7 +361,7 @@ def main():
                            "max_abs_stat_err_vs_fp64": float(np.max(np.abs(stat - s64))),
                            "max_price_stat_ratio": float(np.max(np.abs(price - p64) / s64)),
                            "reference": "staged FP64 path, same Philox counters"}
-    if not args.no_latency and args.precision in ("fp16", "bf16") and args.method == "rt":
+    if not args.no_latency and args.precision in ("fp16", "bf16", "fp32gemm") and args.method == "rt":
         # request latency under load: 256 serving requests, Poisson arrivals at 0.7 x capacity
         lat = run_serving(rb, serving_requests(256), PREC_CODE[args.precision])
         out["latency"] = {k: lat[k] for k in ("requests", "p50_ms", "p99_ms", "offered_req_per_s",

@@ -71,7 +71,8 @@ enum rb_precision {
     RB_PREC_FP64 = 0,        /* every stage in FP64 (the parity path)               */
     RB_PREC_FP32 = 1,        /* Volterra convolution by FP32 FFT in LDS, the rest FP64 */
     RB_PREC_SPLIT_FP16 = 2,  /* Toeplitz GEMM as FP16 hi+lo products, FP32 accumulate */
-    RB_PREC_SPLIT_BF16 = 3   /* Toeplitz GEMM as BF16 hi+lo products, FP32 accumulate */
+    RB_PREC_SPLIT_BF16 = 3,  /* Toeplitz GEMM as BF16 hi+lo products, FP32 accumulate */
+    RB_PREC_FP32_GEMM = 4    /* Toeplitz GEMM on FP32 operands (v_mfma_f32_32x32x2_f32), N <= 1024 */
 };
 enum rb_compensator {
     RB_COMP_REF = 0,   /* 0.5 eta^2 ((i-1) dt)^(2H), rBergLib/rBergomiMT.cpp:77-79  */
@@ -143,8 +144,8 @@ void rb_plan_destroy(rb_plan* plan);
  * incomplete net carries a bias of the order of the Monte Carlo error at moderate M).  With a
  * seed, a random digital shift keyed by the seed is XORed into every dimension and points 0..M-1
  * are used at cell midpoints: an unbiased randomised QMC estimate whose error is estimated from
- * independent seeds.  M <= 2^32 - 2.  cfg as for rb_price_v2 (split precisions take the
- * FP32-FFT staged path). */
+ * independent seeds.  M <= 2^32 - 2.  cfg as for rb_price_v2 (the fused-kernel precisions
+ * take the FP32-FFT staged path). */
 int rb_price_sobol(double xi, const double* H, const double* eta, const double* rho, int par_size,
                    const double* T, const double* K, int op_size, int N, long M,
                    const uint64_t* seed, int seed_size, const rb_config* cfg, double* price,
@@ -207,8 +208,9 @@ typedef struct rb_request {
     double* stat;
 } rb_request;
 
-/* Prices n requests with one launch of the split-precision kernel (cfg->precision must be
- * RB_PREC_SPLIT_FP16 or RB_PREC_SPLIT_BF16, N <= 1024; cfg->method and cfg->compensator apply to
+/* Prices n requests with one launch of the fused kernel (cfg->precision must be
+ * RB_PREC_SPLIT_FP16, RB_PREC_SPLIT_BF16 or RB_PREC_FP32_GEMM, N <= 1024; cfg->method and
+ * cfg->compensator apply to
  * every request).  Every request's results are bit-identical to pricing it alone with
  * rb_price_v2 and the same cfg.  The server prices Romano-Touzi requests. */
 int rb_price_batch(const rb_request* reqs, int n, const rb_config* cfg);
@@ -219,7 +221,7 @@ int rb_price_batch(const rb_request* reqs, int n, const rb_config* cfg);
  * copies each request's results to its output arrays when its batch completes. */
 typedef struct rb_server rb_server;
 typedef struct rb_server_config {
-    int precision;            /* RB_PREC_SPLIT_FP16 or RB_PREC_SPLIT_BF16 */
+    int precision;            /* RB_PREC_SPLIT_FP16, RB_PREC_SPLIT_BF16 or RB_PREC_FP32_GEMM */
     int max_inflight;         /* batches on the device at once (>= 1) */
     int max_batch_requests;   /* requests per batch (>= 1) */
     long max_device_bytes;    /* admission budget for the batches in flight */
@@ -263,9 +265,10 @@ int rb_probe_normals(const uint64_t* seed, int seed_size, long p0, long np, int 
                      double* out);
 /* Volterra process of every path of Z (2 blocks of N per path): Wtilde[m*N + i] */
 int rb_probe_wtilde(const double* Z, long M, int N, double H, double* Wtilde);
-/* One 32x32x16 MFMA (bf16 = 0: FP16 operands, 1: BF16) of A[32][16] by B[16][32] built with
- * the operand layout of the fused kernel; out[lane*16 + r] = accumulator register r of lane. */
-int rb_probe_mfma_layout(const float* A, const float* B, int bf16, float* out);
+/* A[32][16] by B[16][32] on the matrix cores with the operand layout of the fused kernel
+ * (mode 0: one 32x32x16 MFMA on FP16 operands, 1: on BF16, 2: eight 32x32x2 FP32 MFMAs);
+ * out[lane*16 + r] = accumulator register r of lane. */
+int rb_probe_mfma_layout(const float* A, const float* B, int mode, float* out);
 /* Device-to-device copy of `bytes` (multiple of 16) by a plain streaming kernel, reps times;
  * *avg_ms receives the mean kernel time (HIP events).  Used to calibrate HBM traffic counters. */
 int rb_probe_copy(long bytes, int reps, double* avg_ms);

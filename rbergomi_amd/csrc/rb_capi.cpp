@@ -483,15 +483,19 @@ done:
     return rc;
 }
 
-int rb_probe_mfma_layout(const float* A, const float* B, int bf16, float* out) {
+int rb_probe_mfma_layout(const float* A, const float* B, int mode, float* out) {
     int rc = RB_OK;
+    if (mode < 0 || mode > 2) {
+        set_last_error("mode must be 0 (FP16), 1 (BF16) or 2 (FP32)");
+        return RB_ERR_ARG;
+    }
     float *dA = nullptr, *dB = nullptr, *dO = nullptr;
     PROBE_TRY(hipMalloc(&dA, 512 * sizeof(float)));
     PROBE_TRY(hipMalloc(&dB, 512 * sizeof(float)));
     PROBE_TRY(hipMalloc(&dO, 1024 * sizeof(float)));
     PROBE_TRY(hipMemcpy(dA, A, 512 * sizeof(float), hipMemcpyHostToDevice));
     PROBE_TRY(hipMemcpy(dB, B, 512 * sizeof(float), hipMemcpyHostToDevice));
-    PROBE_TRY(launch_probe_mfma_layout(dA, dB, dO, bf16, nullptr));
+    PROBE_TRY(launch_probe_mfma_layout(dA, dB, dO, mode, nullptr));
     PROBE_TRY(hipMemcpy(out, dO, 1024 * sizeof(float), hipMemcpyDeviceToHost));
 done:
     if (dA) (void)hipFree(dA);

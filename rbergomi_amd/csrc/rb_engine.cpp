@@ -256,7 +256,7 @@ int Plan::init(const Request& r, hipStream_t st, bool quadrature, std::string& e
         RB_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         own_stream = true;
     }
-    const bool split = r.precision == PREC_SPLIT_FP16 || r.precision == PREC_SPLIT_BF16;
+    const bool split = fused_mode_of(r.precision) >= 0;  // a fused-kernel precision
     if (r.turbo && (r.precision != PREC_FP64 || r.method != METHOD_RT || quadrature)) {
         err = "the turbo-charged estimator runs the Romano-Touzi FP64 path on device Gaussians";
         return ST_ERR_ARG;
@@ -286,7 +286,7 @@ int Plan::init(const Request& r, hipStream_t st, bool quadrature, std::string& e
             return ST_ERR_ARG;
         }
         fused = true;
-        fused_mode = r.precision == PREC_SPLIT_FP16 ? 0 : 1;
+        fused_mode = fused_mode_of(r.precision);
         return init_fused(err);
     }
     if (r.precision != PREC_FP64 && !conv_fft) {
@@ -384,6 +384,11 @@ void Plan::begin_timing() {
     ev_used = 0;
     times.assign(K_COUNT, KernelTime());
     for (int i = 0; i < K_COUNT; ++i) times[i].name = kernel_name(i);
+    // the convolution slot holds whichever kernel computes Wtilde
+    if (req.scheme == SCHEME_CHOLESKY)
+        times[K_VOLTERRA].name = "k_volterra_chol_f64";
+    else if (conv_fft)
+        times[K_VOLTERRA].name = "k_volterra_fft";
 }
 
 int Plan::timed(int id) {

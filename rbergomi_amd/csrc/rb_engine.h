@@ -14,7 +14,13 @@
 
 namespace rb {
 
-enum Precision { PREC_FP64 = 0, PREC_FP32 = 1, PREC_SPLIT_FP16 = 2, PREC_SPLIT_BF16 = 3 };
+enum Precision { PREC_FP64 = 0, PREC_FP32 = 1, PREC_SPLIT_FP16 = 2, PREC_SPLIT_BF16 = 3, PREC_FP32_GEMM = 4 };
+
+// operand mode of the fused kernel for a precision (0 FP16 hi+lo, 1 BF16 hi+lo, 2 FP32), -1 if
+// the precision does not run on the fused kernel
+inline int fused_mode_of(int precision) {
+    return precision == PREC_SPLIT_FP16 ? 0 : precision == PREC_SPLIT_BF16 ? 1 : precision == PREC_FP32_GEMM ? 2 : -1;
+}
 enum Compensator { COMP_REF = 0, COMP_EXACT = 1 };
 enum Status {
     ST_OK = 0,

@@ -1,7 +1,7 @@
 // rb_fused_host.cpp -- host planning and launching of the fused mixed-precision kernel.
 //
 // plan_fused_request builds, from one request's deduplicated layout: the per-H tables of
-// Toeplitz A fragments (hi + lo), the per-H tables P_j = (j - off)^(2H), the group exponent
+// Toeplitz A fragments (hi + lo, or FP32), the per-H tables P_j = (j - off)^(2H), the group exponent
 // coefficients, the payoff groups (group, rho), the option list and the assignment of 32-row
 // output blocks to the four waves of a workgroup (balanced triangular work).
 // FusedBatch concatenates any number of such requests into one launch: one table of requests,
@@ -166,14 +166,23 @@ int plan_fused_request(const Request& req, const Layout& lay, int mode, FusedHos
             for (int lane = 0; lane < 64; ++lane) {
                 const int m = lane & 31, hh = lane >> 5;
                 uint16_t hi[8], lo[8];
+                float g32[8];
                 for (int j = 0; j < 8; ++j) {
                     const int t = d + m - 8 * hh - j;
                     const double g = t < 0 ? 0.0 : (t == 0 ? diag : (t < N ? gam[t] : 0.0));
-                    split16(g, mode, hi[j], lo[j]);
+                    if (mode == 2)
+                        g32[j] = float(g);
+                    else
+                        split16(g, mode, hi[j], lo[j]);
                 }
                 uint4* e = &out.frag[(size_t(u) * a.NK + f) * 128];
-                std::memcpy(&e[lane], hi, 16);
-                std::memcpy(&e[64 + lane], lo, 16);
+                if (mode == 2) {  // FP32 operands: slots 0..3, then 4..7
+                    std::memcpy(&e[lane], g32, 16);
+                    std::memcpy(&e[64 + lane], g32 + 4, 16);
+                } else {
+                    std::memcpy(&e[lane], hi, 16);
+                    std::memcpy(&e[64 + lane], lo, 16);
+                }
             }
         }
     }

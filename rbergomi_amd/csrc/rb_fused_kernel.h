@@ -34,6 +34,33 @@ struct SplitT<1> {
     }
 };
 
+// FP32 operands: the matrix product runs on v_mfma_f32_32x32x2_f32 (K = 2 per instruction).  The
+// fragment tables and the LDS W1 tile keep the byte layout of the 16-bit modes: a lane's eight
+// k-slots of a 16-wide chunk, k = 8 (l/32) + j, are eight floats, j = 0..3 in the first 16-byte
+// half (where the 16-bit modes keep hi) and j = 4..7 in the second (lo).  Instruction j pairs the
+// A value of slot j with the B value of slot j, as the 16-bit modes pair their k-slots.
+template <>
+struct SplitT<2> {
+    typedef float T;
+    typedef float V __attribute__((ext_vector_type(4)));
+    typedef float V4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ floatx16_t mma1(float a, float b, floatx16_t c) {
+        return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
+    }
+};
+
+// W1 of path column `col`, step 16 ks + kk, from the LDS tile (the value the product used)
+template <int MODE>
+__device__ __forceinline__ float tile_w1(const uint4* chunk, int kk, int col) {
+    if constexpr (MODE == 2) {
+        return reinterpret_cast<const float*>(chunk + 64 * ((kk & 7) >> 2) + 32 * (kk >> 3) + col)[kk & 3];
+    } else {
+        typedef typename SplitT<MODE>::T T;
+        const uint4* Bp = chunk + 32 * (kk >> 3) + col;
+        return float(reinterpret_cast<const T*>(Bp)[kk & 7]) + float(reinterpret_cast<const T*>(Bp + 64)[kk & 7]);
+    }
+}
+
 __device__ __forceinline__ double half_sum_f64(double v) {
     // sum over the 32 lanes of one half-wave; every lane of the half receives the total
 #pragma unroll
@@ -48,7 +75,6 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
     constexpr int NS = MC ? 3 : 2;                        // sums per (path, group)
     constexpr bool REQ_IN_LDS = MC || UPW <= 2;           // where the request record lives
     typedef SplitT<MODE> S;
-    typedef typename S::T T;
     typedef typename S::V V;
     typedef typename S::V4 V4;
     extern __shared__ uint4 lds4[];
@@ -114,17 +140,22 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
                         for (int j = 0; j < 4; ++j)
                             if (4 * q + j >= A.N) z[j] = 0.f;
                     }
-                    V4 hi, lo;
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) {
-                        hi[j] = S::cvt(z[j]);
-                        lo[j] = S::cvt(z[j] - float(hi[j]));
-                    }
                     const int ks = q >> 2, hh = (q & 3) >> 1, j0 = (q & 1);
                     const int ln = 32 * hh + (pl & 31);
                     uint4* base = Bf + ((pb * NK + ks) * 2) * 64;
-                    reinterpret_cast<uint2*>(base + ln)[j0] = __builtin_bit_cast(uint2, hi);
-                    reinterpret_cast<uint2*>(base + 64 + ln)[j0] = __builtin_bit_cast(uint2, lo);
+                    if constexpr (MODE == 2) {
+                        const V4 v = {z[0], z[1], z[2], z[3]};
+                        base[64 * j0 + ln] = __builtin_bit_cast(uint4, v);
+                    } else {
+                        V4 hi, lo;
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            hi[j] = S::cvt(z[j]);
+                            lo[j] = S::cvt(z[j] - float(hi[j]));
+                        }
+                        reinterpret_cast<uint2*>(base + ln)[j0] = __builtin_bit_cast(uint2, hi);
+                        reinterpret_cast<uint2*>(base + 64 + ln)[j0] = __builtin_bit_cast(uint2, lo);
+                    }
                 }
             }
             __syncthreads();
@@ -157,12 +188,24 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
                                 bhi[s] = __builtin_bit_cast(V, Bp[lane]);
                                 blo[s] = __builtin_bit_cast(V, Bp[64 + lane]);
                             }
+                            if constexpr (MODE == 2) {
+                                // k-slots 0..3 from the first halves, 4..7 from the second
 #pragma unroll
-                            for (int s = 0; s < k; ++s) acc[s] = S::mma(ahi, bhi[s], acc[s]);
+                                for (int j = 0; j < 4; ++j)
 #pragma unroll
-                            for (int s = 0; s < k; ++s) acc[s] = S::mma(ahi, blo[s], acc[s]);
+                                    for (int s = 0; s < k; ++s) acc[s] = S::mma1(ahi[j], bhi[s][j], acc[s]);
 #pragma unroll
-                            for (int s = 0; s < k; ++s) acc[s] = S::mma(alo, bhi[s], acc[s]);
+                                for (int j = 0; j < 4; ++j)
+#pragma unroll
+                                    for (int s = 0; s < k; ++s) acc[s] = S::mma1(alo[j], blo[s][j], acc[s]);
+                            } else {
+#pragma unroll
+                                for (int s = 0; s < k; ++s) acc[s] = S::mma(ahi, bhi[s], acc[s]);
+#pragma unroll
+                                for (int s = 0; s < k; ++s) acc[s] = S::mma(ahi, blo[s], acc[s]);
+#pragma unroll
+                                for (int s = 0; s < k; ++s) acc[s] = S::mma(alo, bhi[s], acc[s]);
+                            }
                         }
                     }
                     // ---- 3a. Wtilde = sc (Y' + b W1perp), W1perp from the same counters (stream 1)
@@ -243,9 +286,7 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
                                 const int j = 32 * rb + mfma32_row(r, lane) + 1;  // v_j uses Wtilde_{j-1}
                                 if (j < A.N) {
                                     const int ks = j >> 4, kk = j & 15;
-                                    const uint4* Bp = Bf + ((pbw * NK + ks) * 2) * 64 + 32 * (kk >> 3) + col;
-                                    const float w1 = float(reinterpret_cast<const T*>(Bp)[kk & 7]) +
-                                                     float(reinterpret_cast<const T*>(Bp + 64)[kk & 7]);
+                                    const float w1 = tile_w1<MODE>(Bf + ((pbw * NK + ks) * 2) * 64, kk, col);
                                     const float pj = P[j];
                                     const float x = acc[s][r];
 #pragma unroll
@@ -306,9 +347,7 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
                             const int pl = e & (A.TP - 1), gg = e >> A.log2TP;
                             const int pb = pl >> 5, c = pl & 31;
                             const FusedGroup G = A.groups[g0 + c0 + gg];
-                            const uint4* B0 = Bf + ((pb * NK) * 2) * 64 + c;  // k = 0
-                            const double w10 = double(float(reinterpret_cast<const T*>(B0)[0]) +
-                                                      float(reinterpret_cast<const T*>(B0 + 64)[0]));
+                            const double w10 = double(tile_w1<MODE>(Bf + ((pb * NK) * 2) * 64, 0, c));  // k = 0
                             double a = G.xi, b = G.sxi * w10;   // j = 0: v_0 = xi
                             double cp = 0.0;
                             if constexpr (MC) {
@@ -444,6 +483,10 @@ static hipError_t launch_mu(int mode, int upw, const FusedBatchArgs& a, int grid
         case 18: return launch_t<1, 2, MC, GR>(a, grid, st);
         case 20: return launch_t<1, 4, MC, GR>(a, grid, st);
         case 24: return launch_t<1, 8, MC, GR>(a, grid, st);
+        case 33: return launch_t<2, 1, MC, GR>(a, grid, st);
+        case 34: return launch_t<2, 2, MC, GR>(a, grid, st);
+        case 36: return launch_t<2, 4, MC, GR>(a, grid, st);
+        case 40: return launch_t<2, 8, MC, GR>(a, grid, st);
         default: return hipErrorInvalidValue;
     }
 }

@@ -33,7 +33,7 @@ hipError_t launch_iv_f64(const double* price, const Option* opts, int n, double*
                          hipStream_t st);
 hipError_t launch_philox_probe(const void* ctr, PhiloxKey key, long n, void* out, hipStream_t st);
 // MFMA layout probe (rb_probe.hip)
-hipError_t launch_probe_mfma_layout(const float* A, const float* B, float* out, int bf16,
+hipError_t launch_probe_mfma_layout(const float* A, const float* B, float* out, int mode,
                                     hipStream_t st);
 
 hipError_t launch_probe_payoff(const double* lnS, const double* sig, const double* K, int n,

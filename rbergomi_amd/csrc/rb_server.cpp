@@ -68,14 +68,14 @@ int plan_job(const rb_request* q, int precision, int method, int comp, Job& j, s
     r.method = method;
     r.comp = comp;
     r.precision = precision;
-    if (precision != PREC_SPLIT_FP16 && precision != PREC_SPLIT_BF16) {
-        err = "batches run the split-precision kernel (RB_PREC_SPLIT_FP16 or RB_PREC_SPLIT_BF16)";
+    if (fused_mode_of(precision) < 0) {
+        err = "batches run the fused kernel (RB_PREC_SPLIT_FP16, RB_PREC_SPLIT_BF16 or RB_PREC_FP32_GEMM)";
         return ST_ERR_ARG;
     }
     int rc = validate(r, err);
     if (rc) return rc;
     j.lay = make_layout(r);
-    rc = plan_fused_request(r, j.lay, precision == PREC_SPLIT_FP16 ? 0 : 1, j.fh, err);
+    rc = plan_fused_request(r, j.lay, fused_mode_of(precision), j.fh, err);
     if (rc) return rc;
     j.price = q->price;
     j.iv = q->iv;
@@ -192,7 +192,7 @@ bool rb_server::launch(std::unique_lock<std::mutex>& lk) {
     std::vector<const FusedHostReq*> hr;
     for (Job* j : take) hr.push_back(&j->fh);
     F->batch.reset(new FusedBatch());
-    const int mode = cfg.precision == PREC_SPLIT_FP16 ? 0 : 1;
+    const int mode = fused_mode_of(cfg.precision);
     if (!rc) rc = F->batch->build(hr, mode, device, F->stream, err);
     if (!rc) rc = F->batch->launch(F->stream, err);
     const int total = rc ? 0 : F->batch->total_out;
@@ -280,7 +280,7 @@ int rb_price_batch(const rb_request* reqs, int n, const rb_config* cfg) {
     std::vector<const FusedHostReq*> hr;
     for (auto& j : jobs) hr.push_back(&j->fh);
     FusedBatch B;
-    int rc = B.build(hr, cfg->precision == PREC_SPLIT_FP16 ? 0 : 1, device, st, err);
+    int rc = B.build(hr, fused_mode_of(cfg->precision), device, st, err);
     if (!rc) rc = B.launch(st, err);
     for (int i = 0; !rc && i < n; ++i) rc = B.results(i, reqs[i].price, reqs[i].iv, reqs[i].stat, st, err);
     if (!rc && hipStreamSynchronize(st) != hipSuccess) {
@@ -299,8 +299,8 @@ int rb_server_create(const rb_server_config* cfg, rb_server** out) {
         set_last_error("null argument");
         return RB_ERR_ARG;
     }
-    if (cfg->precision != RB_PREC_SPLIT_FP16 && cfg->precision != RB_PREC_SPLIT_BF16) {
-        set_last_error("the server runs the split-precision kernel");
+    if (fused_mode_of(cfg->precision) < 0) {
+        set_last_error("the server runs the fused kernel (RB_PREC_SPLIT_FP16, RB_PREC_SPLIT_BF16 or RB_PREC_FP32_GEMM)");
         return RB_ERR_ARG;
     }
     std::unique_ptr<rb_server> s(new rb_server());

@@ -1,4 +1,4 @@
-"""GPU checks of the fused split-precision kernel.
+"""GPU checks of the fused mixed-precision kernel (FP16 / BF16 hi+lo split and FP32 operands).
 
 - the MFMA operand pairing and output-row layout it relies on (exact small-integer product);
 - prices against the FP64 path on identical Philox counters: <= 1e-4 absolute (the stated
@@ -25,14 +25,14 @@ def rb():
     return rbergomi_amd
 
 
-@pytest.mark.parametrize("bf16", [0, 1])
-def test_mfma_layout(rb, bf16):
-    rng = np.random.default_rng(3 + bf16)
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_mfma_layout(rb, mode):
+    rng = np.random.default_rng(3 + mode)
     A = rng.integers(-2, 3, (32, 16)).astype(np.float32)
     B = rng.integers(-4, 5, (16, 32)).astype(np.float32)
     out = np.zeros(1024, dtype=np.float32)
     fp = ctypes.POINTER(ctypes.c_float)
-    assert rb.load().rb_probe_mfma_layout(A.ctypes.data_as(fp), B.ctypes.data_as(fp), bf16,
+    assert rb.load().rb_probe_mfma_layout(A.ctypes.data_as(fp), B.ctypes.data_as(fp), mode,
                                           out.ctypes.data_as(fp)) == 0
     D = A.astype(np.float64) @ B.astype(np.float64)
     for lane in range(64):
@@ -61,7 +61,10 @@ CASES = [
 ]
 
 
-@pytest.mark.parametrize("prec", ["fp16", "bf16"])
+CODES = {"fp16": "PREC_SPLIT_FP16", "bf16": "PREC_SPLIT_BF16", "fp32": "PREC_FP32_GEMM"}
+
+
+@pytest.mark.parametrize("prec", ["fp16", "bf16", "fp32"])
 @pytest.mark.parametrize("case", range(len(CASES)))
 def test_fused_matches_fp64_on_same_counters(rb, case, prec):
     N, M, H, eta, rho, T, K = CASES[case]
@@ -71,7 +74,7 @@ def test_fused_matches_fp64_on_same_counters(rb, case, prec):
     seed = [11, case]
     say("fused %s case %d N=%d M=%d" % (prec, case, N, M))
     p64, iv64, s64 = rb.price_v2(xi, H, eta, rho, T, K, N, M, seed, precision=rb.PREC_FP64)
-    code = rb.PREC_SPLIT_FP16 if prec == "fp16" else rb.PREC_SPLIT_BF16
+    code = getattr(rb, CODES[prec])
     pm, ivm, sm = rb.price_v2(xi, H, eta, rho, T, K, N, M, seed, precision=code)
     err = np.max(np.abs(pm - p64))
     say("   max |price - fp64| = %.3e   max |stat - fp64| = %.3e" % (err, np.max(np.abs(sm - s64))))
@@ -82,7 +85,7 @@ def test_fused_matches_fp64_on_same_counters(rb, case, prec):
 MC_CASES = [0, 1, 2, 3, 4, 6, 7]
 
 
-@pytest.mark.parametrize("prec", ["fp16", "bf16"])
+@pytest.mark.parametrize("prec", ["fp16", "bf16", "fp32"])
 @pytest.mark.parametrize("case", MC_CASES)
 def test_fused_plain_mc_matches_fp64_on_same_counters(rb, case, prec):
     # computeIV_tensor's estimator: ln S_T from a third Gaussian stream, payoff (S_T - K)^+
@@ -94,7 +97,7 @@ def test_fused_plain_mc_matches_fp64_on_same_counters(rb, case, prec):
     say("fused MC %s case %d N=%d M=%d" % (prec, case, N, M))
     p64, iv64, s64 = rb.price_v2(xi, H, eta, rho, T, K, N, M, seed, precision=rb.PREC_FP64,
                                  method=rb.METHOD_MC)
-    code = rb.PREC_SPLIT_FP16 if prec == "fp16" else rb.PREC_SPLIT_BF16
+    code = getattr(rb, CODES[prec])
     pm, ivm, sm = rb.price_v2(xi, H, eta, rho, T, K, N, M, seed, precision=code, method=rb.METHOD_MC)
     err = np.max(np.abs(pm - p64))
     say("   max |price - fp64| = %.3e   max |stat - fp64| = %.3e" % (err, np.max(np.abs(sm - s64))))
@@ -111,7 +114,7 @@ def test_fused_plain_mc_many_maturities(rb, nt):
     say("fused MC %d maturities" % nt)
     args = (0.04, [0.1, 0.3], [1.5, 0.9], [-0.7, -0.2], T, K, 128, 3000, [5, nt])
     p64, _, s64 = rb.price_v2(*args, precision=rb.PREC_FP64, method=rb.METHOD_MC)
-    for code in (rb.PREC_SPLIT_FP16, rb.PREC_SPLIT_BF16):
+    for code in (rb.PREC_SPLIT_FP16, rb.PREC_SPLIT_BF16, rb.PREC_FP32_GEMM):
         pm, _, sm = rb.price_v2(*args, precision=code, method=rb.METHOD_MC)
         assert np.max(np.abs(pm - p64)) <= 1e-4
         assert np.max(np.abs(sm - s64)) <= 1e-4
@@ -131,12 +134,14 @@ def test_split_precision_routes_outside_the_fused_kernel(rb):
     T, K = [0.5, 1.0], [1.0, 1.1]
     for N, method in ((2048, rb.METHOD_RT), (2048, rb.METHOD_MC), (64, rb.METHOD_MC)):
         p64, _, s64 = rb.price_v2(0.04, [0.1], [1.5], [-0.7], T, K, N, 2000, [1], method=method)
-        p16, _, s16 = rb.price_v2(0.04, [0.1], [1.5], [-0.7], T, K, N, 2000, [1], method=method,
-                                  precision=rb.PREC_SPLIT_FP16)
-        assert np.max(np.abs(p16 - p64)) <= 1e-4
-    with pytest.raises(rb.RbError) as e:
-        rb.price_v2(0.04, [0.1], [1.5], [-0.7], [1.0], [1.0], 9000, 100, [1], precision=rb.PREC_SPLIT_FP16)
-    assert e.value.code == 5
+        for code in (rb.PREC_SPLIT_FP16, rb.PREC_FP32_GEMM):
+            pm, _, sm = rb.price_v2(0.04, [0.1], [1.5], [-0.7], T, K, N, 2000, [1], method=method,
+                                    precision=code)
+            assert np.max(np.abs(pm - p64)) <= 1e-4
+    for code in (rb.PREC_SPLIT_FP16, rb.PREC_FP32_GEMM):
+        with pytest.raises(rb.RbError) as e:
+            rb.price_v2(0.04, [0.1], [1.5], [-0.7], [1.0], [1.0], 9000, 100, [1], precision=code)
+        assert e.value.code == 5
 
 
 def test_mixed_payoff_and_mills_ratio(rb):

@@ -36,10 +36,10 @@ def alone(rb, r, prec):
     return rb.price_v2(r.xi, r.H, r.eta, r.rho, r.T, r.K, r.N, r.M, r.seed, precision=prec)
 
 
-@pytest.mark.parametrize("prec_name", ["fp16", "bf16"])
+@pytest.mark.parametrize("prec_name", ["fp16", "bf16", "fp32"])
 def test_batch_equals_alone(rb, prec_name):
     from rbergomi_amd.serving import price_batch
-    prec = rb.PREC_SPLIT_FP16 if prec_name == "fp16" else rb.PREC_SPLIT_BF16
+    prec = {"fp16": rb.PREC_SPLIT_FP16, "bf16": rb.PREC_SPLIT_BF16, "fp32": rb.PREC_FP32_GEMM}[prec_name]
     reqs = make_requests(12, seed=1)
     price_batch(reqs, precision=prec)
     for r in reqs:
