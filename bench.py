@@ -177,7 +177,7 @@ def bench_batched(rb, args):
                        precision=rb.PREC_FP64) for d in reqs]
     fp64_s = time.perf_counter() - t0
     out["fp64_single_request"] = {"s": fp64_s, "paths_steps_per_s": work / fp64_s}
-    for name, code in (("fp16", rb.PREC_SPLIT_FP16), ("bf16", rb.PREC_SPLIT_BF16)):
+    for name, code in (("fp16", rb.PREC_SPLIT_FP16), ("bf16", rb.PREC_SPLIT_BF16), ("fp32gemm", rb.PREC_FP32_GEMM)):
         rs = [Request(d["xi"], d["H"], d["eta"], d["rho"], d["T"], d["K"], d["N"], d["M"], d["seed"])
               for d in reqs]
         price_batch(rs, precision=code)  # warm-up (code objects, allocator)

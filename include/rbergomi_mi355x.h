@@ -267,7 +267,8 @@ int rb_probe_normals(const uint64_t* seed, int seed_size, long p0, long np, int 
 int rb_probe_wtilde(const double* Z, long M, int N, double H, double* Wtilde);
 /* A[32][16] by B[16][32] on the matrix cores with the operand layout of the fused kernel
  * (mode 0: one 32x32x16 MFMA on FP16 operands, 1: on BF16, 2: eight 32x32x2 FP32 MFMAs);
- * out[lane*16 + r] = accumulator register r of lane. */
+ * out[lane*16 + r] = accumulator register r of lane.  Mode 3: A[16][4] by B[4][16] (the first
+ * 64 entries of each) on one 16x16x4 FP64 MFMA, out[lane*4 + r], as the FP64 convolution uses. */
 int rb_probe_mfma_layout(const float* A, const float* B, int mode, float* out);
 /* Device-to-device copy of `bytes` (multiple of 16) by a plain streaming kernel, reps times;
  * *avg_ms receives the mean kernel time (HIP events).  Used to calibrate HBM traffic counters. */

@@ -485,8 +485,8 @@ done:
 
 int rb_probe_mfma_layout(const float* A, const float* B, int mode, float* out) {
     int rc = RB_OK;
-    if (mode < 0 || mode > 2) {
-        set_last_error("mode must be 0 (FP16), 1 (BF16) or 2 (FP32)");
+    if (mode < 0 || mode > 3) {
+        set_last_error("mode must be 0 (FP16), 1 (BF16), 2 (FP32) or 3 (FP64 16x16x4)");
         return RB_ERR_ARG;
     }
     float *dA = nullptr, *dB = nullptr, *dO = nullptr;
@@ -584,7 +584,7 @@ int rb_probe_wtilde(const double* Z, long M, int N, double H, double* Wtilde) {
     PROBE_TRY(hipMemcpy(dZ, Z, size_t(M) * 2 * N * sizeof(double), hipMemcpyHostToDevice));
     PROBE_TRY(hipMemcpy(dg, L.gamma.data(), size_t(N) * sizeof(double), hipMemcpyHostToDevice));
     PROBE_TRY(hipMemcpy(du, L.units.data(), sizeof(Unit), hipMemcpyHostToDevice));
-    PROBE_TRY(launch_volterra_f64(dZ, 2L * N, M, N, 1, du, dg, dW, nullptr));
+    PROBE_TRY(launch_volterra_conv_f64(dZ, 2L * N, M, N, 1, du, dg, dW, nullptr));
     PROBE_TRY(hipMemcpy(Wtilde, dW, size_t(M) * N * sizeof(double), hipMemcpyDeviceToHost));
 done:
     if (dZ) (void)hipFree(dZ);

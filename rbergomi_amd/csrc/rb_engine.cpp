@@ -245,7 +245,8 @@ int Plan::init(const Request& r, hipStream_t st, bool quadrature, std::string& e
     {
         hipDeviceProp_t prop;
         RB_TRY(hipGetDeviceProperties(&prop, device));
-        if (volterra_f64_lds_bytes(r.N) > prop.sharedMemPerBlock) {
+        const size_t conv_lds = r.N <= VOLTERRA_MFMA_MAX_N ? volterra_mfma_lds_bytes(r.N) : volterra_f64_lds_bytes(r.N);
+        if (conv_lds > prop.sharedMemPerBlock) {
             err = "N = " + std::to_string(r.N) + " needs more LDS than one workgroup can hold";
             return ST_ERR_LIMIT;
         }
@@ -389,6 +390,8 @@ void Plan::begin_timing() {
         times[K_VOLTERRA].name = "k_volterra_chol_f64";
     else if (conv_fft)
         times[K_VOLTERRA].name = "k_volterra_fft";
+    else if (req.N <= VOLTERRA_MFMA_MAX_N)
+        times[K_VOLTERRA].name = "k_volterra_mfma_f64";
 }
 
 int Plan::timed(int id) {
@@ -593,7 +596,8 @@ hipError_t Plan::conv(long np, long zstride) {
     if (conv_fft)
         return launch_volterra_fft(d_Z, zstride, np, req.N, log2L, int(lay.units.size()), d_units, d_ghat,
                                    d_tw, d_Wt, stream);
-    return launch_volterra_f64(d_Z, zstride, np, req.N, int(lay.units.size()), d_units, d_gamma, d_Wt, stream);
+    return launch_volterra_conv_f64(d_Z, zstride, np, req.N, int(lay.units.size()), d_units, d_gamma, d_Wt,
+                                    stream);
 }
 
 int Plan::turbo_results(double* alpha, double* Q, long* exceptions, std::string& err) {

@@ -18,6 +18,19 @@ size_t volterra_f64_lds_bytes(int N);
 hipError_t launch_volterra_f64(const double* Z, long zstride, long np, int N, int nunits,
                                const Unit* units, const double* gamma, double* Wt,
                                hipStream_t st);
+// the same convolution on the FP64 matrix cores, N <= 1024 (rb_volterra64.hip)
+constexpr int VOLTERRA_MFMA_MAX_N = 1024;
+size_t volterra_mfma_lds_bytes(int N);
+hipError_t launch_volterra_mfma_f64(const double* Z, long zstride, long np, int N, int nunits,
+                                    const Unit* units, const double* gamma, double* Wt, hipStream_t st);
+// FP64 convolution of the staged path: the matrix-core kernel for N <= VOLTERRA_MFMA_MAX_N,
+// the direct sum beyond
+inline hipError_t launch_volterra_conv_f64(const double* Z, long zstride, long np, int N, int nunits,
+                                           const Unit* units, const double* gamma, double* Wt,
+                                           hipStream_t st) {
+    return N <= VOLTERRA_MFMA_MAX_N ? launch_volterra_mfma_f64(Z, zstride, np, N, nunits, units, gamma, Wt, st)
+                                    : launch_volterra_f64(Z, zstride, np, N, nunits, units, gamma, Wt, st);
+}
 hipError_t launch_scan_f64(int method, const double* Z, long zstride, const double* Wt, long np,
                            int N, const Group* groups, int ngroups, const double* comp,
                            double* out, hipStream_t st, double sign = 1.0);

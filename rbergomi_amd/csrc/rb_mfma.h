@@ -15,6 +15,7 @@ namespace rb {
 typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float floatx16_t __attribute__((ext_vector_type(16)));
+typedef double doublex4_t __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ int mfma32_row(int r, int lane) {
     return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);

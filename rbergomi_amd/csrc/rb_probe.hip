@@ -12,10 +12,19 @@ namespace rb {
 // registers: out[lane*16 + r].  Small integer inputs make every product and sum exact.
 // mode 0: one v_mfma_f32_32x32x16_f16; 1: the bf16 form; 2: eight v_mfma_f32_32x32x2_f32, the
 // j-th on slot j of each half (as the FP32-operand fused kernel pairs them).
+// mode 3: one v_mfma_f64_16x16x4_f64 of A[16][4] (A[m*4 + k]) by B[4][16] (B[k*16 + n]) with lane
+// l supplying A[l%16][l/16] and B[l/16][l%16]; out[lane*4 + r] = accumulator register r.
 __global__ void k_probe_mfma_layout(const float* __restrict__ A, const float* __restrict__ B,
                                     float* __restrict__ out, int mode) {
     const int l = threadIdx.x;
     const int m = l & 31, h = l >> 5;
+    if (mode == 3) {
+        doublex4_t d = {0.0, 0.0, 0.0, 0.0};
+        d = __builtin_amdgcn_mfma_f64_16x16x4f64(double(A[(l & 15) * 4 + (l >> 4)]),
+                                                 double(B[(l >> 4) * 16 + (l & 15)]), d, 0, 0, 0);
+        for (int r = 0; r < 4; ++r) out[l * 4 + r] = float(d[r]);
+        return;
+    }
     floatx16_t acc;
     for (int r = 0; r < 16; ++r) acc[r] = 0.f;
     if (mode == 2) {

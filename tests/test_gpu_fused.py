@@ -41,6 +41,24 @@ def test_mfma_layout(rb, mode):
             assert out[lane * 16 + r] == D[row, lane % 32], (lane, r)
 
 
+def test_mfma_f64_layout(rb):
+    # v_mfma_f64_16x16x4_f64 as the FP64 convolution uses it: lane l supplies A[l%16][l/16] and
+    # B[l/16][l%16] and holds D[4 r + l/16][l%16]
+    rng = np.random.default_rng(7)
+    A = np.zeros(512, dtype=np.float32)
+    B = np.zeros(512, dtype=np.float32)
+    A[:64] = rng.integers(-3, 4, 64)
+    B[:64] = rng.integers(-3, 4, 64)
+    out = np.zeros(1024, dtype=np.float32)
+    fp = ctypes.POINTER(ctypes.c_float)
+    assert rb.load().rb_probe_mfma_layout(A.ctypes.data_as(fp), B.ctypes.data_as(fp), 3,
+                                          out.ctypes.data_as(fp)) == 0
+    D = A[:64].reshape(16, 4).astype(np.float64) @ B[:64].reshape(4, 16).astype(np.float64)
+    for lane in range(64):
+        for r in range(4):
+            assert out[lane * 4 + r] == D[4 * r + lane // 16, lane % 16], (lane, r)
+
+
 def opts_config2_like():
     mats = [0.1, 0.25, 0.5, 1.0, 2.0]
     ks = np.linspace(-0.4, 0.2, 41)
