@@ -163,6 +163,20 @@ def run_serving(rb, reqs, prec, rate_fraction=0.7, max_inflight=SERVE_INFLIGHT,
                        "max_batch_path_steps": cap}}
 
 
+def tune(rb, Ns, args, precisions):
+    """Tile-shape autotuning for the step counts of a workload (a 21-strike request shape at
+    M = 2^20); returns what was chosen and the time it took."""
+    from rbergomi_amd.serving import Request, autotune
+    shape = Request(0.04, [0.1], [1.5], [-0.7], [1.0] * 21, list(np.linspace(0.8, 1.2, 21)), 0, 1 << 20, [5, 5])
+    t0 = time.perf_counter()
+    chosen = {}
+    for name, code in precisions:
+        res = autotune(shape, sorted(set((n + 31) // 32 * 32 for n in Ns)), precision=code)
+        chosen[name] = {str(n): pb for n, (pb, _) in res.items()}
+    return {"path_blocks_per_tile": chosen, "tuning_s": time.perf_counter() - t0,
+            "shape": "21 strikes at T = 1, M = 2^20, per distinct N"}
+
+
 def bench_batched(rb, args):
     from rbergomi_amd.serving import Request, price_batch
     reqs = batched_requests()
@@ -171,12 +185,19 @@ def bench_batched(rb, args):
            "scaling": "weak", "vs_baseline": None, "data": "synthetic (device Philox4x32-10 Gaussians)",
            "config": {"workload": "batched: 256 parameter sets, N in {32..512}, M = 2^14..2^20, 21 strikes",
                       "paths_steps": work, "parallelism": "1 GPU"}}
-    # plain FP64 single-request GPU path: the requests one after another
-    t0 = time.perf_counter()
-    ref = [rb.price_v2(d["xi"], d["H"], d["eta"], d["rho"], d["T"], d["K"], d["N"], d["M"], d["seed"],
-                       precision=rb.PREC_FP64) for d in reqs]
-    fp64_s = time.perf_counter() - t0
-    out["fp64_single_request"] = {"s": fp64_s, "paths_steps_per_s": work / fp64_s}
+    # plain FP64 single-request GPU path: the requests one after another, two passes (the first
+    # also pays one-time costs such as first allocations of each size); the faster one counts
+    passes = []
+    for _ in range(2):
+        t0 = time.perf_counter()
+        ref = [rb.price_v2(d["xi"], d["H"], d["eta"], d["rho"], d["T"], d["K"], d["N"], d["M"], d["seed"],
+                           precision=rb.PREC_FP64) for d in reqs]
+        passes.append(time.perf_counter() - t0)
+    fp64_s = min(passes)
+    out["fp64_single_request"] = {"s": fp64_s, "paths_steps_per_s": work / fp64_s, "passes_s": passes}
+    precs = (("fp16", rb.PREC_SPLIT_FP16), ("bf16", rb.PREC_SPLIT_BF16), ("fp32gemm", rb.PREC_FP32_GEMM))
+    if args.autotune:
+        out["autotune"] = tune(rb, [d["N"] for d in reqs], args, precs)
     for name, code in (("fp16", rb.PREC_SPLIT_FP16), ("bf16", rb.PREC_SPLIT_BF16), ("fp32gemm", rb.PREC_FP32_GEMM)):
         rs = [Request(d["xi"], d["H"], d["eta"], d["rho"], d["T"], d["K"], d["N"], d["M"], d["seed"])
               for d in reqs]
@@ -203,6 +224,7 @@ def bench_batched(rb, args):
 
 def bench_serving(rb, args):
     reqs = serving_requests(args.requests)
+    tuned = tune(rb, [d["N"] for d in reqs], args, (("fp16", rb.PREC_SPLIT_FP16),)) if args.autotune else None
     res = run_serving(rb, reqs, rb.PREC_SPLIT_FP16)
     out = {"metric": "paths*steps/s", "value": res["open_loop_paths_steps_per_s"], "unit": "paths*steps/s",
            "n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -211,6 +233,8 @@ def bench_serving(rb, args):
                                   "10-200 options over 1-5 maturities, M = 2^14..2^21, N = max(32, 256 T)"
                                   % args.requests, "parallelism": "1 GPU"},
            "serving": res, "p50_ms": res["p50_ms"], "p99_ms": res["p99_ms"]}
+    if tuned:
+        out["autotune"] = tuned
     return out
 
 
@@ -266,6 +290,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--no-latency", action="store_true", help="skip the serving latency sample")
+    ap.add_argument("--autotune", action="store_true",
+                    help="batched / serving workloads: autotune the fused kernel's tile shapes first")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="also run the streaming copy probe over this many MiB (traffic calibration)")
     args = ap.parse_args()

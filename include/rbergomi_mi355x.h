@@ -215,6 +215,27 @@ typedef struct rb_request {
  * rb_price_v2 and the same cfg.  The server prices Romano-Touzi requests. */
 int rb_price_batch(const rb_request* reqs, int n, const rb_config* cfg);
 
+/* Tile-shape autotuning of the fused kernel.  No counterpart in the reference (which has no
+ * tiles).  A tile is 32 PB paths by all steps (PB = 1, 2 or 4 path blocks of 32; the four waves
+ * of a workgroup split 4 / PB per path block).  For each N[i] (1..1024) this times every valid
+ * PB with shape's parameters, options, M and seed (shape->N and its output pointers are ignored)
+ * under cfg's precision and method on the current device, and records the fastest for every
+ * later plan, batch and server request of the process with that padded N (multiple of 32),
+ * operand mode and estimator.  pb_out[i], ms_out[i] (may be NULL): the choice and the time of
+ * one launch with it.  Without an entry the default rule applies (PB = 4 for N <= 32 and
+ * 65..96, 2 for 33..64 and 97..192, 1 above), measured on MI355X.  Tile shapes change prices
+ * only through the order of partial sums: Romano-Touzi prices came out bit-identical for every
+ * shape; plain MC also reorders FP32 sums over a wave's row blocks (measured <= 6e-10 in price). */
+int rb_autotune_fused(const rb_request* shape, const int* N, int nN, const rb_config* cfg, int* pb_out,
+                      double* ms_out);
+/* The tuned table as text lines "Np mode mc pb" (mode 0 FP16, 1 BF16, 2 FP32; mc 1 = plain MC).
+ * The environment variable RBERGOMI_TUNE_CACHE names such a file, loaded on first use. */
+int rb_tune_save(const char* path);
+int rb_tune_load(const char* path);
+int rb_tune_clear(void);
+/* Copies up to max entries as (Np, mode, mc, pb) into rows[4*i..]; returns the entry count. */
+int rb_tune_table(int* rows, int max);
+
 /* Continuous-batching server.  A host thread forms batches from the pending requests, admits a
  * request when the device bytes of the batches in flight leave room for it, runs up to
  * max_inflight batches on separate streams (a new batch starts while earlier ones finish), and

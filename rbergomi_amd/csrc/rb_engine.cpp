@@ -13,7 +13,10 @@
 #include <complex>
 #include <cstdlib>
 #include <cstring>
+#include <deque>
 #include <map>
+#include <mutex>
+#include <set>
 #include <tuple>
 
 #include "rb_kernels.h"
@@ -29,6 +32,32 @@ const char* kernel_name(int id) {
                                          "k_payoff_f64",  "k_reduce_f64",   "k_iv_f64",
                                          "k_fused"};
     return id >= 0 && id < K_COUNT ? names[id] : "?";
+}
+
+hipError_t device_props(int device, hipDeviceProp_t& out) {
+    static std::mutex mu;
+    static std::map<int, hipDeviceProp_t> cache;
+    std::lock_guard<std::mutex> lock(mu);
+    const auto it = cache.find(device);
+    if (it != cache.end()) {
+        out = it->second;
+        return hipSuccess;
+    }
+    const hipError_t e = hipGetDeviceProperties(&out, device);
+    if (e == hipSuccess) cache[device] = out;
+    return e;
+}
+
+void keep_pool_cached(int device) {
+    static std::mutex mu;
+    static std::set<int> done;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!done.insert(device).second) return;
+    hipMemPool_t pool;
+    if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
+        uint64_t keep = uint64_t(64) << 30;
+        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+    }
 }
 
 int validate(const Request& r, std::string& err) {
@@ -203,14 +232,92 @@ PhiloxKey philox_key(const uint64_t* seed, int n) {
         }                                                                            \
     } while (0)
 
+// ---------------------------------------------------------------- workspace cache
+// Device buffers of finished plans are kept for later plans: a one-shot call (rb_price_v2, the
+// legacy symbols) otherwise spends far longer mapping and unmapping its staging buffers (up to
+// ~6 GB) than computing with them -- measured 4 to 12 s against 0.36 s of kernels for 256
+// small requests.  A request takes the smallest free block of its device that is at least its
+// size and at most twice it, else allocates; freed blocks beyond WS_KEEP bytes are returned to
+// the device, oldest first.  A block is reused only after the plan that released it has
+// synchronized its stream.
+namespace {
+struct WsBlock {
+    void* p;
+    size_t bytes;
+    int device;
+};
+std::mutex g_ws_mu;
+std::deque<WsBlock> g_ws_free;          // released blocks, oldest first
+std::map<void*, WsBlock> g_ws_used;
+size_t g_ws_free_bytes = 0;
+constexpr size_t WS_KEEP = size_t(48) << 30;
+constexpr size_t WS_GRAIN = size_t(2) << 20;
+
+// caller holds g_ws_mu
+void ws_evict(size_t keep, int device) {
+    for (auto it = g_ws_free.begin(); it != g_ws_free.end() && g_ws_free_bytes > keep;) {
+        if (device >= 0 && it->device != device) {
+            ++it;
+            continue;
+        }
+        int cur = 0;
+        (void)hipGetDevice(&cur);
+        if (cur != it->device) (void)hipSetDevice(it->device);
+        (void)hipFree(it->p);
+        if (cur != it->device) (void)hipSetDevice(cur);
+        g_ws_free_bytes -= it->bytes;
+        it = g_ws_free.erase(it);
+    }
+}
+}  // namespace
+
+hipError_t ws_alloc(void** p, size_t bytes) {
+    bytes = (std::max<size_t>(bytes, 1) + WS_GRAIN - 1) / WS_GRAIN * WS_GRAIN;
+    int device = 0;
+    hipError_t e = hipGetDevice(&device);
+    if (e != hipSuccess) return e;
+    std::lock_guard<std::mutex> lock(g_ws_mu);
+    auto best = g_ws_free.end();
+    for (auto it = g_ws_free.begin(); it != g_ws_free.end(); ++it)
+        if (it->device == device && it->bytes >= bytes && it->bytes <= 2 * bytes &&
+            (best == g_ws_free.end() || it->bytes < best->bytes))
+            best = it;
+    if (best != g_ws_free.end()) {
+        *p = best->p;
+        g_ws_used[best->p] = *best;
+        g_ws_free_bytes -= best->bytes;
+        g_ws_free.erase(best);
+        return hipSuccess;
+    }
+    e = hipMalloc(p, bytes);
+    if (e == hipErrorOutOfMemory) {  // give the device back what is cached, then try again
+        (void)hipGetLastError();
+        ws_evict(0, device);
+        e = hipMalloc(p, bytes);
+    }
+    if (e == hipSuccess) g_ws_used[*p] = WsBlock{*p, bytes, device};
+    return e;
+}
+
+void ws_free(void* p) {
+    if (!p) return;
+    std::lock_guard<std::mutex> lock(g_ws_mu);
+    const auto it = g_ws_used.find(p);
+    if (it == g_ws_used.end()) return;
+    g_ws_free.push_back(it->second);
+    g_ws_free_bytes += it->second.bytes;
+    g_ws_used.erase(it);
+    ws_evict(WS_KEEP, -1);
+}
+
 template <class T>
-static hipError_t dalloc(T** p, size_t n) {
-    return hipMalloc(reinterpret_cast<void**>(p), std::max<size_t>(n, 1) * sizeof(T));
+static hipError_t dalloc(T** p, size_t n, hipStream_t) {
+    return ws_alloc(reinterpret_cast<void**>(p), std::max<size_t>(n, 1) * sizeof(T));
 }
 
 template <class T>
 static hipError_t upload(T** p, const std::vector<T>& v, hipStream_t st) {
-    hipError_t e = dalloc(p, v.size());
+    hipError_t e = dalloc(p, v.size(), st);
     if (e != hipSuccess || v.empty()) return e;
     return hipMemcpyAsync(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st);
 }
@@ -224,8 +331,7 @@ Plan::~Plan() {
     void* bufs[] = {d_units, d_gamma, d_groups, d_comp, d_opts,  d_Z,     d_Wt,   d_ints,
                     d_partial, d_price, d_stat, d_iv, d_payoff, d_tw, d_ghat, d_ints2,
                     d_Qg, d_alpha, d_exc, d_count, d_chol, d_sobol, d_shift};
-    for (void* b : bufs)
-        if (b) (void)hipFree(b);
+    for (void* b : bufs) ws_free(b);  // the stream has synchronized above
     for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
     if (own_stream && stream) (void)hipStreamDestroy(stream);
 }
@@ -244,7 +350,8 @@ int Plan::init(const Request& r, hipStream_t st, bool quadrature, std::string& e
     RB_TRY(hipGetDevice(&device));
     {
         hipDeviceProp_t prop;
-        RB_TRY(hipGetDeviceProperties(&prop, device));
+        RB_TRY(device_props(device, prop));
+        keep_pool_cached(device);
         const size_t conv_lds = r.N <= VOLTERRA_MFMA_MAX_N ? volterra_mfma_lds_bytes(r.N) : volterra_f64_lds_bytes(r.N);
         if (conv_lds > prop.sharedMemPerBlock) {
             err = "N = " + std::to_string(r.N) + " needs more LDS than one workgroup can hold";
@@ -361,20 +468,20 @@ int Plan::init(const Request& r, hipStream_t st, bool quadrature, std::string& e
         }
         RB_TRY(upload(&d_chol, all, stream));
     }
-    RB_TRY(dalloc(&d_Z, size_t(chunk) * nstreams * N));
-    RB_TRY(dalloc(&d_Wt, size_t(nunits) * chunk * N));
-    RB_TRY(dalloc(&d_ints, size_t(ngroups) * chunk * 2));
-    RB_TRY(dalloc(&d_partial, size_t(nopt) * nblk * 2));
-    RB_TRY(dalloc(&d_price, nopt));
-    RB_TRY(dalloc(&d_stat, nopt));
-    RB_TRY(dalloc(&d_iv, nopt));
-    if (quadrature) RB_TRY(dalloc(&d_payoff, size_t(chunk) * nopt));
+    RB_TRY(dalloc(&d_Z, size_t(chunk) * nstreams * N, stream));
+    RB_TRY(dalloc(&d_Wt, size_t(nunits) * chunk * N, stream));
+    RB_TRY(dalloc(&d_ints, size_t(ngroups) * chunk * 2, stream));
+    RB_TRY(dalloc(&d_partial, size_t(nopt) * nblk * 2, stream));
+    RB_TRY(dalloc(&d_price, nopt, stream));
+    RB_TRY(dalloc(&d_stat, nopt, stream));
+    RB_TRY(dalloc(&d_iv, nopt, stream));
+    if (quadrature) RB_TRY(dalloc(&d_payoff, size_t(chunk) * nopt, stream));
     if (r.turbo) {
-        RB_TRY(dalloc(&d_ints2, size_t(ngroups) * chunk * 2));
-        RB_TRY(dalloc(&d_Qg, size_t(ngroups)));
-        RB_TRY(dalloc(&d_alpha, size_t(nopt)));
-        RB_TRY(dalloc(&d_exc, size_t(nopt) * nblk));
-        RB_TRY(dalloc(&d_count, size_t(nopt)));
+        RB_TRY(dalloc(&d_ints2, size_t(ngroups) * chunk * 2, stream));
+        RB_TRY(dalloc(&d_Qg, size_t(ngroups), stream));
+        RB_TRY(dalloc(&d_alpha, size_t(nopt), stream));
+        RB_TRY(dalloc(&d_exc, size_t(nopt) * nblk, stream));
+        RB_TRY(dalloc(&d_count, size_t(nopt), stream));
     }
     RB_TRY(hipStreamSynchronize(stream));
     return ST_OK;
@@ -567,7 +674,7 @@ int Plan::init_fft(std::string& err) {
     if (log2L < 1) log2L = 1;
     const int L = 1 << log2L;
     hipDeviceProp_t prop;
-    RB_TRY(hipGetDeviceProperties(&prop, device));
+    RB_TRY(device_props(device, prop));
     if (volterra_fft_lds_bytes(log2L) > prop.sharedMemPerBlock) {
         err = "N too large for the LDS FFT";
         return ST_ERR_LIMIT;

@@ -171,6 +171,15 @@ struct Plan {
 enum KernelId { K_NORMALS = 0, K_VOLTERRA, K_SCAN, K_PAYOFF, K_REDUCE, K_IV, K_FUSED, K_COUNT };
 const char* kernel_name(int id);
 
+// Device properties, queried once per device (the query is slow next to a small request).
+hipError_t device_props(int device, hipDeviceProp_t& out);
+// Device buffers of plans, reused across plans of the process (see rb_engine.cpp)
+hipError_t ws_alloc(void** p, size_t bytes);
+void ws_free(void* p);
+// Keeps up to 64 GiB of freed stream-ordered memory in the device's default pool (set once per
+// device), so that plans and batches reuse it instead of remapping.
+void keep_pool_cached(int device);
+
 // thread-local error channel of the C API
 void set_last_error(const std::string& s);
 const char* last_error();

@@ -12,8 +12,13 @@
 #include "rb_fused_host.h"
 
 #include <algorithm>
+#include <array>
+#include <cstdio>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -22,6 +27,89 @@
 #include "rb_kernels.h"
 
 namespace rb {
+
+// ---------------------------------------------------------------- tile shape choice
+// Path blocks of 32 per tile (the tile is 32 PB paths by Np steps; the four waves split into
+// 4 / PB per path block).  Measured on MI355X (split FP16, Romano-Touzi, 205 options, M = 2^21):
+// N = 96: 2.96 / 2.58 / 2.31 ms for PB = 1 / 2 / 4; N = 128: 3.04 / 2.72 / 4.39; N = 160: 3.72 /
+// 3.20 / 5.39; N = 192: 3.91 / 3.33 / 6.16; N = 256: 4.27 / 6.63 / 7.29; N >= 320: PB = 1 best.
+// The default follows that; autotune_fused replaces it per (Np, mode, estimator) by timing.
+namespace {
+std::mutex g_tune_mu;
+std::map<std::array<int, 3>, int> g_tune;  // (Np, mode, mc) -> PB
+std::once_flag g_tune_env;
+}  // namespace
+
+int fused_default_pb(int NB) { return NB == 1 ? 4 : NB == 2 ? 2 : NB == 3 ? 4 : NB <= 6 ? 2 : 1; }
+
+int fused_pb_choice(int Np, int mode, int mc) {
+    std::call_once(g_tune_env, [] {
+        // a cache file written by tune_save, loaded once per process
+        if (const char* path = std::getenv("RBERGOMI_TUNE_CACHE")) {
+            std::string err;
+            (void)tune_load(path, err);
+        }
+    });
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    const auto it = g_tune.find({Np, mode, mc});
+    return it != g_tune.end() ? it->second : fused_default_pb(Np / 32);
+}
+
+void tune_set(int Np, int mode, int mc, int pb) {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    g_tune[{Np, mode, mc}] = pb;
+}
+
+void tune_clear() {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    g_tune.clear();
+}
+
+// text file, one line per entry: "Np mode mc pb"
+int tune_save(const char* path, std::string& err) {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    FILE* f = std::fopen(path, "w");
+    if (!f) {
+        err = std::string("cannot write ") + path;
+        return ST_ERR_ARG;
+    }
+    for (const auto& e : g_tune) std::fprintf(f, "%d %d %d %d\n", e.first[0], e.first[1], e.first[2], e.second);
+    std::fclose(f);
+    return ST_OK;
+}
+
+int tune_load(const char* path, std::string& err) {
+    FILE* f = std::fopen(path, "r");
+    if (!f) {
+        err = std::string("cannot read ") + path;
+        return ST_ERR_ARG;
+    }
+    std::vector<std::array<int, 4>> rows;
+    int np, mode, mc, pb, line = 0;
+    char buf[256];
+    while (std::fgets(buf, sizeof buf, f)) {
+        ++line;
+        if (buf[0] == '#' || buf[0] == '\n') continue;
+        if (std::sscanf(buf, "%d %d %d %d", &np, &mode, &mc, &pb) != 4 || np < 32 || np % 32 || mode < 0 ||
+            mode > 2 || mc < 0 || mc > 1 || (pb != 1 && pb != 2 && pb != 4)) {
+            std::fclose(f);
+            err = std::string(path) + ": malformed entry on line " + std::to_string(line);
+            return ST_ERR_ARG;
+        }
+        rows.push_back({np, mode, mc, pb});
+    }
+    std::fclose(f);
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    for (const auto& r : rows) g_tune[{r[0], r[1], r[2]}] = r[3];
+    return ST_OK;
+}
+
+std::vector<std::array<int, 4>> tune_table() {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    std::vector<std::array<int, 4>> t;
+    for (const auto& e : g_tune) t.push_back({e.first[0], e.first[1], e.first[2], e.second});
+    return t;
+}
 
 static uint16_t to_bf16_bits(float f) {
     uint32_t u;
@@ -51,7 +139,7 @@ static void split16(double x, int mode, uint16_t& hi, uint16_t& lo) {
 }
 
 int plan_fused_request(const Request& req, const Layout& lay, int mode, FusedHostReq& out,
-                       std::string& err) {
+                       std::string& err, int pb) {
     const int N = req.N;
     const int Np = (N + 31) / 32 * 32;
     if (Np > 1024) {
@@ -64,29 +152,33 @@ int plan_fused_request(const Request& req, const Layout& lay, int mode, FusedHos
     a.Np = Np;
     a.NK = Np / 16;
     const int NB = Np / 32;
-    a.PB = NB >= 3 ? 1 : (NB == 2 ? 2 : 4);
+    // path blocks of 32 per tile: given, else the tuned or default choice for this shape
+    a.PB = pb > 0 ? pb : fused_pb_choice(Np, mode, req.method == METHOD_MC ? 1 : 0);
+    if (a.PB != 1 && a.PB != 2 && a.PB != 4) {
+        err = "path blocks per tile must be 1, 2 or 4";
+        return ST_ERR_ARG;
+    }
     a.TP = 32 * a.PB;
     a.log2TP = a.PB == 1 ? 5 : (a.PB == 2 ? 6 : 7);
-    // ---- waves: output row blocks (cost rb+1), all row blocks of a wave in one path block
+    // ---- waves: 4 / PB per path block; a block's output row blocks (cost rb + 1) go to its
+    //      waves by longest-processing-time first
     std::vector<std::vector<int>> wrb(4);
-    for (int w = 0; w < 4; ++w) a.wave_pb[w] = -1;
-    if (a.PB == 1) {
-        std::vector<long> load(4, 0);
+    const int wpb = 4 / a.PB;
+    for (int b = 0; b < a.PB; ++b) {
+        std::vector<long> load(wpb, 0);
         for (int rb = NB - 1; rb >= 0; --rb) {
             const int w = int(std::min_element(load.begin(), load.end()) - load.begin());
-            wrb[w].push_back(rb);
+            wrb[b * wpb + w].push_back(rb);
             load[w] += rb + 1;
         }
-        for (int w = 0; w < 4; ++w)
-            if (!wrb[w].empty()) a.wave_pb[w] = 0;
-    } else {
-        for (int w = 0; w < 4; ++w) {
-            a.wave_pb[w] = w / NB;
-            wrb[w].push_back(w % NB);
-        }
     }
+    for (int w = 0; w < 4; ++w) a.wave_pb[w] = wrb[w].empty() ? -1 : w / wpb;
     size_t upw = 1;
     for (auto& v : wrb) upw = std::max(upw, v.size());
+    if (upw > size_t(FUSED_MAX_UPW)) {
+        err = "too many row blocks per wave for " + std::to_string(a.PB) + " path blocks per tile";
+        return ST_ERR_LIMIT;
+    }
     out.upw = upw <= 1 ? 1 : upw <= 2 ? 2 : upw <= 4 ? 4 : 8;
     for (int w = 0; w < 4; ++w) {
         // row blocks of a wave in decreasing order (the GEMM phases rely on it)
@@ -342,7 +434,8 @@ int FusedBatch::build(const std::vector<const FusedHostReq*>& reqs, int mode_, i
     args.pgd_n = pg;
     args.acc_d = ac;
     hipDeviceProp_t prop;
-    RB_TRY(hipGetDeviceProperties(&prop, device));
+    RB_TRY(device_props(device, prop));
+    keep_pool_cached(device);
     lds_need = fused_lds_bytes(args);
     if (lds_need > prop.sharedMemPerBlock) {
         err = "fused kernel needs " + std::to_string(lds_need) + " bytes of LDS (too many options or groups)";
@@ -427,6 +520,76 @@ int FusedBatch::results(int q, double* price, double* iv, double* stat, hipStrea
     if (iv) RB_TRY(hipMemcpyAsync(iv, d_iv + out_off[q], n, hipMemcpyDeviceToHost, st));
     if (stat) RB_TRY(hipMemcpyAsync(stat, d_stat + out_off[q], n, hipMemcpyDeviceToHost, st));
     return ST_OK;
+}
+
+// ---------------------------------------------------------------- autotuning
+int autotune_fused(const Request& shape, const std::vector<int>& Ns, int mode, int device,
+                   std::vector<int>& chosen, std::vector<double>& best_ms, std::string& err) {
+    chosen.assign(Ns.size(), 0);
+    best_ms.assign(Ns.size(), 0.0);
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = ST_OK;
+    auto hip_ok = [&](hipError_t e, const char* what) {
+        if (e == hipSuccess) return true;
+        err = std::string(what) + ": " + hipGetErrorString(e);
+        rc = e == hipErrorOutOfMemory ? ST_ERR_NOMEM : ST_ERR_DEVICE;
+        return false;
+    };
+    if (!hip_ok(hipSetDevice(device), "hipSetDevice") ||
+        !hip_ok(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate") ||
+        !hip_ok(hipEventCreate(&e0), "hipEventCreate") || !hip_ok(hipEventCreate(&e1), "hipEventCreate"))
+        goto done;
+    for (size_t n = 0; n < Ns.size() && rc == ST_OK; ++n) {
+        Request r = shape;
+        r.N = Ns[n];
+        rc = validate(r, err);
+        if (rc) break;
+        const Layout lay = make_layout(r);
+        const int Np = (r.N + 31) / 32 * 32;
+        double best = 0.0;
+        int best_pb = 0;
+        for (int pb : {1, 2, 4}) {
+            FusedHostReq fh;
+            std::string e2;
+            if (plan_fused_request(r, lay, mode, fh, e2, pb) != ST_OK) continue;  // shape not valid
+            FusedBatch B;
+            if (B.build({&fh}, mode, device, st, e2) != ST_OK) continue;          // e.g. LDS
+            // one untimed launch, then the fastest of three
+            double ms = 0.0;
+            for (int it = 0; it < 4 && rc == ST_OK; ++it) {
+                if (!hip_ok(hipEventRecord(e0, st), "hipEventRecord")) break;
+                if ((rc = B.launch(st, err)) != ST_OK) break;
+                if (!hip_ok(hipEventRecord(e1, st), "hipEventRecord") ||
+                    !hip_ok(hipEventSynchronize(e1), "hipEventSynchronize"))
+                    break;
+                float t = 0.f;
+                if (!hip_ok(hipEventElapsedTime(&t, e0, e1), "hipEventElapsedTime")) break;
+                if (it > 0 && (ms == 0.0 || t < ms)) ms = t;
+            }
+            B.release();
+            (void)hipStreamSynchronize(st);
+            if (rc != ST_OK) break;
+            if (best_pb == 0 || ms < best) {
+                best = ms;
+                best_pb = pb;
+            }
+        }
+        if (rc != ST_OK) break;
+        if (best_pb == 0) {
+            err = "no tile shape of the fused kernel fits N = " + std::to_string(r.N);
+            rc = ST_ERR_LIMIT;
+            break;
+        }
+        tune_set(Np, mode, r.method == METHOD_MC ? 1 : 0, best_pb);
+        chosen[n] = best_pb;
+        best_ms[n] = best;
+    }
+done:
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    return rc;
 }
 
 // ---------------------------------------------------------------- single-request plan

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <array>
 #include <vector>
 
 #include "rb_fused.h"
@@ -29,8 +30,24 @@ struct FusedHostReq {
     std::vector<Option> ivopts;  // per output I: K, T for the implied vols
 };
 
+// pb: path blocks of 32 per tile (1, 2 or 4); 0 takes fused_pb_choice
 int plan_fused_request(const Request& req, const Layout& lay, int mode, FusedHostReq& out,
-                       std::string& err);
+                       std::string& err, int pb = 0);
+
+// Tile shape (path blocks per tile) per (padded N, operand mode, plain MC): an autotuned entry,
+// else the measured default rule.  The table is process-wide and thread-safe; the environment
+// variable RBERGOMI_TUNE_CACHE names a file (tune_save's format) loaded on first use.
+int fused_default_pb(int NB);
+int fused_pb_choice(int Np, int mode, int mc);
+void tune_set(int Np, int mode, int mc, int pb);
+void tune_clear();
+int tune_save(const char* path, std::string& err);
+int tune_load(const char* path, std::string& err);
+std::vector<std::array<int, 4>> tune_table();
+// Times every valid tile shape for each N with `shape`'s parameters, options and M on `device`
+// and records the fastest; chosen[i] / best_ms[i] report it for Ns[i].
+int autotune_fused(const Request& shape, const std::vector<int>& Ns, int mode, int device,
+                   std::vector<int>& chosen, std::vector<double>& best_ms, std::string& err);
 
 // Any number of requests priced by one launch of the fused kernel.
 struct FusedBatch {

@@ -46,15 +46,11 @@ struct Job {
     bool done = false;
 };
 
-int plan_job(const rb_request* q, int precision, int method, int comp, Job& j, std::string& err) {
-    Request& r = j.req;
+// the request's inputs (not its output pointers) as a fused-kernel Request
+int request_of(const rb_request* q, int precision, int method, int comp, Request& r, std::string& err) {
     r.xi = q->xi;
     if (q->par_size < 1 || q->op_size < 1 || !q->H || !q->eta || !q->rho || !q->T || !q->K) {
         err = "empty or null parameter arrays";
-        return ST_ERR_ARG;
-    }
-    if (!q->price || !q->stat) {
-        err = "null output arrays";
         return ST_ERR_ARG;
     }
     r.H.assign(q->H, q->H + q->par_size);
@@ -72,8 +68,17 @@ int plan_job(const rb_request* q, int precision, int method, int comp, Job& j, s
         err = "batches run the fused kernel (RB_PREC_SPLIT_FP16, RB_PREC_SPLIT_BF16 or RB_PREC_FP32_GEMM)";
         return ST_ERR_ARG;
     }
-    int rc = validate(r, err);
+    return validate(r, err);
+}
+
+int plan_job(const rb_request* q, int precision, int method, int comp, Job& j, std::string& err) {
+    if (!q->price || !q->stat) {
+        err = "null output arrays";
+        return ST_ERR_ARG;
+    }
+    int rc = request_of(q, precision, method, comp, j.req, err);
     if (rc) return rc;
+    const Request& r = j.req;
     j.lay = make_layout(r);
     rc = plan_fused_request(r, j.lay, fused_mode_of(precision), j.fh, err);
     if (rc) return rc;
@@ -291,6 +296,74 @@ int rb_price_batch(const rb_request* reqs, int n, const rb_config* cfg) {
     (void)hipStreamDestroy(st);
     if (rc) set_last_error(err);
     return rc;
+}
+
+int rb_autotune_fused(const rb_request* shape, const int* N, int nN, const rb_config* cfg, int* pb_out,
+                      double* ms_out) {
+    set_last_error("");
+    if (!shape || !N || nN < 1 || !cfg) {
+        set_last_error("need a request shape, at least one N and a configuration");
+        return RB_ERR_ARG;
+    }
+    Request r;
+    std::string err;
+    rb_request q = *shape;
+    q.N = N[0];
+    int rc = request_of(&q, cfg->precision, cfg->method, cfg->compensator, r, err);
+    if (rc) {
+        set_last_error(err);
+        return rc;
+    }
+    for (int i = 0; i < nN; ++i)
+        if (N[i] < 1 || N[i] > 1024) {
+            set_last_error("the fused kernel covers 1 <= N <= 1024");
+            return RB_ERR_LIMIT;
+        }
+    int device = 0;
+    if (hipGetDevice(&device) != hipSuccess) {
+        set_last_error("no HIP device");
+        return RB_ERR_DEVICE;
+    }
+    std::vector<int> chosen;
+    std::vector<double> ms;
+    rc = autotune_fused(r, std::vector<int>(N, N + nN), fused_mode_of(cfg->precision), device, chosen, ms, err);
+    if (rc) {
+        set_last_error(err);
+        return rc;
+    }
+    for (int i = 0; i < nN; ++i) {
+        if (pb_out) pb_out[i] = chosen[i];
+        if (ms_out) ms_out[i] = ms[i];
+    }
+    return RB_OK;
+}
+
+int rb_tune_save(const char* path) {
+    set_last_error("");
+    std::string err;
+    const int rc = path ? tune_save(path, err) : RB_ERR_ARG;
+    if (rc) set_last_error(path ? err : "null path");
+    return rc;
+}
+
+int rb_tune_load(const char* path) {
+    set_last_error("");
+    std::string err;
+    const int rc = path ? tune_load(path, err) : RB_ERR_ARG;
+    if (rc) set_last_error(path ? err : "null path");
+    return rc;
+}
+
+int rb_tune_clear(void) {
+    tune_clear();
+    return RB_OK;
+}
+
+int rb_tune_table(int* rows, int max) {
+    const std::vector<std::array<int, 4>> t = tune_table();
+    for (int i = 0; rows && i < max && i < int(t.size()); ++i)
+        for (int c = 0; c < 4; ++c) rows[4 * i + c] = t[i][c];
+    return int(t.size());
 }
 
 int rb_server_create(const rb_server_config* cfg, rb_server** out) {

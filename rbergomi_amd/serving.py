@@ -35,6 +35,12 @@ def _declare(L):
     L.rb_server_stats.argtypes = [vp, dp, dp, ctypes.POINTER(ctypes.c_long), dp, ctypes.POINTER(ctypes.c_long)]
     L.rb_server_destroy.argtypes = [vp]
     L.rb_server_destroy.restype = None
+    ip = ctypes.POINTER(ctypes.c_int)
+    L.rb_autotune_fused.argtypes = [ctypes.POINTER(RbRequest), ip, ctypes.c_int, ctypes.POINTER(RbConfig), ip, dp]
+    L.rb_tune_save.argtypes = [ctypes.c_char_p]
+    L.rb_tune_load.argtypes = [ctypes.c_char_p]
+    L.rb_tune_clear.argtypes = []
+    L.rb_tune_table.argtypes = [ip, ctypes.c_int]
     L._serving_declared = True
 
 
@@ -65,6 +71,50 @@ def price_batch(requests, precision=_lib.PREC_SPLIT_FP16, method=_lib.METHOD_RT,
     cfg = RbConfig(precision, compensator, method, 0)
     check(L.rb_price_batch(arr, len(requests), ctypes.byref(cfg)))
     return requests
+
+
+def autotune(shape, Ns, precision=_lib.PREC_SPLIT_FP16, method=_lib.METHOD_RT, compensator=_lib.COMP_REF):
+    """Times the fused kernel's tile shapes for each N in Ns with the request shape `shape` (a
+    Request; its N is ignored) and records the fastest for later plans, batches and server
+    requests of this process (rb_autotune_fused).  Returns {N: (path blocks per tile, ms)}."""
+    L = _lib.load()
+    _declare(L)
+    Ns = np.ascontiguousarray(np.asarray(Ns, dtype=np.int32))
+    pb = np.zeros(len(Ns), dtype=np.int32)
+    ms = np.zeros(len(Ns))
+    ip = ctypes.POINTER(ctypes.c_int)
+    cfg = RbConfig(precision, compensator, method, 0)
+    check(L.rb_autotune_fused(ctypes.byref(shape.c), Ns.ctypes.data_as(ip), len(Ns), ctypes.byref(cfg),
+                              pb.ctypes.data_as(ip), ms.ctypes.data_as(dp)))
+    return {int(n): (int(p), float(t)) for n, p, t in zip(Ns, pb, ms)}
+
+
+def tune_table():
+    """The tuned entries as (padded N, operand mode, plain MC, path blocks per tile) tuples."""
+    L = _lib.load()
+    _declare(L)
+    n = L.rb_tune_table(None, 0)
+    rows = np.zeros(4 * max(n, 1), dtype=np.int32)
+    L.rb_tune_table(rows.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), n)
+    return [tuple(int(v) for v in rows[4 * i:4 * i + 4]) for i in range(n)]
+
+
+def tune_save(path):
+    L = _lib.load()
+    _declare(L)
+    check(L.rb_tune_save(str(path).encode()))
+
+
+def tune_load(path):
+    L = _lib.load()
+    _declare(L)
+    check(L.rb_tune_load(str(path).encode()))
+
+
+def tune_clear():
+    L = _lib.load()
+    _declare(L)
+    check(L.rb_tune_clear())
 
 
 class Server:
