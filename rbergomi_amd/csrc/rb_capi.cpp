@@ -584,7 +584,13 @@ int rb_probe_wtilde(const double* Z, long M, int N, double H, double* Wtilde) {
     PROBE_TRY(hipMemcpy(dZ, Z, size_t(M) * 2 * N * sizeof(double), hipMemcpyHostToDevice));
     PROBE_TRY(hipMemcpy(dg, L.gamma.data(), size_t(N) * sizeof(double), hipMemcpyHostToDevice));
     PROBE_TRY(hipMemcpy(du, L.units.data(), sizeof(Unit), hipMemcpyHostToDevice));
-    PROBE_TRY(launch_volterra_conv_f64(dZ, 2L * N, M, N, 1, du, dg, dW, nullptr));
+    {
+        int dev = 0;
+        hipDeviceProp_t prop;
+        PROBE_TRY(hipGetDevice(&dev));
+        PROBE_TRY(device_props(dev, prop));
+        PROBE_TRY(launch_volterra_conv_f64(dZ, 2L * N, M, N, 1, du, dg, dW, prop.sharedMemPerBlock, nullptr));
+    }
     PROBE_TRY(hipMemcpy(Wtilde, dW, size_t(M) * N * sizeof(double), hipMemcpyDeviceToHost));
 done:
     if (dZ) (void)hipFree(dZ);

@@ -352,7 +352,11 @@ int Plan::init(const Request& r, hipStream_t st, bool quadrature, std::string& e
         hipDeviceProp_t prop;
         RB_TRY(device_props(device, prop));
         keep_pool_cached(device);
-        const size_t conv_lds = r.N <= VOLTERRA_MFMA_MAX_N ? volterra_mfma_lds_bytes(r.N) : volterra_f64_lds_bytes(r.N);
+        lds_limit = prop.sharedMemPerBlock;
+        const int kind = volterra_conv_kind(r.N, lds_limit);
+        const size_t conv_lds = kind == 0 ? volterra_mfma_lds_bytes(r.N)
+                                : kind == 1 ? volterra_mfma_long_lds_bytes(r.N, volterra_mfma_long_segment(r.N, lds_limit))
+                                            : volterra_f64_lds_bytes(r.N);
         if (conv_lds > prop.sharedMemPerBlock) {
             err = "N = " + std::to_string(r.N) + " needs more LDS than one workgroup can hold";
             return ST_ERR_LIMIT;
@@ -497,8 +501,10 @@ void Plan::begin_timing() {
         times[K_VOLTERRA].name = "k_volterra_chol_f64";
     else if (conv_fft)
         times[K_VOLTERRA].name = "k_volterra_fft";
-    else if (req.N <= VOLTERRA_MFMA_MAX_N)
+    else if (volterra_conv_kind(req.N, lds_limit) == 0)
         times[K_VOLTERRA].name = "k_volterra_mfma_f64";
+    else if (volterra_conv_kind(req.N, lds_limit) == 1)
+        times[K_VOLTERRA].name = "k_volterra_mfma_long_f64";
 }
 
 int Plan::timed(int id) {
@@ -704,7 +710,7 @@ hipError_t Plan::conv(long np, long zstride) {
         return launch_volterra_fft(d_Z, zstride, np, req.N, log2L, int(lay.units.size()), d_units, d_ghat,
                                    d_tw, d_Wt, stream);
     return launch_volterra_conv_f64(d_Z, zstride, np, req.N, int(lay.units.size()), d_units, d_gamma, d_Wt,
-                                    stream);
+                                    lds_limit, stream);
 }
 
 int Plan::turbo_results(double* alpha, double* Q, long* exceptions, std::string& err) {

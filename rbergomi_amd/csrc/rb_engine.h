@@ -127,7 +127,8 @@ struct Plan {
     double* d_alpha = nullptr;    // [options] control-variate coefficients
     int* d_exc = nullptr;         // [options][nblk] Q <= Iv evaluations per path block
     long* d_count = nullptr;      // [options] their totals
-    // long-path FP32 FFT convolution (replaces k_volterra_f64 in the staged pipeline)
+    size_t lds_limit = 65536;     // LDS bytes one workgroup may use on the plan's device
+    // long-path FP32 FFT convolution (replaces the FP64 convolution in the staged pipeline)
     bool conv_fft = false;
     int log2L = 0;
     float2* d_tw = nullptr;       // exp(-2 pi i k / L), k < L/2

@@ -23,13 +23,25 @@ constexpr int VOLTERRA_MFMA_MAX_N = 1024;
 size_t volterra_mfma_lds_bytes(int N);
 hipError_t launch_volterra_mfma_f64(const double* Z, long zstride, long np, int N, int nunits,
                                     const Unit* units, const double* gamma, double* Wt, hipStream_t st);
-// FP64 convolution of the staged path: the matrix-core kernel for N <= VOLTERRA_MFMA_MAX_N,
-// the direct sum beyond
+// beyond: the W1 tile streamed through LDS in segments (segment length for an LDS limit)
+int volterra_mfma_long_segment(int N, size_t lds_limit);
+size_t volterra_mfma_long_lds_bytes(int N, int S);
+hipError_t launch_volterra_mfma_long_f64(const double* Z, long zstride, long np, int N, int nunits,
+                                         const Unit* units, const double* gamma, double* Wt,
+                                         size_t lds_limit, hipStream_t st);
+// FP64 convolution of the staged path: on the matrix cores (whole W1 tile in LDS up to
+// VOLTERRA_MFMA_MAX_N, segmented beyond while a 16-step segment fits), else the direct sum
+inline int volterra_conv_kind(int N, size_t lds_limit) {
+    return N <= VOLTERRA_MFMA_MAX_N ? 0 : volterra_mfma_long_segment(N, lds_limit) >= 16 ? 1 : 2;
+}
 inline hipError_t launch_volterra_conv_f64(const double* Z, long zstride, long np, int N, int nunits,
                                            const Unit* units, const double* gamma, double* Wt,
-                                           hipStream_t st) {
-    return N <= VOLTERRA_MFMA_MAX_N ? launch_volterra_mfma_f64(Z, zstride, np, N, nunits, units, gamma, Wt, st)
-                                    : launch_volterra_f64(Z, zstride, np, N, nunits, units, gamma, Wt, st);
+                                           size_t lds_limit, hipStream_t st) {
+    switch (volterra_conv_kind(N, lds_limit)) {
+        case 0: return launch_volterra_mfma_f64(Z, zstride, np, N, nunits, units, gamma, Wt, st);
+        case 1: return launch_volterra_mfma_long_f64(Z, zstride, np, N, nunits, units, gamma, Wt, lds_limit, st);
+        default: return launch_volterra_f64(Z, zstride, np, N, nunits, units, gamma, Wt, st);
+    }
 }
 hipError_t launch_scan_f64(int method, const double* Z, long zstride, const double* Wt, long np,
                            int N, const Group* groups, int ngroups, const double* comp,

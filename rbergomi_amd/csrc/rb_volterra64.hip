@@ -133,6 +133,145 @@ k_volterra_mfma_f64(const double* __restrict__ Z, long zstride, long np, int N,
     }
 }
 
+// ---------------------------------------------------------------- N > 1024
+// The W1 tile of all steps no longer fits in LDS.  A workgroup owns 16 paths and a range of
+// V64_RR output rows (32 row blocks, 8 per wave in the same equal-cost pairs), keeps their
+// accumulators in registers, and streams W1 through LDS in segments of S steps (S from the LDS
+// left after the zero-padded Gamma).  A row block takes, in each segment, the chunks of that
+// segment at or below its diagonal.  Consecutive workgroups share their paths (row range
+// fastest), so the repeated W1 reads of one path tile meet in L2.
+constexpr int V64_RR = 512;
+
+size_t volterra_mfma_long_lds_bytes(int N, int S) {
+    const int Np = (N + 15) / 16 * 16;
+    return size_t(16 + Np) * sizeof(double) + size_t(S) * V64_STR * sizeof(double);
+}
+
+// Grid: (row ranges x path tiles, units), x = rr + nrr * tile.  Block 256.
+__global__ void __launch_bounds__(256)
+k_volterra_mfma_long_f64(const double* __restrict__ Z, long zstride, long np, int N, int S,
+                         const Unit* __restrict__ units, const double* __restrict__ gamma,
+                         double* __restrict__ Wt) {
+    extern __shared__ double lds[];
+    const int Np = (N + 15) & ~15;
+    double* Gs = lds;            // as in k_volterra_mfma_f64
+    double* Ws = lds + 16 + Np;  // Ws[kk * V64_STR + p] = W1 of path p0 + p at step ks + kk
+    const int nrr = (Np + V64_RR - 1) / V64_RR;
+    const int rr = int(blockIdx.x % unsigned(nrr));
+    const long p0 = long(blockIdx.x / unsigned(nrr)) * V64_PATHS;
+    const int u = blockIdx.y;
+    const Unit un = units[u];
+    const double* gu = gamma + long(u) * N;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int m = lane & 15, kq = lane >> 4;
+    const int R0 = rr * V64_RR, R1 = R0 + V64_RR < Np ? R0 + V64_RR : Np;
+    for (int t = tid; t < 16 + Np; t += 256) {
+        const int s = t - 16;
+        Gs[t] = (s >= 1 && s < N) ? gu[s] : 0.0;
+    }
+    // row block h of this wave: local block 8 (h / 2) + (h odd ? 7 - wave : wave)
+    doublex4_t acc0[8], acc1[8];
+#pragma unroll
+    for (int h = 0; h < 8; ++h)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc0[h][r] = acc1[h][r] = 0.0;
+    // steps k <= R1 - 2 feed rows below R1
+    for (int ks = 0; ks < R1 - 1; ks += S) {
+        __syncthreads();  // the previous segment's reads are done
+        for (int k0 = 0; k0 < S; k0 += 256) {
+            double v[4][4];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                const long pp = p0 + wave + 4 * a;
+                const double* src = Z + pp * zstride;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int kk = k0 + 64 * j + lane, k = ks + kk;
+                    v[a][j] = (kk < S && pp < np && k < N) ? src[k] : 0.0;
+                }
+            }
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int kk = k0 + 64 * j + lane;
+                    if (kk < S) Ws[kk * V64_STR + wave + 4 * a] = v[a][j];
+                }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int h = 0; h < 8; ++h) {
+            const int rb = (R0 >> 4) + 8 * (h >> 1) + ((h & 1) ? 7 - wave : wave);
+            if (16 * rb >= R1) continue;
+            const int i0 = 16 * rb;
+            // chunks c (steps 4c .. 4c + 3) of this segment at or below the diagonal: c < 4 rb + 4
+            const int c_lo = ks >> 2;
+            const int c_hi = min((ks + S) >> 2, 4 * rb + 4);
+            const double* ga = Gs + 16 + i0 + m - kq;
+            const double* wb = Ws + (kq - ks) * V64_STR + m;  // B value of chunk c: wb[4c V64_STR]
+            for (int c = c_lo; c < c_hi; c += 4) {
+                double a[4], b[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    a[q] = ga[-4 * (c + q)];
+                    b[q] = wb[4 * (c + q) * V64_STR];
+                }
+#pragma unroll
+                for (int q = 0; q < 4; q += 2) {
+                    acc0[h] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[q], b[q], acc0[h], 0, 0, 0);
+                    acc1[h] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[q + 1], b[q + 1], acc1[h], 0, 0, 0);
+                }
+            }
+        }
+    }
+    const long p = p0 + m;
+    if (p >= np) return;
+    const double* W1 = Z + p * zstride;
+    const double* W1p = W1 + N;
+    double* out = Wt + (long(u) * np + p) * N;
+#pragma unroll
+    for (int h = 0; h < 8; ++h) {
+        const int rb = (R0 >> 4) + 8 * (h >> 1) + ((h & 1) ? 7 - wave : wave);
+        if (16 * rb >= R1) continue;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int i = 16 * rb + 4 * r + kq;
+            if (i < N) {
+                const double y = acc0[h][r] + acc1[h][r];
+                out[i] = un.sc * y + un.sc * (un.a * W1[i] + un.b * W1p[i]);
+            }
+        }
+    }
+}
+
+// segment length for N: the LDS left after Gamma, in multiples of 16 steps, at most Np
+int volterra_mfma_long_segment(int N, size_t lds_limit) {
+    const int Np = (N + 15) / 16 * 16;
+    const long avail = long(lds_limit) - long(16 + Np) * long(sizeof(double));
+    long S = avail / long(V64_STR * sizeof(double));
+    S = S / 16 * 16;
+    return int(S < Np ? S : Np);
+}
+
+hipError_t launch_volterra_mfma_long_f64(const double* Z, long zstride, long np, int N, int nunits,
+                                         const Unit* units, const double* gamma, double* Wt,
+                                         size_t lds_limit, hipStream_t st) {
+    const int S = volterra_mfma_long_segment(N, lds_limit);
+    if (S < 16) return hipErrorInvalidValue;
+    const size_t lds = volterra_mfma_long_lds_bytes(N, S);
+    if (lds > 65536) {
+        const hipError_t e = allow_lds(reinterpret_cast<const void*>(&k_volterra_mfma_long_f64), lds);
+        if (e != hipSuccess) return e;
+    }
+    const int Np = (N + 15) / 16 * 16;
+    const long nrr = (Np + V64_RR - 1) / V64_RR;
+    const long tiles = (np + V64_PATHS - 1) / V64_PATHS;
+    dim3 grid(unsigned(nrr * tiles), unsigned(nunits));
+    hipLaunchKernelGGL(k_volterra_mfma_long_f64, grid, dim3(256), lds, st, Z, zstride, np, N, S, units, gamma,
+                       Wt);
+    return hipGetLastError();
+}
+
 hipError_t launch_volterra_mfma_f64(const double* Z, long zstride, long np, int N, int nunits,
                                     const Unit* units, const double* gamma, double* Wt, hipStream_t st) {
     const size_t lds = volterra_mfma_lds_bytes(N);
