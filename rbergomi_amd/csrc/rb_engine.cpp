@@ -327,6 +327,7 @@ Plan::~Plan() {
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (graph) (void)hipGraphDestroy(graph);
     fused_batch.reset();  // stream-ordered frees on this plan's stream
+    fused_fft.reset();
     if (stream) (void)hipStreamSynchronize(stream);
     void* bufs[] = {d_units, d_gamma, d_groups, d_comp, d_opts,  d_Z,     d_Wt,   d_ints,
                     d_partial, d_price, d_stat, d_iv, d_payoff, d_tw, d_ghat, d_ints2,
@@ -388,6 +389,15 @@ int Plan::init(const Request& r, hipStream_t st, bool quadrature, std::string& e
     if (r.scheme == SCHEME_CHOLESKY && r.N > 2048) {
         err = "the Cholesky scheme covers N <= 2048";
         return ST_ERR_LIMIT;
+    }
+    if (split && r.N > 1024 && r.N <= 8192 && !quadrature && r.gauss == 0 && r.method == METHOD_RT) {
+        // long paths, Romano-Touzi, device Gaussians: the single-launch FFT kernel
+        fused_fft.reset(new FftFused());
+        const int rc = fused_fft->build(r, lay, device, stream, err);
+        if (rc) return rc;
+        nblk = fused_fft->args.nitems;
+        chunk = r.M;
+        return ST_OK;
     }
     if (r.precision == PREC_FP32 || (split && (r.N > 1024 || quadrature || r.gauss == 1))) {
         // staged pipeline with the FP32 FFT convolution (long paths, user Gaussians)
@@ -496,6 +506,7 @@ void Plan::begin_timing() {
     ev_used = 0;
     times.assign(K_COUNT, KernelTime());
     for (int i = 0; i < K_COUNT; ++i) times[i].name = kernel_name(i);
+    if (fused_fft) times[K_FUSED].name = "k_fused_fft";
     // the convolution slot holds whichever kernel computes Wtilde
     if (req.scheme == SCHEME_CHOLESKY)
         times[K_VOLTERRA].name = "k_volterra_chol_f64";
@@ -549,6 +560,7 @@ void Plan::end_timing() {
 // Enqueue one pricing run on the plan's stream (no synchronization).
 int Plan::enqueue(std::string& err) {
     if (fused) return fused_batch->launch(stream, err, capturing ? nullptr : this);
+    if (fused_fft) return fused_fft->launch(stream, err, capturing ? nullptr : this);
     const int N = req.N;
     const int nopt = int(lay.options.size());
     const PhiloxKey key = philox_key(req.seed.data(), int(req.seed.size()));
@@ -653,7 +665,7 @@ int Plan::run_samples(const double* Z_host, double* payoff_host, std::string& er
 }
 
 // FFT in FP64 (iterative radix-2, sign -1 forward) for the host-side table FFT(Gamma)
-static void fft_host(std::vector<std::complex<double>>& a, int sign) {
+void fft_host(std::vector<std::complex<double>>& a, int sign) {
     const int n = int(a.size());
     for (int i = 1, j = 0; i < n; ++i) {
         int bit = n >> 1;
@@ -731,8 +743,9 @@ int Plan::turbo_results(double* alpha, double* Q, long* exceptions, std::string&
 }
 
 int Plan::results(double* price, double* iv, double* stat, std::string& err) {
-    if (fused) {
-        const int rc = fused_batch->results(0, price, iv, stat, stream, err);
+    if (fused || fused_fft) {
+        const int rc = fused ? fused_batch->results(0, price, iv, stat, stream, err)
+                             : fused_fft->results(price, iv, stat, stream, err);
         if (rc) return rc;
         RB_TRY(hipStreamSynchronize(stream));
         return ST_OK;

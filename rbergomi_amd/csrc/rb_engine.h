@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <complex>
 #include <memory>
 #include <string>
 #include <vector>
@@ -84,6 +85,8 @@ constexpr long TURBO_PILOT_BASE = 1L << 62;
 int validate(const Request& r, std::string& err);
 Layout make_layout(const Request& r);
 PhiloxKey philox_key(const uint64_t* seed, int n);
+// FFT in FP64 on the host (radix-2, a.size() a power of two; sign -1 forward, unnormalised)
+void fft_host(std::vector<std::complex<double>>& a, int sign);
 
 // Per-kernel timing accumulated over the launches of one run (HIP events on the run's stream).
 struct KernelTime {
@@ -138,6 +141,8 @@ struct Plan {
     int fused_mode = 0;           // 0 = FP16 split, 1 = BF16 split
     std::unique_ptr<FusedHostReq> fused_req;
     std::unique_ptr<FusedBatch> fused_batch;
+    // long-path mixed-precision path (single-launch FFT kernel, Romano-Touzi)
+    std::unique_ptr<FftFused> fused_fft;
     // graph replay: the first run is captured into a hipGraph, later runs relaunch it
     bool use_graph = false;
     bool capturing = false;

@@ -156,4 +156,25 @@ hipError_t launch_fused_mc8(int mode, int upw, const FusedBatchArgs& b, int grid
 inline int fused_mc_greg(int gmax) { return (gmax + 7) / 8 < (gmax + 5) / 6 ? 8 : 6; }
 hipError_t launch_zero_int(int* p, int n, hipStream_t st);
 
+// ---------------------------------------------------------------- long paths (N > 1024)
+// One request priced by the single-launch FFT kernel (rb_fused_fft.hip), Romano-Touzi.
+struct FftFusedArgs {
+    int N, log2L, nunits, out_size;
+    int pgu;                     // most payoff groups of one unit (LDS records per path)
+    int pairs_per_item;          // path pairs of one work item
+    long M, npairs, nitems;
+    PhiloxKey key;
+    const FusedUnit* units;      // b, sc, p_off and the group / payoff-group / option ranges
+    const FusedGroup* groups;
+    const FusedPG* pgs;
+    const FusedOption* opts;
+    const float* ptab;           // per unit: P_j = (j - off)^(2H)
+    const float2* ghat;          // per unit [L]: FFT of (a, Gamma_1, ..., Gamma_{N-1}, 0, ...)
+    const float2* tw;            // [L/2]: exp(-2 pi i k / L)
+    double* partial;             // [out_size][nitems][2]
+    int* counter;                // next work item (zeroed before the launch)
+};
+size_t fused_fft_lds_bytes(const FftFusedArgs& a);
+hipError_t launch_fused_fft(const FftFusedArgs& a, int grid, hipStream_t st);
+
 }  // namespace rb

@@ -1,0 +1,278 @@
+// rb_fused_fft.hip -- single-launch mixed-precision Romano-Touzi pricing for long paths
+// (N > 1024, where the Toeplitz GEMM's W1 tile no longer fits LDS).
+//
+// A persistent workgroup takes work items (pairs_per_item pairs of paths) from an atomic
+// counter and runs the whole path of the reference for each pair without touching HBM:
+//   1. Philox4x32-10 W1 of both paths (FP32 Box-Muller, stream 0), kept in registers: thread t
+//      owns the 4-step quads q = t + 256 j of both paths (genGaussianMT,
+//      rBergLib/rBergomiMT.cpp:169-172).
+//   2. Per unit (H): z = W1_a + i W1_b zero-padded to L = 2^k >= 2N - 1 in LDS, radix-2 FFT,
+//      times FFT(a, Gamma_1, ..., Gamma_{N-1}) (the first-cell weight a folded in), inverse FFT:
+//      Gamma is real, so the real and imaginary parts are the two paths' convolutions
+//      (updateWtilde's FFTW convolution, :83-109).
+//   3. Wtilde = sc (Y + b W1perp) for the thread's steps, W1perp regenerated from the same
+//      counters (stream 1), written back to LDS in step order; then per (H, eta, T) group
+//      sqrt(v_i) = exp2(c1 Wtilde_{i-1} - al P_i - be) and the sums of v and sqrt(v) W1 over
+//      the thread's steps in FP32, over the workgroup in FP64 (scaleWtilde/updateV/intVdt/
+//      intRootVdW, :73-80, 163-167, 427-436).
+//   4. Per (path, group, rho) ln S', sigma; per option the conditioned Black-Scholes payoff in
+//      FP32 with the FP64 tail (payoff_mixed), summed in FP64 per option over the item's paths.
+// Outputs the same per-item option sums as the GEMM kernel, reduced by k_reduce_rows.
+#include <hip/hip_runtime.h>
+
+#include "rb_fused.h"
+#include "rb_kernels.h"
+
+namespace rb {
+
+constexpr int FFT_GR = 8;  // groups integrated per pass
+
+__device__ __forceinline__ float2 cmulf(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+__device__ __forceinline__ double wave_sum64(double v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+size_t fused_fft_lds_bytes(const FftFusedArgs& a) {
+    const size_t L = size_t(1) << a.log2L;
+    return L * sizeof(float2) + size_t(4) * FFT_GR * 2 * 2 * sizeof(double) + size_t(2) * a.pgu * sizeof(PathPG) +
+           size_t(a.out_size) * 2 * sizeof(double);
+}
+
+// QPT: quads per thread and path, ceil(ceil(N/4) / 256)
+template <int QPT>
+__global__ void __launch_bounds__(256) k_fused_fft(FftFusedArgs A) {
+    extern __shared__ float2 lds2[];
+    const int L = 1 << A.log2L;
+    const int shift = 32 - A.log2L;
+    float2* z = lds2;                                                      // [L]
+    double* red = reinterpret_cast<double*>(z + L);                        // [4][FFT_GR][2][2]
+    PathPG* pgd = reinterpret_cast<PathPG*>(red + 4 * FFT_GR * 2 * 2);     // [2][pgu]
+    double* acc = reinterpret_cast<double*>(pgd + 2 * A.pgu);              // [out][2]
+    __shared__ int s_item;
+    __shared__ float s_w10[2];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int N = A.N, nq = (N + 3) >> 2;
+    for (;;) {
+        if (tid == 0) s_item = atomicAdd(A.counter, 1);
+        __syncthreads();
+        const long item = s_item;
+        __syncthreads();
+        if (item >= A.nitems) break;
+        for (int o = tid; o < 2 * A.out_size; o += 256) acc[o] = 0.0;
+        const long pr0 = item * A.pairs_per_item;
+        const long pr1 = pr0 + A.pairs_per_item < A.npairs ? pr0 + A.pairs_per_item : A.npairs;
+        for (long pr = pr0; pr < pr1; ++pr) {
+            const long pth[2] = {2 * pr, 2 * pr + 1};
+            // ---- 1. W1 of both paths for this thread's quads (zero past N and past M)
+            float w1[2][QPT][4];
+#pragma unroll
+            for (int c = 0; c < 2; ++c)
+#pragma unroll
+                for (int j = 0; j < QPT; ++j) {
+                    const int q = tid + 256 * j;
+                    float zz[4] = {0.f, 0.f, 0.f, 0.f};
+                    if (q < nq && pth[c] < A.M) normals4_f32(A.key, uint64_t(pth[c]), 0u, uint32_t(q), zz);
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) w1[c][j][t] = 4 * q + t < N ? zz[t] : 0.f;
+                }
+            if (tid == 0) {
+                s_w10[0] = w1[0][0][0];
+                s_w10[1] = w1[1][0][0];
+            }
+            for (int u = 0; u < A.nunits; ++u) {
+                const FusedUnit U = A.units[u];
+                // ---- 2. z in bit-reversed order, transform, product, inverse transform
+#pragma unroll
+                for (int j = 0; j < QPT; ++j) {
+                    const int q = tid + 256 * j;
+                    if (q < nq)
+#pragma unroll
+                        for (int t = 0; t < 4; ++t)
+                            z[__brev(unsigned(4 * q + t)) >> shift] = make_float2(w1[0][j][t], w1[1][j][t]);
+                }
+                for (int i = 4 * nq + tid; i < L; i += 256) z[__brev(unsigned(i)) >> shift] = make_float2(0.f, 0.f);
+                __syncthreads();
+                for (int s = 0; s < A.log2L; ++s) {
+                    const int span = 1 << s, tstride = L >> (s + 1);
+                    for (int b = tid; b < (L >> 1); b += 256) {
+                        const int pos = b & (span - 1);
+                        const int i = ((b >> s) << (s + 1)) + pos;
+                        const float2 t = cmulf(A.tw[pos * tstride], z[i + span]);
+                        const float2 x = z[i];
+                        z[i] = make_float2(x.x + t.x, x.y + t.y);
+                        z[i + span] = make_float2(x.x - t.x, x.y - t.y);
+                    }
+                    __syncthreads();
+                }
+                const float2* G = A.ghat + long(u) * L;
+                for (int k = tid; k < L; k += 256) z[k] = cmulf(z[k], G[k]);
+                __syncthreads();
+                for (int s = A.log2L - 1; s >= 0; --s) {
+                    const int span = 1 << s, tstride = L >> (s + 1);
+                    for (int b = tid; b < (L >> 1); b += 256) {
+                        const int pos = b & (span - 1);
+                        const int i = ((b >> s) << (s + 1)) + pos;
+                        const float2 w0 = A.tw[pos * tstride];
+                        const float2 x = z[i], y = z[i + span];
+                        z[i] = make_float2(x.x + y.x, x.y + y.y);
+                        z[i + span] = cmulf(make_float2(x.x - y.x, x.y - y.y), make_float2(w0.x, -w0.y));
+                    }
+                    __syncthreads();
+                }
+                // ---- 3. Wtilde of the thread's steps (both paths), then back to LDS in step order
+                const float invL = 1.0f / float(L);
+                float wt[2][QPT][4];
+#pragma unroll
+                for (int j = 0; j < QPT; ++j) {
+                    const int q = tid + 256 * j;
+                    float zp[2][4] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
+#pragma unroll
+                    for (int c = 0; c < 2; ++c)
+                        if (q < nq && pth[c] < A.M) normals4_f32(A.key, uint64_t(pth[c]), 1u, uint32_t(q), zp[c]);
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) {
+                        const int i = 4 * q + t;
+                        const float2 y = i < N ? z[__brev(unsigned(i)) >> shift] : make_float2(0.f, 0.f);
+                        wt[0][j][t] = U.sc * fmaf(y.x, invL, U.b * zp[0][t]);
+                        wt[1][j][t] = U.sc * fmaf(y.y, invL, U.b * zp[1][t]);
+                    }
+                }
+                __syncthreads();
+#pragma unroll
+                for (int j = 0; j < QPT; ++j) {
+                    const int q = tid + 256 * j;
+                    if (q < nq)
+#pragma unroll
+                        for (int t = 0; t < 4; ++t)
+                            if (4 * q + t < N) z[4 * q + t] = make_float2(wt[0][j][t], wt[1][j][t]);
+                }
+                __syncthreads();
+                // ---- 4. integrals of the unit's groups, FFT_GR groups per pass
+                const float* P = A.ptab + U.p_off;
+                for (int g0 = U.g_begin; g0 < U.g_end; g0 += FFT_GR) {
+                    const int gn = U.g_end - g0 < FFT_GR ? U.g_end - g0 : FFT_GR;
+                    float c1[FFT_GR], al[FFT_GR], be[FFT_GR];
+                    float fv[FFT_GR][2], fs[FFT_GR][2];
+#pragma unroll
+                    for (int gg = 0; gg < FFT_GR; ++gg) {
+                        c1[gg] = al[gg] = be[gg] = 0.f;
+                        fv[gg][0] = fv[gg][1] = fs[gg][0] = fs[gg][1] = 0.f;
+                        if (gg < gn) {
+                            c1[gg] = A.groups[g0 + gg].c1;
+                            al[gg] = A.groups[g0 + gg].al;
+                            be[gg] = A.groups[g0 + gg].be;
+                        }
+                    }
+#pragma unroll
+                    for (int j = 0; j < QPT; ++j) {
+                        const int q = tid + 256 * j;
+#pragma unroll
+                        for (int t = 0; t < 4; ++t) {
+                            const int i = 4 * q + t;  // v_i uses Wtilde_{i-1}; i = 0 is v_0 = xi
+                            if (i >= 1 && i < N) {
+                                const float2 x = z[i - 1];
+                                const float pj = P[i];
+#pragma unroll
+                                for (int gg = 0; gg < FFT_GR; ++gg) {
+                                    if (gg < gn) {
+                                        const float e = fmaf(al[gg], pj, be[gg]);
+                                        const float ra = __builtin_amdgcn_exp2f(fmaf(x.x, c1[gg], -e));
+                                        const float rb = __builtin_amdgcn_exp2f(fmaf(x.y, c1[gg], -e));
+                                        fv[gg][0] = fmaf(ra, ra, fv[gg][0]);
+                                        fs[gg][0] = fmaf(ra, w1[0][j][t], fs[gg][0]);
+                                        fv[gg][1] = fmaf(rb, rb, fv[gg][1]);
+                                        fs[gg][1] = fmaf(rb, w1[1][j][t], fs[gg][1]);
+                                    }
+                                }
+                            }
+                        }
+                    }
+                    // FP64 over the workgroup: wave sums, then the four waves in order
+#pragma unroll
+                    for (int gg = 0; gg < FFT_GR; ++gg) {
+                        if (gg < gn) {
+#pragma unroll
+                            for (int c = 0; c < 2; ++c) {
+                                const double sv = wave_sum64(double(fv[gg][c]));
+                                const double ss = wave_sum64(double(fs[gg][c]));
+                                if (lane == 0) {
+                                    red[((wave * FFT_GR + gg) * 2 + c) * 2 + 0] = sv;
+                                    red[((wave * FFT_GR + gg) * 2 + c) * 2 + 1] = ss;
+                                }
+                            }
+                        }
+                    }
+                    __syncthreads();
+                    if (tid < 2 * gn) {
+#pragma clang fp contract(off)
+                        const int gg = tid >> 1, c = tid & 1;
+                        const FusedGroup Gr = A.groups[g0 + gg];
+                        double a = Gr.xi, b = Gr.sxi * double(s_w10[c]);  // step 0: v_0 = xi
+                        for (int w = 0; w < 4; ++w) {
+                            a += red[((w * FFT_GR + gg) * 2 + c) * 2 + 0];
+                            b += red[((w * FFT_GR + gg) * 2 + c) * 2 + 1];
+                        }
+                        const double Iv = Gr.dt * a, Is = Gr.sdt * b;
+                        for (int pg = Gr.pg_begin; pg < Gr.pg_end; ++pg) {
+                            const double rho = A.pgs[pg].rho;
+                            PathPG rec;
+                            rec.sig = sqrt((1.0 - rho * rho) * Iv);
+                            rec.lnS = -0.5 * rho * rho * Iv + rho * Is;
+                            rec.S = float(exp(rec.lnS));
+                            rec.invs = float(1.0 / rec.sig);
+                            rec.lnSf = float(rec.lnS);
+                            rec.sigf = float(rec.sig);
+                            pgd[c * A.pgu + (pg - U.pg_begin)] = rec;
+                        }
+                    }
+                    __syncthreads();
+                }
+                // ---- 5. payoffs of the unit's options for both paths (one thread per option)
+                for (int o = U.o_begin + tid; o < U.o_end; o += 256) {
+                    const FusedOption O = A.opts[o];
+                    const float lnKf = float(O.lnK), Kf = float(O.K);
+                    double s1 = 0.0, s2 = 0.0;
+                    for (int c = 0; c < 2; ++c) {
+                        if (pth[c] >= A.M) continue;
+                        const double x = payoff_mixed(pgd[c * A.pgu + (O.pg - U.pg_begin)], lnKf, Kf, O.lnK, O.K);
+                        s1 += x;
+                        s2 += x * x;
+                    }
+                    acc[2 * o + 0] += s1;
+                    acc[2 * o + 1] += s2;
+                }
+                __syncthreads();
+            }
+        }
+        for (int o = tid; o < A.out_size; o += 256) {
+            const int I = A.opts[o].out;
+            A.partial[(long(I) * A.nitems + item) * 2 + 0] = acc[2 * o + 0];
+            A.partial[(long(I) * A.nitems + item) * 2 + 1] = acc[2 * o + 1];
+        }
+        __syncthreads();
+    }
+}
+
+hipError_t launch_fused_fft(const FftFusedArgs& a, int grid, hipStream_t st) {
+    const size_t lds = fused_fft_lds_bytes(a);
+    const int qpt = ((a.N + 3) / 4 + 255) / 256;
+    auto go = [&](auto kern) -> hipError_t {
+        if (lds > 65536) {
+            const hipError_t e = allow_lds(reinterpret_cast<const void*>(kern), lds);
+            if (e != hipSuccess) return e;
+        }
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(256), lds, st, a);
+        return hipGetLastError();
+    };
+    if (qpt <= 2) return go(&k_fused_fft<2>);
+    if (qpt <= 4) return go(&k_fused_fft<4>);
+    if (qpt <= 8) return go(&k_fused_fft<8>);
+    return hipErrorInvalidValue;
+}
+
+}  // namespace rb

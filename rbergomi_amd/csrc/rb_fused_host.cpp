@@ -138,6 +138,75 @@ static void split16(double x, int mode, uint16_t& hi, uint16_t& lo) {
     }
 }
 
+// Units, groups (sorted by unit), payoff groups (group, rho) and options (sorted by unit) of a
+// request for the fused kernels, with the per-unit tables P_j = (j - off)^(2H) of Np entries.
+// pgmax: payoff groups held at once; gmax: most groups of one unit.
+void fused_tables(const Request& req, const Layout& lay, int Np, int frag_stride, FusedHostReq& out,
+                  int& pgmax, int& gmax) {
+    const int nu = int(lay.units.size());
+    out.fu.assign(nu, FusedUnit());
+    out.fg.clear();
+    out.fpg.clear();
+    out.fo.clear();
+    out.ptab.assign(size_t(nu) * Np, 0.f);
+    const double log2e = 1.4426950408889634;
+    const double off = req.comp == COMP_EXACT ? 0.0 : 1.0;
+    pgmax = 1;
+    gmax = 1;
+    for (int u = 0; u < nu; ++u) {
+        FusedUnit& U = out.fu[u];
+        const double H = lay.units[u].H;
+        U.b = float(lay.units[u].b);
+        U.sc = float(lay.units[u].sc);
+        U.frag_off = u * frag_stride;
+        U.p_off = u * Np;
+        for (int j = 1; j < req.N; ++j) out.ptab[size_t(u) * Np + j] = float(std::pow(double(j) - off, 2.0 * H));
+        U.g_begin = int(out.fg.size());
+        U.pg_begin = int(out.fpg.size());
+        U.o_begin = int(out.fo.size());
+        for (size_t g = 0; g < lay.groups.size(); ++g) {
+            const Group& G = lay.groups[g];
+            if (G.unit != u) continue;
+            FusedGroup F;
+            F.c1 = float(0.5 * G.eta * G.TH * log2e);
+            F.al = float(0.25 * G.eta * G.eta * std::pow(G.dt, 2.0 * H) * log2e);
+            F.be = float(-0.5 * std::log(G.xi) * log2e);
+            F.xi = G.xi;
+            F.sxi = std::sqrt(G.xi);
+            F.dt = G.dt;
+            F.sdt = G.sdt;
+            F.pg_begin = int(out.fpg.size());
+            F.o_begin = int(out.fo.size());
+            const int fgi = int(out.fg.size());
+            for (size_t I = 0; I < lay.options.size(); ++I) {
+                const Option& o = lay.options[I];
+                if (o.group != int(g)) continue;
+                int pg = -1;
+                for (int q = F.pg_begin; q < int(out.fpg.size()); ++q)
+                    if (out.fpg[q].rho == o.rho) pg = q;
+                if (pg < 0) {
+                    pg = int(out.fpg.size());
+                    out.fpg.push_back(FusedPG{fgi, 0, o.rho});
+                }
+                out.fo.push_back(FusedOption{pg, int(I), o.K, o.lnK});
+            }
+            F.pg_end = int(out.fpg.size());
+            F.o_end = int(out.fo.size());
+            out.fg.push_back(F);
+        }
+        U.g_end = int(out.fg.size());
+        U.pg_end = int(out.fpg.size());
+        U.o_end = int(out.fo.size());
+        // payoff groups held at once: those of one pass over FUSED_GREG groups (Romano-Touzi);
+        // plain Monte Carlo passes over 6 or 8 groups, so every window of FUSED_GREG groups
+        for (int g0 = U.g_begin; g0 < U.g_end; g0 += req.method == METHOD_MC ? 1 : FUSED_GREG) {
+            const int g1 = std::min(U.g_end, g0 + FUSED_GREG) - 1;
+            pgmax = std::max(pgmax, out.fg[g1].pg_end - out.fg[g0].pg_begin);
+        }
+        gmax = std::max(gmax, U.g_end - U.g_begin);
+    }
+}
+
 int plan_fused_request(const Request& req, const Layout& lay, int mode, FusedHostReq& out,
                        std::string& err, int pb) {
     const int N = req.N;
@@ -186,68 +255,9 @@ int plan_fused_request(const Request& req, const Layout& lay, int mode, FusedHos
         a.wave_nu[w] = int(wrb[w].size());
         for (int s = 0; s < FUSED_MAX_UPW; ++s) a.wave_rb[w][s] = s < int(wrb[w].size()) ? wrb[w][s] : -1;
     }
-    // ---- units, groups (sorted by unit), payoff groups (group, rho), options (sorted by unit)
-    const int nu = int(lay.units.size());
-    out.fu.assign(nu, FusedUnit());
-    out.fg.clear();
-    out.fpg.clear();
-    out.fo.clear();
-    out.ptab.assign(size_t(nu) * Np, 0.f);
-    const double log2e = 1.4426950408889634;
-    const double off = req.comp == COMP_EXACT ? 0.0 : 1.0;
     int pgmax = 1, gmax = 1;
-    for (int u = 0; u < nu; ++u) {
-        FusedUnit& U = out.fu[u];
-        const double H = lay.units[u].H;
-        U.b = float(lay.units[u].b);
-        U.sc = float(lay.units[u].sc);
-        U.frag_off = u * a.NK * 128;
-        U.p_off = u * Np;
-        for (int j = 1; j < N; ++j) out.ptab[size_t(u) * Np + j] = float(std::pow(double(j) - off, 2.0 * H));
-        U.g_begin = int(out.fg.size());
-        U.pg_begin = int(out.fpg.size());
-        U.o_begin = int(out.fo.size());
-        for (size_t g = 0; g < lay.groups.size(); ++g) {
-            const Group& G = lay.groups[g];
-            if (G.unit != u) continue;
-            FusedGroup F;
-            F.c1 = float(0.5 * G.eta * G.TH * log2e);
-            F.al = float(0.25 * G.eta * G.eta * std::pow(G.dt, 2.0 * H) * log2e);
-            F.be = float(-0.5 * std::log(G.xi) * log2e);
-            F.xi = G.xi;
-            F.sxi = std::sqrt(G.xi);
-            F.dt = G.dt;
-            F.sdt = G.sdt;
-            F.pg_begin = int(out.fpg.size());
-            F.o_begin = int(out.fo.size());
-            const int fgi = int(out.fg.size());
-            for (size_t I = 0; I < lay.options.size(); ++I) {
-                const Option& o = lay.options[I];
-                if (o.group != int(g)) continue;
-                int pg = -1;
-                for (int q = F.pg_begin; q < int(out.fpg.size()); ++q)
-                    if (out.fpg[q].rho == o.rho) pg = q;
-                if (pg < 0) {
-                    pg = int(out.fpg.size());
-                    out.fpg.push_back(FusedPG{fgi, 0, o.rho});
-                }
-                out.fo.push_back(FusedOption{pg, int(I), o.K, o.lnK});
-            }
-            F.pg_end = int(out.fpg.size());
-            F.o_end = int(out.fo.size());
-            out.fg.push_back(F);
-        }
-        U.g_end = int(out.fg.size());
-        U.pg_end = int(out.fpg.size());
-        U.o_end = int(out.fo.size());
-        // payoff groups held at once: those of one pass over FUSED_GREG groups (Romano-Touzi);
-        // plain Monte Carlo passes over 6 or 8 groups, so every window of FUSED_GREG groups
-        for (int g0 = U.g_begin; g0 < U.g_end; g0 += req.method == METHOD_MC ? 1 : FUSED_GREG) {
-            const int g1 = std::min(U.g_end, g0 + FUSED_GREG) - 1;
-            pgmax = std::max(pgmax, out.fg[g1].pg_end - out.fg[g0].pg_begin);
-        }
-        gmax = std::max(gmax, U.g_end - U.g_begin);
-    }
+    fused_tables(req, lay, Np, a.NK * 128, out, pgmax, gmax);
+    const int nu = int(lay.units.size());
     // ---- A fragments: entry (d, lane, slot j) = G[d + m - 8h - j], G[0] = a, G[t>0] = Gamma_t
     out.frag.assign(size_t(nu) * a.NK * 128, uint4());
     for (int u = 0; u < nu; ++u) {
@@ -519,6 +529,141 @@ int FusedBatch::results(int q, double* price, double* iv, double* stat, hipStrea
     if (price) RB_TRY(hipMemcpyAsync(price, d_price + out_off[q], n, hipMemcpyDeviceToHost, st));
     if (iv) RB_TRY(hipMemcpyAsync(iv, d_iv + out_off[q], n, hipMemcpyDeviceToHost, st));
     if (stat) RB_TRY(hipMemcpyAsync(stat, d_stat + out_off[q], n, hipMemcpyDeviceToHost, st));
+    return ST_OK;
+}
+
+// ---------------------------------------------------------------- long paths: FFT kernel
+FftFused::~FftFused() { release(); }
+
+void FftFused::release() {
+    void* bufs[] = {d_fu, d_fg, d_fpg, d_fo, d_ptab, d_ghat, d_tw, d_partial, d_counter, d_row_off, d_row_n,
+                    d_row_M, d_price, d_stat, d_iv, d_ivopts};
+    for (void* b : bufs)
+        if (b) (void)hipFreeAsync(b, stream);
+    d_fu = nullptr;
+    d_fg = nullptr;
+    d_fpg = nullptr;
+    d_fo = nullptr;
+    d_ptab = nullptr;
+    d_ghat = d_tw = nullptr;
+    d_partial = nullptr;
+    d_counter = nullptr;
+    d_row_off = d_row_n = d_row_M = nullptr;
+    d_price = d_stat = d_iv = nullptr;
+    d_ivopts = nullptr;
+}
+
+int FftFused::build(const Request& req, const Layout& lay, int device, hipStream_t st, std::string& err) {
+    release();
+    stream = st;
+    const int N = req.N;
+    int log2L = 1;
+    while ((1 << log2L) < 2 * N - 1) ++log2L;
+    const int L = 1 << log2L;
+    if (((N + 3) / 4 + 255) / 256 > 8) {
+        err = "the long-path kernel covers N <= 8192";
+        return ST_ERR_LIMIT;
+    }
+    const int Np = (N + 31) / 32 * 32;
+    FusedHostReq tabs;
+    int pgmax = 1, gmax = 1;
+    fused_tables(req, lay, Np, 0, tabs, pgmax, gmax);
+    int pgu = 1;
+    for (const FusedUnit& U : tabs.fu) pgu = std::max(pgu, U.pg_end - U.pg_begin);
+    // twiddles and FFT(a, Gamma_1, ..., Gamma_{N-1}) per unit, in FP64, stored in FP32
+    std::vector<float2> tw(L / 2);
+    for (int k = 0; k < L / 2; ++k) {
+        const double ang = -2.0 * M_PI * double(k) / double(L);
+        tw[k] = make_float2(float(std::cos(ang)), float(std::sin(ang)));
+    }
+    std::vector<float2> gh(lay.units.size() * size_t(L));
+    for (size_t u = 0; u < lay.units.size(); ++u) {
+        std::vector<std::complex<double>> g(L, 0.0);
+        g[0] = lay.units[u].a;
+        for (int k = 1; k < N; ++k) g[k] = lay.gamma[u * N + k];
+        fft_host(g, -1);
+        for (int k = 0; k < L; ++k) gh[u * L + k] = make_float2(float(g[k].real()), float(g[k].imag()));
+    }
+    FftFusedArgs& a = args;
+    a = FftFusedArgs{};
+    a.N = N;
+    a.log2L = log2L;
+    a.nunits = int(lay.units.size());
+    a.out_size = int(lay.options.size());
+    a.pgu = pgu;
+    a.pairs_per_item = 32;
+    a.M = req.M;
+    a.npairs = (req.M + 1) / 2;
+    a.nitems = (a.npairs + a.pairs_per_item - 1) / a.pairs_per_item;
+    a.key = philox_key(req.seed.data(), int(req.seed.size()));
+    total_out = a.out_size;
+    hipDeviceProp_t prop;
+    RB_TRY(device_props(device, prop));
+    keep_pool_cached(device);
+    const size_t lds = fused_fft_lds_bytes(a);
+    if (lds > prop.sharedMemPerBlock) {
+        err = "long-path kernel needs " + std::to_string(lds) + " bytes of LDS (N or the option count too large)";
+        return ST_ERR_LIMIT;
+    }
+    std::vector<long> row_off(a.out_size), row_n(a.out_size, a.nitems), row_M(a.out_size, a.M);
+    for (int I = 0; I < a.out_size; ++I) row_off[I] = long(I) * a.nitems * 2;
+    RB_TRY(upload_vec(&d_fu, tabs.fu, st));
+    RB_TRY(upload_vec(&d_fg, tabs.fg, st));
+    RB_TRY(upload_vec(&d_fpg, tabs.fpg, st));
+    RB_TRY(upload_vec(&d_fo, tabs.fo, st));
+    RB_TRY(upload_vec(&d_ptab, tabs.ptab, st));
+    RB_TRY(upload_vec(&d_ghat, gh, st));
+    RB_TRY(upload_vec(&d_tw, tw, st));
+    RB_TRY(upload_vec(&d_row_off, row_off, st));
+    RB_TRY(upload_vec(&d_row_n, row_n, st));
+    RB_TRY(upload_vec(&d_row_M, row_M, st));
+    RB_TRY(upload_vec(&d_ivopts, lay.options, st));
+    RB_TRY(alloc_async(&d_partial, size_t(a.out_size) * a.nitems * 2, st));
+    RB_TRY(alloc_async(&d_counter, 1, st));
+    RB_TRY(alloc_async(&d_price, size_t(a.out_size), st));
+    RB_TRY(alloc_async(&d_stat, size_t(a.out_size), st));
+    RB_TRY(alloc_async(&d_iv, size_t(a.out_size), st));
+    a.units = d_fu;
+    a.groups = d_fg;
+    a.pgs = d_fpg;
+    a.opts = d_fo;
+    a.ptab = d_ptab;
+    a.ghat = d_ghat;
+    a.tw = d_tw;
+    a.partial = d_partial;
+    a.counter = d_counter;
+    const size_t lds_cu = size_t(prop.maxSharedMemoryPerMultiProcessor ? prop.maxSharedMemoryPerMultiProcessor
+                                                                        : prop.sharedMemPerBlock);
+    const int per_cu = int(std::max<size_t>(1, std::min<size_t>(4, lds_cu / lds)));
+    grid = int(std::min<long>(a.nitems, long(prop.multiProcessorCount) * per_cu));
+    RB_TRY(hipStreamSynchronize(st));
+    return ST_OK;
+}
+
+int FftFused::launch(hipStream_t st, std::string& err, Plan* timing) {
+    auto timed = [&](int id) { return timing ? timing->timed(id) : -1; };
+    auto mark = [&](int slot) {
+        if (timing) timing->mark(slot, nullptr, nullptr);
+    };
+    int s;
+    RB_TRY(launch_zero_int(d_counter, 1, st));
+    s = timed(K_FUSED);
+    RB_TRY(launch_fused_fft(args, grid, st));
+    mark(s);
+    s = timed(K_REDUCE);
+    RB_TRY(launch_reduce_rows(d_partial, d_row_off, d_row_n, d_row_M, total_out, d_price, d_stat, st));
+    mark(s);
+    s = timed(K_IV);
+    RB_TRY(launch_iv_f64(d_price, d_ivopts, total_out, d_iv, st));
+    mark(s);
+    return ST_OK;
+}
+
+int FftFused::results(double* price, double* iv, double* stat, hipStream_t st, std::string& err) {
+    const size_t n = size_t(total_out) * sizeof(double);
+    if (price) RB_TRY(hipMemcpyAsync(price, d_price, n, hipMemcpyDeviceToHost, st));
+    if (iv) RB_TRY(hipMemcpyAsync(iv, d_iv, n, hipMemcpyDeviceToHost, st));
+    if (stat) RB_TRY(hipMemcpyAsync(stat, d_stat, n, hipMemcpyDeviceToHost, st));
     return ST_OK;
 }
 

@@ -34,6 +34,39 @@ struct FusedHostReq {
 int plan_fused_request(const Request& req, const Layout& lay, int mode, FusedHostReq& out,
                        std::string& err, int pb = 0);
 
+// Units, groups, payoff groups, options and P_j tables (Np entries per unit) of a request
+void fused_tables(const Request& req, const Layout& lay, int Np, int frag_stride, FusedHostReq& out,
+                  int& pgmax, int& gmax);
+
+// A long-path request (N > 1024, Romano-Touzi) on the single-launch FFT kernel.
+struct FftFused {
+    FftFusedArgs args{};
+    int grid = 0, total_out = 0;
+    hipStream_t stream = nullptr;
+    FusedUnit* d_fu = nullptr;
+    FusedGroup* d_fg = nullptr;
+    FusedPG* d_fpg = nullptr;
+    FusedOption* d_fo = nullptr;
+    float* d_ptab = nullptr;
+    float2* d_ghat = nullptr;
+    float2* d_tw = nullptr;
+    double* d_partial = nullptr;
+    int* d_counter = nullptr;
+    long* d_row_off = nullptr;
+    long* d_row_n = nullptr;
+    long* d_row_M = nullptr;
+    double* d_price = nullptr;
+    double* d_stat = nullptr;
+    double* d_iv = nullptr;
+    Option* d_ivopts = nullptr;
+
+    ~FftFused();
+    void release();
+    int build(const Request& req, const Layout& lay, int device, hipStream_t st, std::string& err);
+    int launch(hipStream_t st, std::string& err, Plan* timing = nullptr);
+    int results(double* price, double* iv, double* stat, hipStream_t st, std::string& err);
+};
+
 // Tile shape (path blocks per tile) per (padded N, operand mode, plain MC): an autotuned entry,
 // else the measured default rule.  The table is process-wide and thread-safe; the environment
 // variable RBERGOMI_TUNE_CACHE names a file (tune_save's format) loaded on first use.

@@ -50,3 +50,36 @@ def test_fft_prices_match_fp64(rb, N):
     err = np.max(np.abs(p32 - p64))
     print("N=%d max |price fp32-fft - fp64| = %.2e" % (N, err))
     assert err <= 1e-4
+
+
+@pytest.mark.parametrize("N,M", [(1100, 1999), (2048, 1500), (4096, 700), (8192, 301)])
+def test_fused_long_path_prices_match_fp64(rb, N, M):
+    # the mixed-precision precisions beyond N = 1024 (Romano-Touzi, device Gaussians) run the
+    # single-launch FFT kernel: same Philox counters as the FP64 path, FP32 transform and
+    # integrals, FP64 sums; two units (H), several maturities and correlations, odd M
+    H, eta, rho = [0.05, 0.2], [2.3, 1.2], [-0.9, -0.4]
+    T = [0.25, 1.0, 1.0, 2.0, 0.5]
+    K = [0.9, 0.8, 1.2, 1.0, 1.05]
+    print("fused long path N=%d M=%d" % (N, M), flush=True)
+    p64, iv64, s64 = rb.price_v2(0.03, H, eta, rho, T, K, N, M, [6, N], precision=rb.PREC_FP64)
+    for code in (rb.PREC_SPLIT_FP16, rb.PREC_FP32_GEMM):
+        pm, ivm, sm = rb.price_v2(0.03, H, eta, rho, T, K, N, M, [6, N], precision=code)
+        err = np.max(np.abs(pm - p64))
+        print("   precision %d: max |price - fp64| = %.2e, |stat - fp64| = %.2e" % (code, err, np.max(np.abs(sm - s64))),
+              flush=True)
+        assert err <= 1e-4 and np.max(np.abs(sm - s64)) <= 1e-4
+        assert np.all(np.isfinite(ivm) | ~np.isfinite(iv64))
+
+
+def test_fused_long_path_reruns_and_plan_timing(rb):
+    T, K = [1.0] * 5, list(np.linspace(0.8, 1.2, 5))
+    a = rb.price_v2(0.04, [0.1], [1.5], [-0.7], T, K, 2048, 3001, [2], precision=rb.PREC_SPLIT_FP16)
+    b = rb.price_v2(0.04, [0.1], [1.5], [-0.7], T, K, 2048, 3001, [2], precision=rb.PREC_SPLIT_FP16)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y, equal_nan=True)
+    pl = rb.Plan(0.04, [0.1], [1.5], [-0.7], T, K, 2048, 3001, [2], precision=rb.PREC_SPLIT_FP16, profile=True)
+    pl.run()
+    times = pl.kernel_times()
+    p, _, _ = pl.results()
+    pl.close()
+    assert "k_fused_fft" in times and np.array_equal(p, a[0])

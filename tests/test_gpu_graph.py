@@ -29,6 +29,7 @@ CASES = [
     ("fused_fp16", 2, 256, 1 << 15, 0, 0),
     ("fused_bf16", 3, 128, 20000, 0, 0),
     ("fused_fp32", 4, 512, 9000, 0, 0),
+    ("fused_fft_fp16", 2, 2048, 3001, 0, 0),
     ("fused_fp16_mc", 2, 96, 20000, 1, 0),
     ("staged_fp64", 0, 100, 1 << 14, 0, 0),
     ("staged_fp64_mc", 0, 64, 1 << 14, 1, 0),
