@@ -4,21 +4,18 @@
 // for N > 1024, where the Toeplitz GEMM's W1 tile no longer fits LDS.  One workgroup convolves
 // two paths at once: z = W1_a + i W1_b, zero-padded to L = 2^k >= 2N-1.  Because Gamma is real,
 // IFFT(FFT(z) * FFT(Gamma)) = (Gamma * W1_a) + i (Gamma * W1_b), so no separation step is
-// needed.  Forward transform: radix-2 decimation in time from bit-reversed input to natural
+// needed.  Forward transform: radix-4 decimation in time from bit-reversed input to natural
 // order; pointwise product with the per-H table FFT(Gamma) (computed in FP64 on the host);
-// inverse: radix-2 decimation in frequency from natural order to bit-reversed output, which the
+// inverse: radix-4 decimation in frequency from natural order to bit-reversed output, which the
 // epilogue reads back at bit-reversed indices.  Twiddles exp(-2 pi i k / L) come from an FP64
 // table rounded to FP32.  Epilogue in FP64: Wtilde = sc (Y + a W1 + b W1perp).
 #include <hip/hip_runtime.h>
 
+#include "rb_fft_lds.h"
 #include "rb_kernels.h"
 #include "rb_types.h"
 
 namespace rb {
-
-__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
-    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
 
 // Grid: (ceil(np/2), units).  Block 256.  LDS: L float2.
 __global__ void __launch_bounds__(256)
@@ -39,40 +36,12 @@ k_volterra_fft(const double* __restrict__ Z, long zstride, long np, int N, int l
         z[__brev(unsigned(i)) >> (32 - log2L)] = make_float2(re, im);
     }
     __syncthreads();
-    // forward DIT
-    for (int s = 0; s < log2L; ++s) {
-        const int span = 1 << s;
-        const int tstride = L >> (s + 1);  // twiddle index step
-        for (int b = threadIdx.x; b < (L >> 1); b += 256) {
-            const int pos = b & (span - 1);
-            const int i = ((b >> s) << (s + 1)) + pos;
-            const float2 w = tw[pos * tstride];
-            const float2 t = cmul(w, z[i + span]);
-            const float2 x = z[i];
-            z[i] = make_float2(x.x + t.x, x.y + t.y);
-            z[i + span] = make_float2(x.x - t.x, x.y - t.y);
-        }
-        __syncthreads();
-    }
+    fft_dit_r4(z, tw, log2L, threadIdx.x);
     // pointwise product with FFT(Gamma) of this unit
     const float2* G = ghat + long(u) * L;
-    for (int k = threadIdx.x; k < L; k += 256) z[k] = cmul(z[k], G[k]);
+    for (int k = threadIdx.x; k < L; k += 256) z[k] = cmulf(z[k], G[k]);
     __syncthreads();
-    // inverse DIF with conjugate twiddles
-    for (int s = log2L - 1; s >= 0; --s) {
-        const int span = 1 << s;
-        const int tstride = L >> (s + 1);
-        for (int b = threadIdx.x; b < (L >> 1); b += 256) {
-            const int pos = b & (span - 1);
-            const int i = ((b >> s) << (s + 1)) + pos;
-            const float2 w0 = tw[pos * tstride];
-            const float2 w = make_float2(w0.x, -w0.y);
-            const float2 x = z[i], y = z[i + span];
-            z[i] = make_float2(x.x + y.x, x.y + y.y);
-            z[i + span] = cmul(make_float2(x.x - y.x, x.y - y.y), w);
-        }
-        __syncthreads();
-    }
+    fft_dif_inv_r4(z, tw, log2L, threadIdx.x);
     const Unit un = units[u];
     const double invL = 1.0 / double(L);
     for (int i = threadIdx.x; i < N; i += 256) {

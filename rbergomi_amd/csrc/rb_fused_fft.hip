@@ -6,7 +6,7 @@
 //   1. Philox4x32-10 W1 of both paths (FP32 Box-Muller, stream 0), kept in registers: thread t
 //      owns the 4-step quads q = t + 256 j of both paths (genGaussianMT,
 //      rBergLib/rBergomiMT.cpp:169-172).
-//   2. Per unit (H): z = W1_a + i W1_b zero-padded to L = 2^k >= 2N - 1 in LDS, radix-2 FFT,
+//   2. Per unit (H): z = W1_a + i W1_b zero-padded to L = 2^k >= 2N - 1 in LDS, radix-4 FFT,
 //      times FFT(a, Gamma_1, ..., Gamma_{N-1}) (the first-cell weight a folded in), inverse FFT:
 //      Gamma is real, so the real and imaginary parts are the two paths' convolutions
 //      (updateWtilde's FFTW convolution, :83-109).
@@ -20,16 +20,13 @@
 // Outputs the same per-item option sums as the GEMM kernel, reduced by k_reduce_rows.
 #include <hip/hip_runtime.h>
 
+#include "rb_fft_lds.h"
 #include "rb_fused.h"
 #include "rb_kernels.h"
 
 namespace rb {
 
 constexpr int FFT_GR = 8;  // groups integrated per pass
-
-__device__ __forceinline__ float2 cmulf(float2 a, float2 b) {
-    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
 
 __device__ __forceinline__ double wave_sum64(double v) {
 #pragma unroll
@@ -97,33 +94,11 @@ __global__ void __launch_bounds__(256) k_fused_fft(FftFusedArgs A) {
                 }
                 for (int i = 4 * nq + tid; i < L; i += 256) z[__brev(unsigned(i)) >> shift] = make_float2(0.f, 0.f);
                 __syncthreads();
-                for (int s = 0; s < A.log2L; ++s) {
-                    const int span = 1 << s, tstride = L >> (s + 1);
-                    for (int b = tid; b < (L >> 1); b += 256) {
-                        const int pos = b & (span - 1);
-                        const int i = ((b >> s) << (s + 1)) + pos;
-                        const float2 t = cmulf(A.tw[pos * tstride], z[i + span]);
-                        const float2 x = z[i];
-                        z[i] = make_float2(x.x + t.x, x.y + t.y);
-                        z[i + span] = make_float2(x.x - t.x, x.y - t.y);
-                    }
-                    __syncthreads();
-                }
+                fft_dit_r4(z, A.tw, A.log2L, tid);
                 const float2* G = A.ghat + long(u) * L;
                 for (int k = tid; k < L; k += 256) z[k] = cmulf(z[k], G[k]);
                 __syncthreads();
-                for (int s = A.log2L - 1; s >= 0; --s) {
-                    const int span = 1 << s, tstride = L >> (s + 1);
-                    for (int b = tid; b < (L >> 1); b += 256) {
-                        const int pos = b & (span - 1);
-                        const int i = ((b >> s) << (s + 1)) + pos;
-                        const float2 w0 = A.tw[pos * tstride];
-                        const float2 x = z[i], y = z[i + span];
-                        z[i] = make_float2(x.x + y.x, x.y + y.y);
-                        z[i + span] = cmulf(make_float2(x.x - y.x, x.y - y.y), make_float2(w0.x, -w0.y));
-                    }
-                    __syncthreads();
-                }
+                fft_dif_inv_r4(z, A.tw, A.log2L, tid);
                 // ---- 3. Wtilde of the thread's steps (both paths), then back to LDS in step order
                 const float invL = 1.0f / float(L);
                 float wt[2][QPT][4];
