@@ -17,7 +17,7 @@
 
 namespace rb {
 
-// Grid: (ceil(np/2), units).  Block 256.  LDS: L float2.
+// Grid: (ceil(np/2), units).  Block 256.  LDS: L + L/32 float2 (element i in slot fpad(i)).
 __global__ void __launch_bounds__(256)
 k_volterra_fft(const double* __restrict__ Z, long zstride, long np, int N, int log2L,
                const Unit* __restrict__ units, const float2* __restrict__ ghat,
@@ -33,19 +33,19 @@ k_volterra_fft(const double* __restrict__ Z, long zstride, long np, int N, int l
     for (int i = threadIdx.x; i < L; i += 256) {
         const float re = i < N ? float(Wa[i]) : 0.f;
         const float im = (i < N && hasb) ? float(Wb[i]) : 0.f;
-        z[__brev(unsigned(i)) >> (32 - log2L)] = make_float2(re, im);
+        z[fpad(int(__brev(unsigned(i)) >> (32 - log2L)))] = make_float2(re, im);
     }
     __syncthreads();
     fft_dit_r4(z, tw, log2L, threadIdx.x);
     // pointwise product with FFT(Gamma) of this unit
     const float2* G = ghat + long(u) * L;
-    for (int k = threadIdx.x; k < L; k += 256) z[k] = cmulf(z[k], G[k]);
+    for (int k = threadIdx.x; k < L; k += 256) z[fpad(k)] = cmulf(z[fpad(k)], G[k]);
     __syncthreads();
     fft_dif_inv_r4(z, tw, log2L, threadIdx.x);
     const Unit un = units[u];
     const double invL = 1.0 / double(L);
     for (int i = threadIdx.x; i < N; i += 256) {
-        const float2 y = z[__brev(unsigned(i)) >> (32 - log2L)];
+        const float2 y = z[fpad(int(__brev(unsigned(i)) >> (32 - log2L)))];
         {
             const double Y = double(y.x) * invL;
             Wt[(long(u) * np + pa) * N + i] = un.sc * Y + un.sc * (un.a * Wa[i] + un.b * Wa[N + i]);
@@ -57,7 +57,7 @@ k_volterra_fft(const double* __restrict__ Z, long zstride, long np, int N, int l
     }
 }
 
-size_t volterra_fft_lds_bytes(int log2L) { return (size_t(1) << log2L) * sizeof(float2); }
+size_t volterra_fft_lds_bytes(int log2L) { return fft_lds_slots(log2L) * sizeof(float2); }
 
 hipError_t launch_volterra_fft(const double* Z, long zstride, long np, int N, int log2L, int nunits,
                                const Unit* units, const float2* ghat, const float2* tw, double* Wt,

@@ -36,18 +36,18 @@ __device__ __forceinline__ double wave_sum64(double v) {
 
 size_t fused_fft_lds_bytes(const FftFusedArgs& a) {
     const size_t L = size_t(1) << a.log2L;
-    return L * sizeof(float2) + size_t(4) * FFT_GR * 2 * 2 * sizeof(double) + size_t(2) * a.pgu * sizeof(PathPG) +
+    return fft_lds_slots(a.log2L) * sizeof(float2) + size_t(4) * FFT_GR * 2 * 2 * sizeof(double) + size_t(2) * a.pgu * sizeof(PathPG) +
            size_t(a.out_size) * 2 * sizeof(double);
 }
 
 // QPT: quads per thread and path, ceil(ceil(N/4) / 256)
 template <int QPT>
-__global__ void __launch_bounds__(256) k_fused_fft(FftFusedArgs A) {
+__global__ void __launch_bounds__(256, QPT >= 8 ? 1 : 2) k_fused_fft(FftFusedArgs A) {
     extern __shared__ float2 lds2[];
     const int L = 1 << A.log2L;
     const int shift = 32 - A.log2L;
-    float2* z = lds2;                                                      // [L]
-    double* red = reinterpret_cast<double*>(z + L);                        // [4][FFT_GR][2][2]
+    float2* z = lds2;                                                      // [L + L/32], slot fpad(i)
+    double* red = reinterpret_cast<double*>(z + L + (L >> 5));             // [4][FFT_GR][2][2]
     PathPG* pgd = reinterpret_cast<PathPG*>(red + 4 * FFT_GR * 2 * 2);     // [2][pgu]
     double* acc = reinterpret_cast<double*>(pgd + 2 * A.pgu);              // [out][2]
     __shared__ int s_item;
@@ -90,13 +90,13 @@ __global__ void __launch_bounds__(256) k_fused_fft(FftFusedArgs A) {
                     if (q < nq)
 #pragma unroll
                         for (int t = 0; t < 4; ++t)
-                            z[__brev(unsigned(4 * q + t)) >> shift] = make_float2(w1[0][j][t], w1[1][j][t]);
+                            z[fpad(int(__brev(unsigned(4 * q + t)) >> shift))] = make_float2(w1[0][j][t], w1[1][j][t]);
                 }
-                for (int i = 4 * nq + tid; i < L; i += 256) z[__brev(unsigned(i)) >> shift] = make_float2(0.f, 0.f);
+                for (int i = 4 * nq + tid; i < L; i += 256) z[fpad(int(__brev(unsigned(i)) >> shift))] = make_float2(0.f, 0.f);
                 __syncthreads();
                 fft_dit_r4(z, A.tw, A.log2L, tid);
                 const float2* G = A.ghat + long(u) * L;
-                for (int k = tid; k < L; k += 256) z[k] = cmulf(z[k], G[k]);
+                for (int k = tid; k < L; k += 256) z[fpad(k)] = cmulf(z[fpad(k)], G[k]);
                 __syncthreads();
                 fft_dif_inv_r4(z, A.tw, A.log2L, tid);
                 // ---- 3. Wtilde of the thread's steps (both paths), then back to LDS in step order
@@ -112,7 +112,7 @@ __global__ void __launch_bounds__(256) k_fused_fft(FftFusedArgs A) {
 #pragma unroll
                     for (int t = 0; t < 4; ++t) {
                         const int i = 4 * q + t;
-                        const float2 y = i < N ? z[__brev(unsigned(i)) >> shift] : make_float2(0.f, 0.f);
+                        const float2 y = i < N ? z[fpad(int(__brev(unsigned(i)) >> shift))] : make_float2(0.f, 0.f);
                         wt[0][j][t] = U.sc * fmaf(y.x, invL, U.b * zp[0][t]);
                         wt[1][j][t] = U.sc * fmaf(y.y, invL, U.b * zp[1][t]);
                     }
@@ -124,7 +124,7 @@ __global__ void __launch_bounds__(256) k_fused_fft(FftFusedArgs A) {
                     if (q < nq)
 #pragma unroll
                         for (int t = 0; t < 4; ++t)
-                            if (4 * q + t < N) z[4 * q + t] = make_float2(wt[0][j][t], wt[1][j][t]);
+                            if (4 * q + t < N) z[fpad(4 * q + t)] = make_float2(wt[0][j][t], wt[1][j][t]);
                 }
                 __syncthreads();
                 // ---- 4. integrals of the unit's groups, FFT_GR groups per pass
@@ -150,7 +150,7 @@ __global__ void __launch_bounds__(256) k_fused_fft(FftFusedArgs A) {
                         for (int t = 0; t < 4; ++t) {
                             const int i = 4 * q + t;  // v_i uses Wtilde_{i-1}; i = 0 is v_0 = xi
                             if (i >= 1 && i < N) {
-                                const float2 x = z[i - 1];
+                                const float2 x = z[fpad(i - 1)];
                                 const float pj = P[i];
 #pragma unroll
                                 for (int gg = 0; gg < FFT_GR; ++gg) {
