@@ -390,8 +390,8 @@ int Plan::init(const Request& r, hipStream_t st, bool quadrature, std::string& e
         err = "the Cholesky scheme covers N <= 2048";
         return ST_ERR_LIMIT;
     }
-    if (split && r.N > 1024 && r.N <= 8192 && !quadrature && r.gauss == 0 && r.method == METHOD_RT) {
-        // long paths, Romano-Touzi, device Gaussians: the single-launch FFT kernel
+    if (split && r.N > 1024 && r.N <= 8192 && !quadrature && r.gauss == 0) {
+        // long paths on device Gaussians: the single-launch FFT kernel
         fused_fft.reset(new FftFused());
         const int rc = fused_fft->build(r, lay, device, stream, err);
         if (rc) return rc;

@@ -157,9 +157,10 @@ inline int fused_mc_greg(int gmax) { return (gmax + 7) / 8 < (gmax + 5) / 6 ? 8 
 hipError_t launch_zero_int(int* p, int n, hipStream_t st);
 
 // ---------------------------------------------------------------- long paths (N > 1024)
-// One request priced by the single-launch FFT kernel (rb_fused_fft.hip), Romano-Touzi.
+// One request priced by the single-launch FFT kernel (rb_fused_fft.hip).
 struct FftFusedArgs {
     int N, log2L, nunits, out_size;
+    int mc;                      // 1 = plain Monte Carlo, 0 = Romano-Touzi
     int pgu;                     // most payoff groups of one unit (LDS records per path)
     int pairs_per_item;          // path pairs of one work item
     long M, npairs, nitems;

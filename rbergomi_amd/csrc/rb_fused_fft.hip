@@ -17,6 +17,9 @@
 //      intRootVdW, :73-80, 163-167, 427-436).
 //   4. Per (path, group, rho) ln S', sigma; per option the conditioned Black-Scholes payoff in
 //      FP32 with the FP64 tail (payoff_mixed), summed in FP64 per option over the item's paths.
+// Plain Monte Carlo (computeIV_tensor's estimator, MC = true): a third sum sqrt(v) Wperp with
+// Wperp from stream 2, ln S_T = rho Is + sqrt(1 - rho^2) Ip - Iv / 2 and the payoff (S_T - K)^+
+// in FP64 (updateZ/updateS, rBergLib/rBergomiMT.cpp:111-137).
 // Outputs the same per-item option sums as the GEMM kernel, reduced by k_reduce_rows.
 #include <hip/hip_runtime.h>
 
@@ -34,24 +37,25 @@ __device__ __forceinline__ double wave_sum64(double v) {
     return v;
 }
 
+constexpr int FFT_NS = 3;  // sums per (path, group) held in the reduction buffer
+
 size_t fused_fft_lds_bytes(const FftFusedArgs& a) {
-    const size_t L = size_t(1) << a.log2L;
-    return fft_lds_slots(a.log2L) * sizeof(float2) + size_t(4) * FFT_GR * 2 * 2 * sizeof(double) + size_t(2) * a.pgu * sizeof(PathPG) +
-           size_t(a.out_size) * 2 * sizeof(double);
+    return fft_lds_slots(a.log2L) * sizeof(float2) + size_t(4) * FFT_GR * 2 * FFT_NS * sizeof(double) +
+           size_t(2) * a.pgu * sizeof(PathPG) + size_t(a.out_size) * 2 * sizeof(double);
 }
 
 // QPT: quads per thread and path, ceil(ceil(N/4) / 256)
-template <int QPT>
-__global__ void __launch_bounds__(256, QPT >= 8 ? 1 : 2) k_fused_fft(FftFusedArgs A) {
+template <int QPT, bool MC>
+__global__ void __launch_bounds__(256, (QPT >= 8 || (MC && QPT >= 4)) ? 1 : 2) k_fused_fft(FftFusedArgs A) {
     extern __shared__ float2 lds2[];
     const int L = 1 << A.log2L;
     const int shift = 32 - A.log2L;
     float2* z = lds2;                                                      // [L + L/32], slot fpad(i)
-    double* red = reinterpret_cast<double*>(z + L + (L >> 5));             // [4][FFT_GR][2][2]
-    PathPG* pgd = reinterpret_cast<PathPG*>(red + 4 * FFT_GR * 2 * 2);     // [2][pgu]
+    double* red = reinterpret_cast<double*>(z + L + (L >> 5));             // [4][FFT_GR][2][FFT_NS]
+    PathPG* pgd = reinterpret_cast<PathPG*>(red + 4 * FFT_GR * 2 * FFT_NS);  // [2][pgu]
     double* acc = reinterpret_cast<double*>(pgd + 2 * A.pgu);              // [out][2]
     __shared__ int s_item;
-    __shared__ float s_w10[2];
+    __shared__ float s_w10[2], s_w20[2];  // W1, Wperp of both paths at step 0
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int N = A.N, nq = (N + 3) >> 2;
     for (;;) {
@@ -80,6 +84,13 @@ __global__ void __launch_bounds__(256, QPT >= 8 ? 1 : 2) k_fused_fft(FftFusedArg
             if (tid == 0) {
                 s_w10[0] = w1[0][0][0];
                 s_w10[1] = w1[1][0][0];
+                if constexpr (MC) {
+                    for (int c = 0; c < 2; ++c) {
+                        float zz[4] = {0.f, 0.f, 0.f, 0.f};
+                        if (pth[c] < A.M) normals4_f32(A.key, uint64_t(pth[c]), 2u, 0u, zz);
+                        s_w20[c] = zz[0];
+                    }
+                }
             }
             for (int u = 0; u < A.nunits; ++u) {
                 const FusedUnit U = A.units[u];
@@ -132,11 +143,11 @@ __global__ void __launch_bounds__(256, QPT >= 8 ? 1 : 2) k_fused_fft(FftFusedArg
                 for (int g0 = U.g_begin; g0 < U.g_end; g0 += FFT_GR) {
                     const int gn = U.g_end - g0 < FFT_GR ? U.g_end - g0 : FFT_GR;
                     float c1[FFT_GR], al[FFT_GR], be[FFT_GR];
-                    float fv[FFT_GR][2], fs[FFT_GR][2];
+                    float fv[FFT_GR][2], fs[FFT_GR][2], fp[FFT_GR][2];
 #pragma unroll
                     for (int gg = 0; gg < FFT_GR; ++gg) {
                         c1[gg] = al[gg] = be[gg] = 0.f;
-                        fv[gg][0] = fv[gg][1] = fs[gg][0] = fs[gg][1] = 0.f;
+                        fv[gg][0] = fv[gg][1] = fs[gg][0] = fs[gg][1] = fp[gg][0] = fp[gg][1] = 0.f;
                         if (gg < gn) {
                             c1[gg] = A.groups[g0 + gg].c1;
                             al[gg] = A.groups[g0 + gg].al;
@@ -146,6 +157,12 @@ __global__ void __launch_bounds__(256, QPT >= 8 ? 1 : 2) k_fused_fft(FftFusedArg
 #pragma unroll
                     for (int j = 0; j < QPT; ++j) {
                         const int q = tid + 256 * j;
+                        float wp[2][4] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
+                        if constexpr (MC) {
+#pragma unroll
+                            for (int c = 0; c < 2; ++c)
+                                if (q < nq && pth[c] < A.M) normals4_f32(A.key, uint64_t(pth[c]), 2u, uint32_t(q), wp[c]);
+                        }
 #pragma unroll
                         for (int t = 0; t < 4; ++t) {
                             const int i = 4 * q + t;  // v_i uses Wtilde_{i-1}; i = 0 is v_0 = xi
@@ -162,6 +179,10 @@ __global__ void __launch_bounds__(256, QPT >= 8 ? 1 : 2) k_fused_fft(FftFusedArg
                                         fs[gg][0] = fmaf(ra, w1[0][j][t], fs[gg][0]);
                                         fv[gg][1] = fmaf(rb, rb, fv[gg][1]);
                                         fs[gg][1] = fmaf(rb, w1[1][j][t], fs[gg][1]);
+                                        if constexpr (MC) {
+                                            fp[gg][0] = fmaf(ra, wp[0][t], fp[gg][0]);
+                                            fp[gg][1] = fmaf(rb, wp[1][t], fp[gg][1]);
+                                        }
                                     }
                                 }
                             }
@@ -175,9 +196,12 @@ __global__ void __launch_bounds__(256, QPT >= 8 ? 1 : 2) k_fused_fft(FftFusedArg
                             for (int c = 0; c < 2; ++c) {
                                 const double sv = wave_sum64(double(fv[gg][c]));
                                 const double ss = wave_sum64(double(fs[gg][c]));
+                                double sp = 0.0;
+                                if constexpr (MC) sp = wave_sum64(double(fp[gg][c]));
                                 if (lane == 0) {
-                                    red[((wave * FFT_GR + gg) * 2 + c) * 2 + 0] = sv;
-                                    red[((wave * FFT_GR + gg) * 2 + c) * 2 + 1] = ss;
+                                    red[((wave * FFT_GR + gg) * 2 + c) * FFT_NS + 0] = sv;
+                                    red[((wave * FFT_GR + gg) * 2 + c) * FFT_NS + 1] = ss;
+                                    red[((wave * FFT_GR + gg) * 2 + c) * FFT_NS + 2] = sp;
                                 }
                             }
                         }
@@ -188,12 +212,29 @@ __global__ void __launch_bounds__(256, QPT >= 8 ? 1 : 2) k_fused_fft(FftFusedArg
                         const int gg = tid >> 1, c = tid & 1;
                         const FusedGroup Gr = A.groups[g0 + gg];
                         double a = Gr.xi, b = Gr.sxi * double(s_w10[c]);  // step 0: v_0 = xi
+                        double cp = MC ? Gr.sxi * double(s_w20[c]) : 0.0;
                         for (int w = 0; w < 4; ++w) {
-                            a += red[((w * FFT_GR + gg) * 2 + c) * 2 + 0];
-                            b += red[((w * FFT_GR + gg) * 2 + c) * 2 + 1];
+                            a += red[((w * FFT_GR + gg) * 2 + c) * FFT_NS + 0];
+                            b += red[((w * FFT_GR + gg) * 2 + c) * FFT_NS + 1];
+                            if constexpr (MC) cp += red[((w * FFT_GR + gg) * 2 + c) * FFT_NS + 2];
                         }
                         const double Iv = Gr.dt * a, Is = Gr.sdt * b;
-                        for (int pg = Gr.pg_begin; pg < Gr.pg_end; ++pg) {
+                        if constexpr (MC) {
+                            // ln S_T = rho Is + sqrt(1 - rho^2) Ip - Iv / 2 (log-Euler, left point)
+                            const double Ip = Gr.sdt * cp;
+                            for (int pg = Gr.pg_begin; pg < Gr.pg_end; ++pg) {
+                                const double rho = A.pgs[pg].rho;
+                                PathPG rec;
+                                rec.lnS = rho * Is + sqrt(1.0 - rho * rho) * Ip - 0.5 * Iv;
+                                rec.sig = exp(rec.lnS);
+                                rec.S = float(rec.sig);
+                                rec.invs = 0.f;
+                                rec.lnSf = float(rec.lnS);
+                                rec.sigf = 0.f;
+                                pgd[c * A.pgu + (pg - U.pg_begin)] = rec;
+                            }
+                        }
+                        for (int pg = Gr.pg_begin; !MC && pg < Gr.pg_end; ++pg) {
                             const double rho = A.pgs[pg].rho;
                             PathPG rec;
                             rec.sig = sqrt((1.0 - rho * rho) * Iv);
@@ -214,7 +255,8 @@ __global__ void __launch_bounds__(256, QPT >= 8 ? 1 : 2) k_fused_fft(FftFusedArg
                     double s1 = 0.0, s2 = 0.0;
                     for (int c = 0; c < 2; ++c) {
                         if (pth[c] >= A.M) continue;
-                        const double x = payoff_mixed(pgd[c * A.pgu + (O.pg - U.pg_begin)], lnKf, Kf, O.lnK, O.K);
+                        const PathPG& rec = pgd[c * A.pgu + (O.pg - U.pg_begin)];
+                        const double x = MC ? payoff_mc(rec, O.K) : payoff_mixed(rec, lnKf, Kf, O.lnK, O.K);
                         s1 += x;
                         s2 += x * x;
                     }
@@ -244,9 +286,15 @@ hipError_t launch_fused_fft(const FftFusedArgs& a, int grid, hipStream_t st) {
         hipLaunchKernelGGL(kern, dim3(grid), dim3(256), lds, st, a);
         return hipGetLastError();
     };
-    if (qpt <= 2) return go(&k_fused_fft<2>);
-    if (qpt <= 4) return go(&k_fused_fft<4>);
-    if (qpt <= 8) return go(&k_fused_fft<8>);
+    if (a.mc) {
+        if (qpt <= 2) return go(&k_fused_fft<2, true>);
+        if (qpt <= 4) return go(&k_fused_fft<4, true>);
+        if (qpt <= 8) return go(&k_fused_fft<8, true>);
+    } else {
+        if (qpt <= 2) return go(&k_fused_fft<2, false>);
+        if (qpt <= 4) return go(&k_fused_fft<4, false>);
+        if (qpt <= 8) return go(&k_fused_fft<8, false>);
+    }
     return hipErrorInvalidValue;
 }
 

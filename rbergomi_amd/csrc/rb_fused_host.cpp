@@ -590,6 +590,7 @@ int FftFused::build(const Request& req, const Layout& lay, int device, hipStream
     a.log2L = log2L;
     a.nunits = int(lay.units.size());
     a.out_size = int(lay.options.size());
+    a.mc = req.method == METHOD_MC ? 1 : 0;
     a.pgu = pgu;
     a.pairs_per_item = 32;
     a.M = req.M;

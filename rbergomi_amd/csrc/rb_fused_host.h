@@ -38,7 +38,7 @@ int plan_fused_request(const Request& req, const Layout& lay, int mode, FusedHos
 void fused_tables(const Request& req, const Layout& lay, int Np, int frag_stride, FusedHostReq& out,
                   int& pgmax, int& gmax);
 
-// A long-path request (N > 1024, Romano-Touzi) on the single-launch FFT kernel.
+// A long-path request (N > 1024, Romano-Touzi or plain MC) on the single-launch FFT kernel.
 struct FftFused {
     FftFusedArgs args{};
     int grid = 0, total_out = 0;

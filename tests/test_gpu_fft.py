@@ -52,18 +52,19 @@ def test_fft_prices_match_fp64(rb, N):
     assert err <= 1e-4
 
 
+@pytest.mark.parametrize("method", [0, 1])
 @pytest.mark.parametrize("N,M", [(1100, 1999), (2048, 1500), (4096, 700), (8192, 301)])
-def test_fused_long_path_prices_match_fp64(rb, N, M):
-    # the mixed-precision precisions beyond N = 1024 (Romano-Touzi, device Gaussians) run the
-    # single-launch FFT kernel: same Philox counters as the FP64 path, FP32 transform and
-    # integrals, FP64 sums; two units (H), several maturities and correlations, odd M
+def test_fused_long_path_prices_match_fp64(rb, N, M, method):
+    # the mixed-precision precisions beyond N = 1024 (device Gaussians; Romano-Touzi and plain
+    # Monte Carlo) run the single-launch FFT kernel: same Philox counters as the FP64 path, FP32
+    # transform and integrals, FP64 sums; two units (H), several maturities and correlations, odd M
     H, eta, rho = [0.05, 0.2], [2.3, 1.2], [-0.9, -0.4]
     T = [0.25, 1.0, 1.0, 2.0, 0.5]
     K = [0.9, 0.8, 1.2, 1.0, 1.05]
-    print("fused long path N=%d M=%d" % (N, M), flush=True)
-    p64, iv64, s64 = rb.price_v2(0.03, H, eta, rho, T, K, N, M, [6, N], precision=rb.PREC_FP64)
+    print("fused long path N=%d M=%d method=%d" % (N, M, method), flush=True)
+    p64, iv64, s64 = rb.price_v2(0.03, H, eta, rho, T, K, N, M, [6, N], precision=rb.PREC_FP64, method=method)
     for code in (rb.PREC_SPLIT_FP16, rb.PREC_FP32_GEMM):
-        pm, ivm, sm = rb.price_v2(0.03, H, eta, rho, T, K, N, M, [6, N], precision=code)
+        pm, ivm, sm = rb.price_v2(0.03, H, eta, rho, T, K, N, M, [6, N], precision=code, method=method)
         err = np.max(np.abs(pm - p64))
         print("   precision %d: max |price - fp64| = %.2e, |stat - fp64| = %.2e" % (code, err, np.max(np.abs(sm - s64))),
               flush=True)
