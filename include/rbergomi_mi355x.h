@@ -150,6 +150,13 @@ int rb_price_sobol(double xi, const double* H, const double* eta, const double* 
                    const double* T, const double* K, int op_size, int N, long M,
                    const uint64_t* seed, int seed_size, const rb_config* cfg, double* price,
                    double* iv, double* stat, int out_size);
+/* The same points through the Brownian-bridge (hierarchical) mapping of each stream's N = 2^m
+ * dimensions, hierarchical2increments (rBergomi/aux.h:105-125; used with Sobol' points in
+ * rBergomi/text/tests.cpp:139-140): the first dimensions carry the coarse shape of each path. */
+int rb_price_sobol_bridge(double xi, const double* H, const double* eta, const double* rho, int par_size,
+                          const double* T, const double* K, int op_size, int N, long M,
+                          const uint64_t* seed, int seed_size, const rb_config* cfg, double* price,
+                          double* iv, double* stat, int out_size);
 
 /* ---- unordered rows ----
  * Row i prices the call (T[i], K[i]) under (xi[i], H[i], eta[i], rho[i]); every row uses the same
@@ -273,6 +280,9 @@ int rb_layout(const double* H, const double* eta, const double* rho, int par_siz
 /* Host only: Sobol' points i0 .. i0+n-1 of the quasi-Monte Carlo source in dims dimensions,
  * u[p*dims + d] (the generator behind rb_price_sobol). */
 int rb_probe_sobol(int dims, long i0, long n, double* u);
+/* The device Brownian-bridge mapping of rb_price_sobol_bridge on rows of N = 2^m values:
+ * dW[r*N + i] from Z[r*N ..]. */
+int rb_probe_bridge(const double* Z, long rows, int N, double* dW);
 /* The device inverse normal CDF of the quasi-Monte Carlo source: z[i] = Phi^-1(u[i]). */
 int rb_probe_norm_inv(const double* u, long n, double* z);
 /* Host only: the exact scheme's covariance blocks A21[i][j] = Cov(Wtilde_i, W1_j),

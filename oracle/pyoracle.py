@@ -72,6 +72,8 @@ def lib():
         L.orc_philox4x32_10.argtypes = [_u32p, _u32p, _u32p]
         L.orc_philox_key.argtypes = [_u64p, i, _u32p]
         L.orc_normals_philox.argtypes = [_u64p, i, l, l, i, i, _dp]
+        L.orc_bridge.argtypes = [_dp, i, _dp]
+        L.orc_bridge.restype = None
         _lib = L
     return _lib
 
@@ -270,6 +272,19 @@ def philox_key(seed):
     out = np.zeros(2, dtype=np.uint32)
     lib().orc_philox_key(s.ctypes.data_as(_u64p), len(s), out.ctypes.data_as(_u32p))
     return out
+
+
+def bridge(Z):
+    """hierarchical2increments of each row of Z (rows of 2^m values) -> dW, same shape"""
+    Z = np.ascontiguousarray(np.atleast_2d(np.asarray(Z, dtype=np.float64)))
+    n = Z.shape[1]
+    m = int(n).bit_length() - 1
+    assert 1 << m == n, "rows of 2^m values"
+    dW = np.zeros_like(Z)
+    for r in range(Z.shape[0]):
+        _, pz = _d(Z[r])
+        lib().orc_bridge(pz, m, dW[r].ctypes.data_as(_dp))
+    return dW
 
 
 def normals_philox(seed, p0, np_, nstreams, N):

@@ -766,4 +766,21 @@ void orc_normals_philox(const uint64_t* seed, int seed_size, long p0, long np, i
     }
 }
 
+/* Brownian-bridge (hierarchical) representation, rBergomi/aux.h:105-125: increment i of the
+ * 2^m cells is 2^(-m/2) Z_0 plus, for every level n = 1..m, +-2^((n-m-1)/2) times the
+ * coefficient of the level-n cell containing i (+ in its left half, - in its right half). */
+void orc_bridge(const double* Z, int m, double* dW) {
+    const long M = 1L << m;
+    for (long i = 0; i < M; ++i) {
+        double v = std::pow(2.0, -0.5 * m) * Z[0];
+        for (int n = 1; n <= m; ++n) {
+            const long cell = i >> (m - n);                 // floor(i 2^(n-m))
+            const bool left = (cell % 2) == 0;
+            const long index = (1L << (n - 1)) + cell / 2;  // coefficient of that level-n cell
+            v += std::pow(2.0, 0.5 * (n - m - 1)) * (left ? 1.0 : -1.0) * Z[index];
+        }
+        dW[i] = v;
+    }
+}
+
 }  // extern "C"

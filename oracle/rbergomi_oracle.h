@@ -35,6 +35,9 @@
  *   orc_payoff_rows_samples  equals orc_payoff_rt_samples on the zipped tensor laid out as rows;
  *                        per-row xi equals separate scalar-xi runs; row-order independent
  *                        (test_oracle_rows.py)
+ *   orc_bridge           orthogonal map (unit vectors -> orthonormal rows); sum of dW = 2^(m/2) Z_0;
+ *                        partial sums at dyadic points use only coarser coefficients and give the
+ *                        Brownian-bridge midpoint law; m = 1, 2 written out                (test_qmc_cpu.py)
  *
  * Parity unpinned: the exact roundoff of FFTW (the reference convolution) and the iterate
  * sequence of Boost's bracket_and_solve_root in IV_call (rBergLib/BlackScholes.cpp:36-63).
@@ -192,6 +195,8 @@ void orc_philox_key(const uint64_t* seed, int seed_size, uint32_t* key);
  * out[(p*nstreams + s)*N + j] for p in [p0, p0+np). */
 void orc_normals_philox(const uint64_t* seed, int seed_size, long p0, long np, int nstreams,
                         int N, double* out);
+/* hierarchical2increments (rBergomi/aux.h:105-125): Z[2^m] (bridge coefficients) -> dW[2^m] */
+void orc_bridge(const double* Z, int m, double* dW);
 
 #ifdef __cplusplus
 }

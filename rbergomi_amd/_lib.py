@@ -85,6 +85,8 @@ def load():
     L.rb_probe_sobol.argtypes = [i, l, l, dp]
     L.rb_probe_norm_inv.argtypes = [dp, l, dp]
     L.rb_price_sobol.argtypes = [d, dp, dp, dp, i, dp, dp, i, i, l, u64p, i, cfgp, dp, dp, dp, i]
+    L.rb_price_sobol_bridge.argtypes = [d, dp, dp, dp, i, dp, dp, i, i, l, u64p, i, cfgp, dp, dp, dp, i]
+    L.rb_probe_bridge.argtypes = [dp, l, i, dp]
     L.rb_probe_integrals.argtypes = [d, dp, dp, dp, i, dp, dp, i, i, l, dp, cfgp, dp]
     _lib = L
     return _lib

@@ -115,9 +115,11 @@ def payoff_samples_v2(xi, H, eta, rho, T, K, N, Z, method=_lib.METHOD_RT,
 
 
 def price_sobol(xi, H, eta, rho, T, K, N, M, precision=_lib.PREC_FP64, compensator=_lib.COMP_REF,
-                method=_lib.METHOD_RT, scheme=0, seed=None):
+                method=_lib.METHOD_RT, scheme=0, seed=None, bridge=False):
     """Quasi-Monte Carlo pricing on Sobol' points (rb_price_sobol) -> (price, iv, stat).  seed:
-    None for the reference's unscrambled points 1..M, else a random digital shift keyed by it."""
+    None for the reference's unscrambled points 1..M, else a random digital shift keyed by it.
+    bridge: map each stream's N = 2^m dimensions through the Brownian-bridge construction
+    (rb_price_sobol_bridge, hierarchical2increments of rBergomi/aux.h:105-125)."""
     s, ps = _seed(seed if seed is not None else [])
     H, pH = _arr(H)
     eta, pe = _arr(eta)
@@ -127,10 +129,18 @@ def price_sobol(xi, H, eta, rho, T, K, N, M, precision=_lib.PREC_FP64, compensat
     out = len(H) * len(T)
     price, iv, stat = np.zeros(out), np.zeros(out), np.zeros(out)
     cfg = _cfg(precision, compensator, method, scheme)
-    check(_lib.load().rb_price_sobol(float(xi), pH, pe, pr, len(H), pT, pK, len(T), int(N), int(M),
-                                     ps, len(s), ctypes.byref(cfg), price.ctypes.data_as(dp), iv.ctypes.data_as(dp),
-                                     stat.ctypes.data_as(dp), out))
+    fn = _lib.load().rb_price_sobol_bridge if bridge else _lib.load().rb_price_sobol
+    check(fn(float(xi), pH, pe, pr, len(H), pT, pK, len(T), int(N), int(M), ps, len(s), ctypes.byref(cfg),
+             price.ctypes.data_as(dp), iv.ctypes.data_as(dp), stat.ctypes.data_as(dp), out))
     return price, iv, stat
+
+
+def bridge_increments(Z):
+    """The device Brownian-bridge mapping (rb_probe_bridge) of rows of N = 2^m values."""
+    Z = np.ascontiguousarray(np.atleast_2d(np.asarray(Z, dtype=np.float64)))
+    dW = np.zeros_like(Z)
+    check(_lib.load().rb_probe_bridge(Z.ctypes.data_as(dp), Z.shape[0], Z.shape[1], dW.ctypes.data_as(dp)))
+    return dW
 
 
 def sobol_points(dims, i0, n):

@@ -209,16 +209,16 @@ int rb_price_v2(double xi, const double* H, const double* eta, const double* rho
     return rc;
 }
 
-int rb_price_sobol(double xi, const double* H, const double* eta, const double* rho, int par_size,
-                   const double* T, const double* K, int op_size, int N, long M,
-                   const uint64_t* seed, int seed_size, const rb_config* cfg, double* price,
-                   double* iv, double* stat, int out_size) {
+static int price_sobol_impl(double xi, const double* H, const double* eta, const double* rho, int par_size,
+                            const double* T, const double* K, int op_size, int N, long M,
+                            const uint64_t* seed, int seed_size, const rb_config* cfg, double* price,
+                            double* iv, double* stat, int out_size, int gauss) {
     set_last_error("");
     if (out_size != par_size * op_size) return fail(RB_ERR_SIZE, "out_size != par_size*op_size");
     if (par_size < 1 || op_size < 1) return fail(RB_ERR_ARG, "par_size and op_size must be positive");
     if (!H || !eta || !rho || !T || !K || !price || !stat) return fail(RB_ERR_ARG, "null array");
     Request r = make_request(xi, H, eta, rho, par_size, T, K, op_size, N, M, seed, seed_size, cfg);
-    r.gauss = 1;
+    r.gauss = gauss;
     std::string err;
     Plan p;
     int rc = p.init(r, nullptr, false, err);
@@ -226,6 +226,22 @@ int rb_price_sobol(double xi, const double* H, const double* eta, const double* 
     if (!rc) rc = p.results(price, iv, stat, err);
     if (rc) return fail(rc, err);
     return RB_OK;
+}
+
+int rb_price_sobol(double xi, const double* H, const double* eta, const double* rho, int par_size,
+                   const double* T, const double* K, int op_size, int N, long M,
+                   const uint64_t* seed, int seed_size, const rb_config* cfg, double* price,
+                   double* iv, double* stat, int out_size) {
+    return price_sobol_impl(xi, H, eta, rho, par_size, T, K, op_size, N, M, seed, seed_size, cfg, price, iv,
+                            stat, out_size, 1);
+}
+
+int rb_price_sobol_bridge(double xi, const double* H, const double* eta, const double* rho, int par_size,
+                          const double* T, const double* K, int op_size, int N, long M,
+                          const uint64_t* seed, int seed_size, const rb_config* cfg, double* price,
+                          double* iv, double* stat, int out_size) {
+    return price_sobol_impl(xi, H, eta, rho, par_size, T, K, op_size, N, M, seed, seed_size, cfg, price, iv,
+                            stat, out_size, 2);
 }
 
 int rb_probe_sobol(int dims, long i0, long n, double* u) {
@@ -563,6 +579,22 @@ int rb_probe_normals(const uint64_t* seed, int seed_size, long p0, long np, int 
     PROBE_TRY(hipMalloc(&d, n * sizeof(double)));
     PROBE_TRY(launch_normals_f64(philox_key(seed, seed_size), p0, np, nstreams, N, d, nullptr));
     PROBE_TRY(hipMemcpy(out, d, n * sizeof(double), hipMemcpyDeviceToHost));
+done:
+    if (d) (void)hipFree(d);
+    return rc;
+}
+
+int rb_probe_bridge(const double* Z, long rows, int N, double* dW) {
+    set_last_error("");
+    if (!Z || !dW || rows < 1 || N < 1 || (N & (N - 1)) != 0)
+        return fail(RB_ERR_ARG, "rows >= 1 of N = 2^m values required");
+    int rc = RB_OK;
+    double* d = nullptr;
+    const size_t n = size_t(rows) * N;
+    PROBE_TRY(hipMalloc(&d, n * sizeof(double)));
+    PROBE_TRY(hipMemcpy(d, Z, n * sizeof(double), hipMemcpyHostToDevice));
+    PROBE_TRY(launch_bridge_f64(d, rows, N, nullptr));
+    PROBE_TRY(hipMemcpy(dW, d, n * sizeof(double), hipMemcpyDeviceToHost));
 done:
     if (d) (void)hipFree(d);
     return rc;

@@ -378,11 +378,15 @@ int Plan::init(const Request& r, hipStream_t st, bool quadrature, std::string& e
         err = "the Cholesky scheme runs on the FP64 path";
         return ST_ERR_ARG;
     }
-    if (r.gauss == 1 && (r.turbo || quadrature)) {
+    if (r.gauss >= 1 && (r.turbo || quadrature)) {
         err = "Sobol' points are not combined with caller Gaussians or the turbo-charged estimator";
         return ST_ERR_ARG;
     }
-    if (r.gauss == 1 && r.M > 4294967294L) {
+    if (r.gauss == 2 && (r.N & (r.N - 1)) != 0) {
+        err = "the Brownian-bridge mapping needs N a power of two (hierarchical2increments)";
+        return ST_ERR_ARG;
+    }
+    if (r.gauss >= 1 && r.M > 4294967294L) {
         err = "Sobol' points: M above 2^32 - 2";
         return ST_ERR_LIMIT;
     }
@@ -399,7 +403,7 @@ int Plan::init(const Request& r, hipStream_t st, bool quadrature, std::string& e
         chunk = r.M;
         return ST_OK;
     }
-    if (r.precision == PREC_FP32 || (split && (r.N > 1024 || quadrature || r.gauss == 1))) {
+    if (r.precision == PREC_FP32 || (split && (r.N > 1024 || quadrature || r.gauss >= 1))) {
         // staged pipeline with the FP32 FFT convolution (long paths, user Gaussians)
         conv_fft = true;
     } else if (split) {
@@ -460,7 +464,7 @@ int Plan::init(const Request& r, hipStream_t st, bool quadrature, std::string& e
         rc = init_fft(err);
         if (rc) return rc;
     }
-    if (r.gauss == 1) {
+    if (r.gauss >= 1) {
         std::vector<uint32_t> V;
         sobol_directions(nstreams * N, V);
         RB_TRY(upload(&d_sobol, V, stream));
@@ -577,9 +581,10 @@ int Plan::enqueue(std::string& err) {
     }
     for (long p0 = 0; p0 < req.M; p0 += chunk) {
         const long np = std::min(chunk, req.M - p0);
-        if (req.gauss == 1)
+        if (req.gauss >= 1) {
             RB_LAUNCH(K_NORMALS, launch_normals_sobol_f64(d_sobol, d_shift, p0, np, nstreams, N, d_Z, stream));
-        else
+            if (req.gauss == 2) RB_LAUNCH(K_NORMALS, launch_bridge_f64(d_Z, np * nstreams, N, stream));
+        } else
             RB_LAUNCH(K_NORMALS, launch_normals_f64(key, p0, np, nstreams, N, d_Z, stream));
         RB_LAUNCH(K_VOLTERRA, conv(np, long(nstreams) * N));
         RB_LAUNCH(K_SCAN, launch_scan_f64(req.method, d_Z, long(nstreams) * N, d_Wt, np, N, d_groups,

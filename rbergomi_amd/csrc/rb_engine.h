@@ -50,7 +50,8 @@ struct Request {
     // turbo-charged RT estimator (control variate + antithetics, ML_console_ST/RBergomi.cpp:219-288):
     // m_control pilot paths estimate Q and alpha; q_max replaces Q where Q <= Iv
     int scheme = 0;              // 0 hybrid scheme, 1 exact Cholesky sampling of Wtilde
-    int gauss = 0;               // Gaussian source: 0 Philox (seeded), 1 Sobol' points (QMC)
+    int gauss = 0;               // Gaussian source: 0 Philox (seeded), 1 Sobol' points (QMC),
+                                 // 2 Sobol' points through the Brownian-bridge mapping
     bool turbo = false;
     long m_control = 10000;
     double q_max = 10000.0;

@@ -76,6 +76,8 @@ hipError_t launch_volterra_fft(const double* Z, long zstride, long np, int N, in
 hipError_t launch_normals_sobol_f64(const uint32_t* V, const uint32_t* shift, long p0, long np,
                                     int nstreams, int N, double* out, hipStream_t st);
 hipError_t launch_norm_inv_probe(const double* u, long n, double* z, hipStream_t st);
+// Brownian-bridge (hierarchical) mapping of rows of N = 2^m Gaussians, in place
+hipError_t launch_bridge_f64(double* Z, long rows, int N, hipStream_t st);
 // exact scheme: Wtilde = [L21 | L22] (x1, x2) per unit
 hipError_t launch_volterra_chol(const double* Z, long zstride, long np, int n, int nunits,
                                 const double* Lc, double* Wt, hipStream_t st);

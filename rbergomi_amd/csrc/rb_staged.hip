@@ -63,6 +63,46 @@ __global__ void k_normals_f64(PhiloxKey key, long p0, long np, int nstreams, int
     }
 }
 
+// ---------------------------------------------------------------- Brownian-bridge mapping
+// hierarchical2increments (rBergomi/aux.h:105-125) on every row of N = 2^m values, in place: the
+// coarse-to-fine coefficients Z of a Brownian bridge on 2^m cells become i.i.d. N(0,1) increments
+//   dW_i = 2^(-m/2) Z_0 + sum_{n=1}^{m} 2^((n-m-1)/2) s_n(i) Z_{2^(n-1) + floor(k_n / 2)},
+// k_n = floor(i 2^(n-m)), s_n(i) = +1 for even k_n (left half of its level-n cell), -1 for odd.
+// One workgroup per row; the row is staged in LDS.
+__global__ void __launch_bounds__(256) k_bridge_f64(double* __restrict__ Z, long rows, int m) {
+    extern __shared__ double zr[];
+    const int N = 1 << m;
+    for (long r = blockIdx.x; r < rows; r += gridDim.x) {
+        double* row = Z + r * N;
+        for (int i = threadIdx.x; i < N; i += 256) zr[i] = row[i];
+        __syncthreads();
+        for (int i = threadIdx.x; i < N; i += 256) {
+            double v = exp2(-0.5 * m) * zr[0];
+            for (int n = 1; n <= m; ++n) {
+                const int k = i >> (m - n);
+                const double w = exp2(0.5 * (n - m - 1));
+                v += (k & 1) ? -w * zr[(1 << (n - 1)) + (k >> 1)] : w * zr[(1 << (n - 1)) + (k >> 1)];
+            }
+            row[i] = v;
+        }
+        __syncthreads();
+    }
+}
+
+hipError_t launch_bridge_f64(double* Z, long rows, int N, hipStream_t st) {
+    int m = 0;
+    while ((1 << m) < N) ++m;
+    if ((1 << m) != N || rows < 1) return hipErrorInvalidValue;
+    const size_t lds = size_t(N) * sizeof(double);
+    if (lds > 65536) {
+        const hipError_t e = allow_lds(reinterpret_cast<const void*>(&k_bridge_f64), lds);
+        if (e != hipSuccess) return e;
+    }
+    const unsigned grid = unsigned(rows < 65536 ? rows : 65536);
+    hipLaunchKernelGGL(k_bridge_f64, dim3(grid), dim3(256), lds, st, Z, rows, m);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------- Volterra convolution
 // Y_i = sum_{k=1}^{i} Gamma_k W1_{i-k}.  A thread owns four consecutive outputs i = 4b+r and
 // slides a four-entry window of W1 down the Toeplitz diagonal: per k one broadcast read of

@@ -92,3 +92,42 @@ def test_sobol_split_precision_within_budget(rb):
     p64 = rb.price_sobol(*args)[0]
     p16 = rb.price_sobol(*args, precision=rb.PREC_SPLIT_FP16)[0]
     assert np.max(np.abs(p16 - p64)) <= 1e-4
+
+
+# ---- Brownian-bridge mapping of the Sobol' points (rb_price_sobol_bridge)
+
+@pytest.mark.parametrize("m", [0, 1, 4, 9, 13])
+def test_device_bridge_matches_oracle(rb, orc, m):
+    Z = np.random.default_rng(m).standard_normal((7, 1 << m))
+    dW = rb.bridge_increments(Z)
+    assert np.max(np.abs(dW - orc.bridge(Z))) < 1e-13
+
+
+@pytest.mark.parametrize("method", [0, 1])
+def test_bridged_sobol_prices_match_oracle(rb, orc, method):
+    # host points -> ndtri -> per stream (breakZ order) -> bridge -> oracle payoffs
+    H, eta, rho = [0.07, 0.3], [1.9, 0.9], [-0.9, -0.4]
+    T, K = [0.5, 1.0], [0.95, 1.1]
+    N, M = 32, 3000
+    ns = 3 if method else 2
+    p, iv, s = rb.price_sobol(0.04, H, eta, rho, T, K, N, M, method=method, bridge=True)
+    Z = sobol_gaussians(rb, M, N, ns)
+    Zb = orc.bridge(Z.reshape(M * ns, N)).reshape(M, ns, N)
+    pay = (orc.payoff_mc_samples if method else orc.payoff_rt_samples)(0.04, H, eta, rho, T, K, N, Zb)
+    pr, st = orc.price_stat(pay)
+    assert np.max(np.abs(p - pr)) < 1e-12
+    assert np.max(np.abs(s - st)) < 1e-12
+
+
+def test_bridged_rqmc_is_unbiased(rb):
+    args = (0.04, [0.1], [1.5], [-0.7], [0.5, 1.0, 1.0], [1.0, 0.9, 1.1], 64)
+    ref, _, sref = rb.price_v2(*args, 1 << 22, [7, 8])
+    est = np.array([rb.price_sobol(*args, 1 << 14, seed=[s, 1], bridge=True)[0] for s in range(8)])
+    mean, se = est.mean(0), est.std(0, ddof=1) / np.sqrt(len(est))
+    assert np.all(np.abs(mean - ref) <= 4.5 * np.sqrt(se ** 2 + sref ** 2))
+
+
+def test_bridge_needs_power_of_two_steps(rb):
+    with pytest.raises(rb.RbError) as e:
+        rb.price_sobol(0.04, [0.1], [1.5], [-0.7], [1.0], [1.0], 48, 1000, bridge=True)
+    assert e.value.code == 1

@@ -78,3 +78,49 @@ def test_pairs_are_t_s_sequences(rb, d1, d2):
 def test_points_are_never_zero_after_the_first(rb):
     u = rb.sobol_points(64, 1, 5000)
     assert np.all(u > 0) and np.all(u < 1)
+
+
+# ---- Brownian-bridge (hierarchical) mapping: orc_bridge restates rBergomi/aux.h:105-125; the
+# checks below pin it by the construction's own properties, not by retyping the formula.
+
+def test_bridge_small_cases_written_out():
+    from oracle import pyoracle as orc
+    z = np.array([[0.3, -1.1]])
+    assert np.allclose(orc.bridge(z), [[(0.3 - 1.1) / np.sqrt(2), (0.3 + 1.1) / np.sqrt(2)]])
+    z = np.array([[1.0, 2.0, 3.0, 4.0]])
+    r2 = 1 / np.sqrt(2)
+    assert np.allclose(orc.bridge(z), [[0.5 + 1.0 + 3 * r2, 0.5 + 1.0 - 3 * r2, 0.5 - 1.0 + 4 * r2, 0.5 - 1.0 - 4 * r2]])
+
+
+@pytest.mark.parametrize("m", [0, 1, 2, 3, 5, 7])
+def test_bridge_is_orthogonal(m):
+    # i.i.d. N(0,1) coefficients map to i.i.d. N(0,1) increments: the map is orthogonal
+    # (unit variances are the reference's own check, rBergomi/text/tests.cpp:205-216)
+    from oracle import pyoracle as orc
+    n = 1 << m
+    A = orc.bridge(np.eye(n)).T  # column k = image of the k-th unit coefficient
+    assert np.allclose(A @ A.T, np.eye(n), atol=1e-14)
+
+
+@pytest.mark.parametrize("m", [1, 3, 6])
+def test_bridge_builds_the_path_coarse_to_fine(m):
+    # B(t_k) = 2^(-m/2) sum_{i<k} dW_i on the unit interval.  B(1) = Z_0; at every level n the
+    # midpoint of a level-(n-1) cell [a, b] deviates from the chord by sqrt(b - a) / 2 times that
+    # cell's level-n coefficient Z_{2^(n-1) + c}; path values at level-n points use no finer
+    # coefficient.
+    from oracle import pyoracle as orc
+    n_cells = 1 << m
+    Z = np.random.default_rng(m).standard_normal((5, n_cells))
+    B = np.concatenate([np.zeros((5, 1)), np.cumsum(orc.bridge(Z), axis=1)], axis=1) * 2.0 ** (-m / 2)
+    assert np.allclose(B[:, -1], Z[:, 0])
+    for n in range(1, m + 1):
+        w = n_cells >> (n - 1)       # grid steps per level-(n-1) cell
+        for c in range(1 << (n - 1)):
+            a, b = c * w, (c + 1) * w
+            mid = B[:, a + w // 2] - 0.5 * (B[:, a] + B[:, b])
+            assert np.allclose(mid, 0.5 * np.sqrt(w / n_cells) * Z[:, (1 << (n - 1)) + c])
+        coarse = Z.copy()
+        coarse[:, 1 << n:] = 0.0
+        Bc = np.concatenate([np.zeros((5, 1)), np.cumsum(orc.bridge(coarse), axis=1)], axis=1) * 2.0 ** (-m / 2)
+        step = n_cells >> n
+        assert np.allclose(Bc[:, ::step], B[:, ::step])
