@@ -18,6 +18,8 @@
 namespace rb {
 
 // Grid: (ceil(np/2), units).  Block 256.  LDS: L + L/32 float2 (element i in slot fpad(i)).
+// LOG2L: the transform length fixed at compile time (10..14), or 0 for any length.
+template <int LOG2L>
 __global__ void __launch_bounds__(256)
 k_volterra_fft(const double* __restrict__ Z, long zstride, long np, int N, int log2L,
                const Unit* __restrict__ units, const float2* __restrict__ ghat,
@@ -36,12 +38,18 @@ k_volterra_fft(const double* __restrict__ Z, long zstride, long np, int N, int l
         z[fpad(int(__brev(unsigned(i)) >> (32 - log2L)))] = make_float2(re, im);
     }
     __syncthreads();
-    fft_dit_r4(z, tw, log2L, threadIdx.x);
+    if constexpr (LOG2L > 0)
+        fft_dit_r4_t<LOG2L>(z, tw, threadIdx.x);
+    else
+        fft_dit_r4(z, tw, log2L, threadIdx.x);
     // pointwise product with FFT(Gamma) of this unit
     const float2* G = ghat + long(u) * L;
     for (int k = threadIdx.x; k < L; k += 256) z[fpad(k)] = cmulf(z[fpad(k)], G[k]);
     __syncthreads();
-    fft_dif_inv_r4(z, tw, log2L, threadIdx.x);
+    if constexpr (LOG2L > 0)
+        fft_dif_inv_r4_t<LOG2L>(z, tw, threadIdx.x);
+    else
+        fft_dif_inv_r4(z, tw, log2L, threadIdx.x);
     const Unit un = units[u];
     const double invL = 1.0 / double(L);
     for (int i = threadIdx.x; i < N; i += 256) {
@@ -63,13 +71,23 @@ hipError_t launch_volterra_fft(const double* Z, long zstride, long np, int N, in
                                const Unit* units, const float2* ghat, const float2* tw, double* Wt,
                                hipStream_t st) {
     const size_t lds = volterra_fft_lds_bytes(log2L);
-    if (lds > 65536) {
-        const hipError_t e = allow_lds(reinterpret_cast<const void*>(&k_volterra_fft), lds);
-        if (e != hipSuccess) return e;
-    }
     dim3 grid(unsigned((np + 1) / 2), unsigned(nunits));
-    hipLaunchKernelGGL(k_volterra_fft, grid, dim3(256), lds, st, Z, zstride, np, N, log2L, units, ghat, tw, Wt);
-    return hipGetLastError();
+    auto go = [&](auto kern) -> hipError_t {
+        if (lds > 65536) {
+            const hipError_t e = allow_lds(reinterpret_cast<const void*>(kern), lds);
+            if (e != hipSuccess) return e;
+        }
+        hipLaunchKernelGGL(kern, grid, dim3(256), lds, st, Z, zstride, np, N, log2L, units, ghat, tw, Wt);
+        return hipGetLastError();
+    };
+    switch (log2L) {
+        case 10: return go(&k_volterra_fft<10>);
+        case 11: return go(&k_volterra_fft<11>);
+        case 12: return go(&k_volterra_fft<12>);
+        case 13: return go(&k_volterra_fft<13>);
+        case 14: return go(&k_volterra_fft<14>);
+        default: return go(&k_volterra_fft<0>);
+    }
 }
 
 }  // namespace rb

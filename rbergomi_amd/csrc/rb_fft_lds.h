@@ -88,4 +88,122 @@ __device__ __forceinline__ void fft_dif_inv_r4(float2* z, const float2* __restri
     }
 }
 
+// ---- the same transforms with the length a compile-time constant (1024 <= L): every pass
+// unrolled, butterfly spans known, and from span 32 up the four LDS slots of a butterfly at fixed
+// offsets from the first (element i + k span is slot fpad(i) + k (span + span / 32)).
+template <int LOG2L, int S>
+__device__ __forceinline__ void fft_dit_pass(float2* z, const float2* __restrict__ tw, int tid) {
+    if constexpr (S < LOG2L) {
+        constexpr int L = 1 << LOG2L, span = 1 << S;
+        constexpr int step = span + (span >> 5);
+#pragma unroll 2
+        for (int u = 0; u < (L >> 2) / 256; ++u) {
+            const int t = tid + 256 * u;
+            const int pos = t & (span - 1);
+            const int i0 = ((t >> S) << (S + 2)) + pos;
+            int q0, q1, q2, q3;
+            if constexpr (span >= 32) {
+                q0 = fpad(i0);
+                q1 = q0 + step;
+                q2 = q1 + step;
+                q3 = q2 + step;
+            } else {
+                q0 = fpad(i0);
+                q1 = fpad(i0 + span);
+                q2 = fpad(i0 + 2 * span);
+                q3 = fpad(i0 + 3 * span);
+            }
+            const float2 a = tw[pos << (LOG2L - S - 1)], b = tw[pos << (LOG2L - S - 2)];
+            const float2 x0 = z[q0], x1 = z[q1], x2 = z[q2], x3 = z[q3];
+            const float2 t1 = cmulf(a, x1), t3 = cmulf(a, x3);
+            const float2 x0p = make_float2(x0.x + t1.x, x0.y + t1.y), x1p = make_float2(x0.x - t1.x, x0.y - t1.y);
+            const float2 x2p = make_float2(x2.x + t3.x, x2.y + t3.y), x3p = make_float2(x2.x - t3.x, x2.y - t3.y);
+            const float2 uu = cmulf(b, x2p), v = cmulf(b, x3p);
+            z[q0] = make_float2(x0p.x + uu.x, x0p.y + uu.y);
+            z[q2] = make_float2(x0p.x - uu.x, x0p.y - uu.y);
+            z[q1] = make_float2(x1p.x + v.y, x1p.y - v.x);
+            z[q3] = make_float2(x1p.x - v.y, x1p.y + v.x);
+        }
+        __syncthreads();
+        fft_dit_pass<LOG2L, S + 2>(z, tw, tid);
+    }
+}
+
+template <int LOG2L>
+__device__ __forceinline__ void fft_dit_r4_t(float2* z, const float2* __restrict__ tw, int tid) {
+    static_assert(LOG2L >= 10, "at least 256 radix-4 butterflies per pass");
+    constexpr int L = 1 << LOG2L;
+    if constexpr (LOG2L & 1) {
+#pragma unroll 2
+        for (int u = 0; u < (L >> 1) / 256; ++u) {
+            const int b = tid + 256 * u;
+            const int q0 = fpad(2 * b), q1 = fpad(2 * b + 1);
+            const float2 x = z[q0], y = z[q1];
+            z[q0] = make_float2(x.x + y.x, x.y + y.y);
+            z[q1] = make_float2(x.x - y.x, x.y - y.y);
+        }
+        __syncthreads();
+        fft_dit_pass<LOG2L, 1>(z, tw, tid);
+    } else {
+        fft_dit_pass<LOG2L, 0>(z, tw, tid);
+    }
+}
+
+template <int LOG2L, int S>
+__device__ __forceinline__ void fft_dif_pass(float2* z, const float2* __restrict__ tw, int tid) {
+    constexpr int L = 1 << LOG2L;
+    if constexpr (S >= 1) {
+        constexpr int sh = S - 1, span1 = 1 << sh;
+        constexpr int step = span1 + (span1 >> 5);
+#pragma unroll 2
+        for (int u = 0; u < (L >> 2) / 256; ++u) {
+            const int t = tid + 256 * u;
+            const int pos = t & (span1 - 1);
+            const int i0 = ((t >> sh) << (S + 1)) + pos;
+            int q0, q1, q2, q3;
+            if constexpr (span1 >= 32) {
+                q0 = fpad(i0);
+                q1 = q0 + step;
+                q2 = q1 + step;
+                q3 = q2 + step;
+            } else {
+                q0 = fpad(i0);
+                q1 = fpad(i0 + span1);
+                q2 = fpad(i0 + 2 * span1);
+                q3 = fpad(i0 + 3 * span1);
+            }
+            const float2 b0 = tw[pos << (LOG2L - S - 1)], a0 = tw[pos << (LOG2L - S)];
+            const float2 cb = make_float2(b0.x, -b0.y), ca = make_float2(a0.x, -a0.y);
+            const float2 x0 = z[q0], x1 = z[q1], x2 = z[q2], x3 = z[q3];
+            const float2 u0 = make_float2(x0.x + x2.x, x0.y + x2.y);
+            const float2 u2 = cmulf(make_float2(x0.x - x2.x, x0.y - x2.y), cb);
+            const float2 u1 = make_float2(x1.x + x3.x, x1.y + x3.y);
+            const float2 w = cmulf(make_float2(x1.x - x3.x, x1.y - x3.y), cb);
+            const float2 u3 = make_float2(-w.y, w.x);
+            z[q0] = make_float2(u0.x + u1.x, u0.y + u1.y);
+            z[q1] = cmulf(make_float2(u0.x - u1.x, u0.y - u1.y), ca);
+            z[q2] = make_float2(u2.x + u3.x, u2.y + u3.y);
+            z[q3] = cmulf(make_float2(u2.x - u3.x, u2.y - u3.y), ca);
+        }
+        __syncthreads();
+        fft_dif_pass<LOG2L, S - 2>(z, tw, tid);
+    } else if constexpr (S == 0) {
+#pragma unroll 2
+        for (int u = 0; u < (L >> 1) / 256; ++u) {
+            const int b = tid + 256 * u;
+            const int q0 = fpad(2 * b), q1 = fpad(2 * b + 1);
+            const float2 x = z[q0], y = z[q1];
+            z[q0] = make_float2(x.x + y.x, x.y + y.y);
+            z[q1] = make_float2(x.x - y.x, x.y - y.y);
+        }
+        __syncthreads();
+    }
+}
+
+template <int LOG2L>
+__device__ __forceinline__ void fft_dif_inv_r4_t(float2* z, const float2* __restrict__ tw, int tid) {
+    static_assert(LOG2L >= 10, "at least 256 radix-4 butterflies per pass");
+    fft_dif_pass<LOG2L, LOG2L - 1>(z, tw, tid);
+}
+
 }  // namespace rb

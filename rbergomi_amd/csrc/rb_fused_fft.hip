@@ -44,7 +44,11 @@ size_t fused_fft_lds_bytes(const FftFusedArgs& a) {
            size_t(2) * a.pgu * sizeof(PathPG) + size_t(a.out_size) * 2 * sizeof(double);
 }
 
-// QPT: quads per thread and path, ceil(ceil(N/4) / 256)
+// QPT: quads per thread and path, ceil(ceil(N/4) / 256).  For 1024 < N <= 8192 it fixes the
+// transform length: L = 2^(11 + log2 QPT) (QPT = 2, 4, 8 for N <= 2048, 4096, 8192).
+template <int QPT>
+constexpr int fft_log2L() { return QPT == 2 ? 12 : QPT == 4 ? 13 : 14; }
+
 template <int QPT, bool MC>
 __global__ void __launch_bounds__(256, (QPT >= 8 || (MC && QPT >= 4)) ? 1 : 2) k_fused_fft(FftFusedArgs A) {
     extern __shared__ float2 lds2[];
@@ -105,11 +109,11 @@ __global__ void __launch_bounds__(256, (QPT >= 8 || (MC && QPT >= 4)) ? 1 : 2) k
                 }
                 for (int i = 4 * nq + tid; i < L; i += 256) z[fpad(int(__brev(unsigned(i)) >> shift))] = make_float2(0.f, 0.f);
                 __syncthreads();
-                fft_dit_r4(z, A.tw, A.log2L, tid);
+                fft_dit_r4_t<fft_log2L<QPT>()>(z, A.tw, tid);
                 const float2* G = A.ghat + long(u) * L;
                 for (int k = tid; k < L; k += 256) z[fpad(k)] = cmulf(z[fpad(k)], G[k]);
                 __syncthreads();
-                fft_dif_inv_r4(z, A.tw, A.log2L, tid);
+                fft_dif_inv_r4_t<fft_log2L<QPT>()>(z, A.tw, tid);
                 // ---- 3. Wtilde of the thread's steps (both paths), then back to LDS in step order
                 const float invL = 1.0f / float(L);
                 float wt[2][QPT][4];
@@ -278,6 +282,9 @@ __global__ void __launch_bounds__(256, (QPT >= 8 || (MC && QPT >= 4)) ? 1 : 2) k
 hipError_t launch_fused_fft(const FftFusedArgs& a, int grid, hipStream_t st) {
     const size_t lds = fused_fft_lds_bytes(a);
     const int qpt = ((a.N + 3) / 4 + 255) / 256;
+    // the kernels are compiled for the transform length of their quad count
+    const int want = qpt <= 2 ? 12 : qpt <= 4 ? 13 : 14;
+    if (a.N <= 1024 || a.log2L != want) return hipErrorInvalidValue;
     auto go = [&](auto kern) -> hipError_t {
         if (lds > 65536) {
             const hipError_t e = allow_lds(reinterpret_cast<const void*>(kern), lds);
