@@ -84,3 +84,23 @@ def test_fused_long_path_reruns_and_plan_timing(rb):
     p, _, _ = pl.results()
     pl.close()
     assert "k_fused_fft" in times and np.array_equal(p, a[0])
+
+
+@pytest.mark.parametrize("N,M", [(1025, 1), (1025, 2), (2049, 3), (4097, 5)])
+def test_fused_long_path_edges(rb, N, M):
+    # the first long N of each transform length, a single path (the pair's second path absent)
+    T, K = [0.5, 1.0], [1.0, 1.1]
+    for method in (0, 1):
+        p64, _, s64 = rb.price_v2(0.04, [0.1], [1.5], [-0.7], T, K, N, M, [3, M], method=method)
+        pm, _, sm = rb.price_v2(0.04, [0.1], [1.5], [-0.7], T, K, N, M, [3, M], method=method,
+                                precision=rb.PREC_SPLIT_FP16)
+        assert np.all(np.isfinite(pm))
+        assert np.max(np.abs(pm - p64)) <= 1e-4
+
+
+def test_fused_long_path_rejects_what_lds_cannot_hold(rb):
+    # per-option FP64 sums live in LDS beside the L = 16384 transform at N = 8192
+    K = list(np.linspace(0.5, 1.5, 4000))
+    with pytest.raises(rb.RbError) as e:
+        rb.price_v2(0.04, [0.1], [1.5], [-0.7], [1.0] * len(K), K, 8192, 10, [1], precision=rb.PREC_SPLIT_FP16)
+    assert e.value.code == 5
