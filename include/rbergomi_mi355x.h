@@ -216,8 +216,8 @@ typedef struct rb_request {
 } rb_request;
 
 /* Prices n requests with one launch of the fused kernel (cfg->precision must be
- * RB_PREC_SPLIT_FP16, RB_PREC_SPLIT_BF16 or RB_PREC_FP32_GEMM, N <= 1024; cfg->method and
- * cfg->compensator apply to
+ * RB_PREC_SPLIT_FP16, RB_PREC_SPLIT_BF16 or RB_PREC_FP32_GEMM; requests with 1024 < N <= 8192 take a
+ * launch of the long-path FFT kernel each); cfg->method and cfg->compensator apply to
  * every request).  Every request's results are bit-identical to pricing it alone with
  * rb_price_v2 and the same cfg.  The server prices Romano-Touzi requests. */
 int rb_price_batch(const rb_request* reqs, int n, const rb_config* cfg);
@@ -246,7 +246,8 @@ int rb_tune_table(int* rows, int max);
 /* Continuous-batching server.  A host thread forms batches from the pending requests, admits a
  * request when the device bytes of the batches in flight leave room for it, runs up to
  * max_inflight batches on separate streams (a new batch starts while earlier ones finish), and
- * copies each request's results to its output arrays when its batch completes. */
+ * copies each request's results to its output arrays when its batch completes.  A request with
+ * 1024 < N <= 8192 runs as a batch of its own on the long-path FFT kernel. */
 typedef struct rb_server rb_server;
 typedef struct rb_server_config {
     int precision;            /* RB_PREC_SPLIT_FP16, RB_PREC_SPLIT_BF16 or RB_PREC_FP32_GEMM */

@@ -553,6 +553,37 @@ void FftFused::release() {
     d_ivopts = nullptr;
 }
 
+int FftFused::plan(const Request& req, const Layout& lay, size_t lds_limit, size_t& footprint, std::string& err) {
+    const int N = req.N;
+    if (N <= 1024 || ((N + 3) / 4 + 255) / 256 > 8) {
+        err = "the long-path kernel covers 1024 < N <= 8192";
+        return ST_ERR_LIMIT;
+    }
+    int log2L = 1;
+    while ((1 << log2L) < 2 * N - 1) ++log2L;
+    const size_t L = size_t(1) << log2L;
+    const int Np = (N + 31) / 32 * 32;
+    FusedHostReq tabs;
+    int pgmax = 1, gmax = 1;
+    fused_tables(req, lay, Np, 0, tabs, pgmax, gmax);
+    FftFusedArgs a{};
+    a.log2L = log2L;
+    a.out_size = int(lay.options.size());
+    a.pgu = 1;
+    for (const FusedUnit& U : tabs.fu) a.pgu = std::max(a.pgu, U.pg_end - U.pg_begin);
+    if (fused_fft_lds_bytes(a) > lds_limit) {
+        err = "long-path kernel needs " + std::to_string(fused_fft_lds_bytes(a)) +
+              " bytes of LDS (N or the option count too large)";
+        return ST_ERR_LIMIT;
+    }
+    const long nitems = ((req.M + 1) / 2 + 31) / 32;
+    footprint = lay.units.size() * L * sizeof(float2) + L / 2 * sizeof(float2) + tabs.ptab.size() * sizeof(float) +
+                tabs.fo.size() * sizeof(FusedOption) + tabs.fpg.size() * sizeof(FusedPG) +
+                tabs.fg.size() * sizeof(FusedGroup) + tabs.fu.size() * sizeof(FusedUnit) +
+                lay.options.size() * (sizeof(Option) + 3 * sizeof(double) + 3 * sizeof(long) + size_t(nitems) * 16) + 16;
+    return ST_OK;
+}
+
 int FftFused::build(const Request& req, const Layout& lay, int device, hipStream_t st, std::string& err) {
     release();
     stream = st;

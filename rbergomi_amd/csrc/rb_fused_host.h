@@ -62,6 +62,9 @@ struct FftFused {
 
     ~FftFused();
     void release();
+    // device bytes the request would allocate, after checking N and the LDS it needs (lds_limit)
+    static int plan(const Request& req, const Layout& lay, size_t lds_limit, size_t& footprint,
+                    std::string& err);
     int build(const Request& req, const Layout& lay, int device, hipStream_t st, std::string& err);
     int launch(hipStream_t st, std::string& err, Plan* timing = nullptr);
     int results(double* price, double* iv, double* stat, hipStream_t st, std::string& err);

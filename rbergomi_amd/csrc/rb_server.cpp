@@ -44,6 +44,7 @@ struct Job {
     Clock::time_point t_submit, t_done;
     int status = -1;  // -1 pending, else an rb_status
     bool done = false;
+    bool long_path = false;  // N > 1024: priced alone by the single-launch FFT kernel
 };
 
 // the request's inputs (not its output pointers) as a fused-kernel Request
@@ -80,18 +81,30 @@ int plan_job(const rb_request* q, int precision, int method, int comp, Job& j, s
     if (rc) return rc;
     const Request& r = j.req;
     j.lay = make_layout(r);
-    rc = plan_fused_request(r, j.lay, fused_mode_of(precision), j.fh, err);
-    if (rc) return rc;
     j.price = q->price;
     j.iv = q->iv;
     j.stat = q->stat;
     j.out = r.out_size();
+    if (r.N > 1024) {
+        // long paths: one request per launch of the FFT kernel
+        int device = 0;
+        hipDeviceProp_t prop;
+        if (hipGetDevice(&device) != hipSuccess || device_props(device, prop) != hipSuccess) {
+            err = "no HIP device";
+            return ST_ERR_DEVICE;
+        }
+        j.long_path = true;
+        return FftFused::plan(r, j.lay, prop.sharedMemPerBlock, j.footprint, err);
+    }
+    rc = plan_fused_request(r, j.lay, fused_mode_of(precision), j.fh, err);
+    if (rc) return rc;
     j.footprint = FusedBatch::footprint({&j.fh});
     return ST_OK;
 }
 
 struct Flight {
     std::unique_ptr<FusedBatch> batch;
+    std::unique_ptr<FftFused> fft;  // a long-path request (then jobs holds just it)
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
     double* pinned = nullptr;  // [3][total_out]: price, iv, stat
@@ -134,12 +147,12 @@ bool rb_server::reap(std::unique_lock<std::mutex>& lk) {
             ++f;
             continue;
         }
-        const int total = F.batch->total_out;
+        const int total = F.batch ? F.batch->total_out : F.fft->total_out;
         const Clock::time_point now = Clock::now();
         for (size_t i = 0; i < F.jobs.size(); ++i) {
             Job& j = *F.jobs[i];
             if (q == hipSuccess) {
-                const long o = F.batch->out_off[i];
+                const long o = F.batch ? F.batch->out_off[i] : 0;
                 std::memcpy(j.price, F.pinned + o, j.out * sizeof(double));
                 if (j.iv) std::memcpy(j.iv, F.pinned + total + o, j.out * sizeof(double));
                 std::memcpy(j.stat, F.pinned + 2 * total + o, j.out * sizeof(double));
@@ -155,6 +168,7 @@ bool rb_server::reap(std::unique_lock<std::mutex>& lk) {
         inflight_bytes -= F.footprint;
         (void)hipEventDestroy(F.done);
         F.batch.reset();
+        F.fft.reset();
         free_streams.push_back(F.stream);
         flights.erase(flights.begin() + f);
         any = true;
@@ -172,6 +186,8 @@ bool rb_server::launch(std::unique_lock<std::mutex>& lk) {
     double work = 0.0;
     while (!pending.empty() && int(take.size()) < cfg.max_batch_requests) {
         Job* j = pending.front();
+        // a long-path request runs alone; it closes the batch before it
+        if (j->long_path && !take.empty()) break;
         if (inflight_bytes + bytes + j->footprint > size_t(cfg.max_device_bytes)) break;
         const double w = double(j->req.M) * j->req.N;
         if (!take.empty() && cfg.max_batch_path_steps > 0.0 && work + w > cfg.max_batch_path_steps) break;
@@ -179,6 +195,7 @@ bool rb_server::launch(std::unique_lock<std::mutex>& lk) {
         bytes += j->footprint;
         work += w;
         pending.pop_front();
+        if (j->long_path) break;
     }
     if (take.empty()) return false;
     inflight_bytes += bytes;
@@ -194,13 +211,34 @@ bool rb_server::launch(std::unique_lock<std::mutex>& lk) {
     lk.unlock();
     std::string err;
     int rc = F->stream ? ST_OK : ST_ERR_DEVICE;
-    std::vector<const FusedHostReq*> hr;
-    for (Job* j : take) hr.push_back(&j->fh);
-    F->batch.reset(new FusedBatch());
-    const int mode = fused_mode_of(cfg.precision);
-    if (!rc) rc = F->batch->build(hr, mode, device, F->stream, err);
-    if (!rc) rc = F->batch->launch(F->stream, err);
-    const int total = rc ? 0 : F->batch->total_out;
+    int total = 0;
+    double* d_price = nullptr;
+    double* d_iv = nullptr;
+    double* d_stat = nullptr;
+    if (take[0]->long_path) {
+        F->fft.reset(new FftFused());
+        if (!rc) rc = F->fft->build(take[0]->req, take[0]->lay, device, F->stream, err);
+        if (!rc) rc = F->fft->launch(F->stream, err);
+        if (!rc) {
+            total = F->fft->total_out;
+            d_price = F->fft->d_price;
+            d_iv = F->fft->d_iv;
+            d_stat = F->fft->d_stat;
+        }
+    } else {
+        std::vector<const FusedHostReq*> hr;
+        for (Job* j : take) hr.push_back(&j->fh);
+        F->batch.reset(new FusedBatch());
+        const int mode = fused_mode_of(cfg.precision);
+        if (!rc) rc = F->batch->build(hr, mode, device, F->stream, err);
+        if (!rc) rc = F->batch->launch(F->stream, err);
+        if (!rc) {
+            total = F->batch->total_out;
+            d_price = F->batch->d_price;
+            d_iv = F->batch->d_iv;
+            d_stat = F->batch->d_stat;
+        }
+    }
     if (!rc) {
         auto& pb = pinned[F->stream];  // only the server thread touches the pool
         const size_t need = size_t(std::max(1, 3 * total));
@@ -216,9 +254,9 @@ bool rb_server::launch(std::unique_lock<std::mutex>& lk) {
     }
     if (!rc) {
         const size_t n = size_t(total) * sizeof(double);
-        if (hipMemcpyAsync(F->pinned, F->batch->d_price, n, hipMemcpyDeviceToHost, F->stream) ||
-            hipMemcpyAsync(F->pinned + total, F->batch->d_iv, n, hipMemcpyDeviceToHost, F->stream) ||
-            hipMemcpyAsync(F->pinned + 2 * total, F->batch->d_stat, n, hipMemcpyDeviceToHost, F->stream) ||
+        if (hipMemcpyAsync(F->pinned, d_price, n, hipMemcpyDeviceToHost, F->stream) ||
+            hipMemcpyAsync(F->pinned + total, d_iv, n, hipMemcpyDeviceToHost, F->stream) ||
+            hipMemcpyAsync(F->pinned + 2 * total, d_stat, n, hipMemcpyDeviceToHost, F->stream) ||
             hipEventCreateWithFlags(&F->done, hipEventDisableTiming) ||
             hipEventRecord(F->done, F->stream))
             rc = ST_ERR_DEVICE;
@@ -234,6 +272,7 @@ bool rb_server::launch(std::unique_lock<std::mutex>& lk) {
         if (F->stream) (void)hipStreamSynchronize(F->stream);
         if (F->done) (void)hipEventDestroy(F->done);
         F->batch.reset();
+        F->fft.reset();
         if (F->stream) free_streams.push_back(F->stream);
         fatal = err;
         cv_done.notify_all();
@@ -282,12 +321,35 @@ int rb_price_batch(const rb_request* reqs, int n, const rb_config* cfg) {
         set_last_error("no HIP device");
         return RB_ERR_DEVICE;
     }
+    // the requests up to N = 1024 in one launch, each longer one in a launch of its own
     std::vector<const FusedHostReq*> hr;
-    for (auto& j : jobs) hr.push_back(&j->fh);
+    std::vector<int> short_idx;
+    for (int i = 0; i < n; ++i)
+        if (!jobs[i]->long_path) {
+            hr.push_back(&jobs[i]->fh);
+            short_idx.push_back(i);
+        }
     FusedBatch B;
-    int rc = B.build(hr, fused_mode_of(cfg->precision), device, st, err);
-    if (!rc) rc = B.launch(st, err);
-    for (int i = 0; !rc && i < n; ++i) rc = B.results(i, reqs[i].price, reqs[i].iv, reqs[i].stat, st, err);
+    int rc = RB_OK;
+    if (!hr.empty()) {
+        rc = B.build(hr, fused_mode_of(cfg->precision), device, st, err);
+        if (!rc) rc = B.launch(st, err);
+        for (size_t k = 0; !rc && k < short_idx.size(); ++k) {
+            const rb_request& q = reqs[short_idx[k]];
+            rc = B.results(int(k), q.price, q.iv, q.stat, st, err);
+        }
+    }
+    for (int i = 0; !rc && i < n; ++i) {
+        if (!jobs[i]->long_path) continue;
+        FftFused F;
+        rc = F.build(jobs[i]->req, jobs[i]->lay, device, st, err);
+        if (!rc) rc = F.launch(st, err);
+        if (!rc) rc = F.results(reqs[i].price, reqs[i].iv, reqs[i].stat, st, err);
+        if (!rc && hipStreamSynchronize(st) != hipSuccess) {
+            err = "long-path execution failed";
+            rc = RB_ERR_DEVICE;
+        }
+    }
     if (!rc && hipStreamSynchronize(st) != hipSuccess) {
         err = "batch execution failed";
         rc = RB_ERR_DEVICE;

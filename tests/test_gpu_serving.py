@@ -103,3 +103,24 @@ def test_server_rejects_oversize_request(rb):
         with pytest.raises(rb.RbError) as e:
             srv.submit(big)
     assert e.value.code == 5
+
+
+def test_long_path_requests_in_batches_and_server(rb):
+    # requests beyond N = 1024 run the long-path kernel in a launch of their own, between the
+    # batched short ones; every result equals pricing the request alone
+    from rbergomi_amd.serving import Request, Server, price_batch
+    reqs = make_requests(6, seed=9)
+    reqs.insert(2, Request(0.05, [0.1, 0.2], [1.5, 1.1], [-0.7, -0.5], [0.5, 1.0], [1.0, 1.1], 2048, 3001, [2, 2048]))
+    reqs.append(Request(0.03, [0.05], [2.3], [-0.9], [1.0], [0.9], 1500, 777, [3, 1500]))
+    price_batch(reqs, precision=rb.PREC_SPLIT_FP16)
+    for r in reqs:
+        p, iv, st = alone(rb, r, rb.PREC_SPLIT_FP16)
+        assert np.array_equal(p, r.price) and np.array_equal(st, r.stat)
+    fresh = [Request(r.xi, r.H, r.eta, r.rho, r.T, r.K, r.N, r.M, r.seed) for r in reqs]
+    with Server(rb.PREC_SPLIT_FP16, max_inflight=2, max_batch_requests=4) as srv:
+        for r in fresh:
+            srv.submit(r)
+        for r in fresh:
+            srv.wait(r)
+    for r, q in zip(fresh, reqs):
+        assert np.array_equal(r.price, q.price) and np.array_equal(r.stat, q.stat)
