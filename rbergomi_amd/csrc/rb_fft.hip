@@ -4,11 +4,13 @@
 // for N > 1024, where the Toeplitz GEMM's W1 tile no longer fits LDS.  One workgroup convolves
 // two paths at once: z = W1_a + i W1_b, zero-padded to L = 2^k >= 2N-1.  Because Gamma is real,
 // IFFT(FFT(z) * FFT(Gamma)) = (Gamma * W1_a) + i (Gamma * W1_b), so no separation step is
-// needed.  Forward transform: radix-4 decimation in time from bit-reversed input to natural
-// order; pointwise product with the per-H table FFT(Gamma) (computed in FP64 on the host);
-// inverse: radix-4 decimation in frequency from natural order to bit-reversed output, which the
-// epilogue reads back at bit-reversed indices.  Twiddles exp(-2 pi i k / L) come from an FP64
-// table rounded to FP32.  Epilogue in FP64: Wtilde = sc (Y + a W1 + b W1perp).
+// needed.  Forward transform: decimation in time from bit-reversed input to natural order;
+// pointwise product with the per-H table FFT(Gamma) (computed in FP64 on the host); inverse:
+// decimation in frequency from natural order to bit-reversed output, which the epilogue reads
+// back at bit-reversed indices.  L = 2^12..2^14 take radix-16 passes with the last forward pass,
+// the product and the first inverse pass in one; shorter transforms radix-4 passes.  Twiddles
+// exp(-2 pi i k / L) come from an FP64 table rounded to FP32.  Epilogue in FP64:
+// Wtilde = sc (Y + a W1 + b W1perp).
 #include <hip/hip_runtime.h>
 
 #include "rb_fft_lds.h"
@@ -38,18 +40,21 @@ k_volterra_fft(const double* __restrict__ Z, long zstride, long np, int N, int l
         z[fpad(int(__brev(unsigned(i)) >> (32 - log2L)))] = make_float2(re, im);
     }
     __syncthreads();
-    if constexpr (LOG2L > 0)
-        fft_dit_r4_t<LOG2L>(z, tw, threadIdx.x);
-    else
-        fft_dit_r4(z, tw, log2L, threadIdx.x);
-    // pointwise product with FFT(Gamma) of this unit
-    const float2* G = ghat + long(u) * L;
-    for (int k = threadIdx.x; k < L; k += 256) z[fpad(k)] = cmulf(z[fpad(k)], G[k]);
-    __syncthreads();
-    if constexpr (LOG2L > 0)
-        fft_dif_inv_r4_t<LOG2L>(z, tw, threadIdx.x);
-    else
-        fft_dif_inv_r4(z, tw, log2L, threadIdx.x);
+    const float2* G = ghat + long(u) * L;  // FFT(Gamma) of this unit
+    if constexpr (LOG2L >= 12) {
+        fft_convolve_r16_t<LOG2L>(z, tw, G, threadIdx.x);
+    } else {
+        if constexpr (LOG2L > 0)
+            fft_dit_r4_t<LOG2L>(z, tw, threadIdx.x);
+        else
+            fft_dit_r4(z, tw, log2L, threadIdx.x);
+        for (int k = threadIdx.x; k < L; k += 256) z[fpad(k)] = cmulf(z[fpad(k)], G[k]);
+        __syncthreads();
+        if constexpr (LOG2L > 0)
+            fft_dif_inv_r4_t<LOG2L>(z, tw, threadIdx.x);
+        else
+            fft_dif_inv_r4(z, tw, log2L, threadIdx.x);
+    }
     const Unit un = units[u];
     const double invL = 1.0 / double(L);
     for (int i = threadIdx.x; i < N; i += 256) {

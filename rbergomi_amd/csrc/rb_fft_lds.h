@@ -7,11 +7,11 @@ namespace rb {
 
 // LDS slot of element i: one pad slot after every 32 elements, so the power-of-two strides of
 // the butterflies and of the bit-reversed loads spread over the banks
-__device__ __forceinline__ int fpad(int i) { return i + (i >> 5); }
+__host__ __device__ __forceinline__ int fpad(int i) { return i + (i >> 5); }
 // LDS slots for L elements
 inline size_t fft_lds_slots(int log2L) { return (size_t(1) << log2L) + (size_t(1) << log2L >> 5); }
 
-__device__ __forceinline__ float2 cmulf(float2 a, float2 b) {
+__host__ __device__ __forceinline__ float2 cmulf(float2 a, float2 b) {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
@@ -204,6 +204,228 @@ template <int LOG2L>
 __device__ __forceinline__ void fft_dif_inv_r4_t(float2* z, const float2* __restrict__ tw, int tid) {
     static_assert(LOG2L >= 10, "at least 256 radix-4 butterflies per pass");
     fft_dif_pass<LOG2L, LOG2L - 1>(z, tw, tid);
+}
+
+// ---- radix 2^R passes (R <= 4): the 2^R points of a butterfly group stay in registers for R
+// radix-2 stages, so a transform of L = 2^12..2^14 takes 3-4 LDS passes and barriers instead of
+// 6-7.  Element i0 + k span of a group sits in slot fpad(i0) + k span + (k span) / 32 for every
+// power-of-two span (a group never straddles a pad in a way that breaks this).
+
+// Forward group t of stages S .. S + R - 1 (decimation in time): stage S + r pairs points k and
+// k + 2^r of the group (bit r of k clear) with the twiddle W_{2 span 2^r}^{pos + span m},
+// m = k mod 2^r; the upper half of those twiddles is -i times the lower half.
+template <int LOG2L, int S, int R>
+__host__ __device__ __forceinline__ void fft_dit_group(float2* z, const float2* __restrict__ tw, int t) {
+    constexpr int span = 1 << S, G = 1 << R;
+    const int pos = t & (span - 1);
+    const int q = fpad(((t >> S) << (S + R)) + pos);
+    float2 x[G];
+#pragma unroll
+    for (int k = 0; k < G; ++k) x[k] = z[q + k * span + ((k * span) >> 5)];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int h = 1 << r;
+        float2 w[8];
+        if (S == 0 && r <= 1) {  // W_2^0 = 1, W_4^1 = -i
+            w[0] = make_float2(1.f, 0.f);
+            w[1] = make_float2(0.f, -1.f);
+        } else {
+#pragma unroll
+            for (int m = 0; m < (h > 1 ? h / 2 : 1); ++m) w[m] = tw[(pos + span * m) << (LOG2L - S - r - 1)];
+#pragma unroll
+            for (int m = h / 2; m < h && h > 1; ++m) w[m] = make_float2(w[m - h / 2].y, -w[m - h / 2].x);
+        }
+#pragma unroll
+        for (int k = 0; k < G; ++k) {
+            if (k & h) continue;
+            const int m = k & (h - 1);
+            float2 y;
+            if (S == 0 && r == 0)
+                y = x[k + h];
+            else if (S == 0 && r == 1 && m == 1)
+                y = make_float2(x[k + h].y, -x[k + h].x);
+            else
+                y = cmulf(w[m], x[k + h]);
+            x[k + h] = make_float2(x[k].x - y.x, x[k].y - y.y);
+            x[k] = make_float2(x[k].x + y.x, x[k].y + y.y);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < G; ++k) z[q + k * span + ((k * span) >> 5)] = x[k];
+}
+
+// Inverse group t of stages S down to S - R + 1 (decimation in frequency, conjugate twiddles):
+// stage sb + r (sb = S - R + 1, distance span1 2^r, span1 = 2^sb) maps points (k, k + 2^r) to
+// (a + b, (a - b) conj W_{2 span1 2^r}^{pos + span1 m}); the upper half of those twiddles is i
+// times the lower half.
+template <int LOG2L, int S, int R>
+__host__ __device__ __forceinline__ void fft_dif_group(float2* z, const float2* __restrict__ tw, int t) {
+    constexpr int sb = S - R + 1, span1 = 1 << sb, G = 1 << R;
+    const int pos = t & (span1 - 1);
+    const int q = fpad(((t >> sb) << (S + 1)) + pos);
+    float2 x[G];
+#pragma unroll
+    for (int k = 0; k < G; ++k) x[k] = z[q + k * span1 + ((k * span1) >> 5)];
+#pragma unroll
+    for (int r = R - 1; r >= 0; --r) {
+        const int h = 1 << r;
+        float2 w[8];
+        if (sb == 0 && r <= 1) {  // conj W_2^0 = 1, conj W_4^1 = i
+            w[0] = make_float2(1.f, 0.f);
+            w[1] = make_float2(0.f, 1.f);
+        } else {
+#pragma unroll
+            for (int m = 0; m < (h > 1 ? h / 2 : 1); ++m) {
+                const float2 c = tw[(pos + span1 * m) << (LOG2L - sb - r - 1)];
+                w[m] = make_float2(c.x, -c.y);
+            }
+#pragma unroll
+            for (int m = h / 2; m < h && h > 1; ++m) w[m] = make_float2(-w[m - h / 2].y, w[m - h / 2].x);
+        }
+#pragma unroll
+        for (int k = 0; k < G; ++k) {
+            if (k & h) continue;
+            const int m = k & (h - 1);
+            const float2 a = x[k], b = x[k + h];
+            const float2 d = make_float2(a.x - b.x, a.y - b.y);
+            x[k] = make_float2(a.x + b.x, a.y + b.y);
+            if (sb == 0 && r == 0)
+                x[k + h] = d;
+            else if (sb == 0 && r == 1 && m == 1)
+                x[k + h] = make_float2(-d.y, d.x);
+            else
+                x[k + h] = cmulf(d, w[m]);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < G; ++k) z[q + k * span1 + ((k * span1) >> 5)] = x[k];
+}
+
+// Stages S .. LOG2L - 1 of the forward transform: radix-16 passes, the remainder (radix 2, 4 or
+// 8) last, where the spans are largest
+template <int LOG2L, int S>
+__device__ __forceinline__ void fft_dit16_pass(float2* z, const float2* __restrict__ tw, int tid) {
+    if constexpr (S < LOG2L) {
+        constexpr int R = LOG2L - S >= 4 ? 4 : LOG2L - S;
+        constexpr int NG = (1 << LOG2L) >> R;
+        static_assert(NG % 256 == 0, "whole groups per thread");
+#pragma unroll 2
+        for (int u = 0; u < NG / 256; ++u) fft_dit_group<LOG2L, S, R>(z, tw, tid + 256 * u);
+        __syncthreads();
+        fft_dit16_pass<LOG2L, S + R>(z, tw, tid);
+    }
+}
+
+// Stages S down to 0 of the inverse transform: the remainder first, then radix-16 passes
+template <int LOG2L, int S>
+__device__ __forceinline__ void fft_dif16_pass(float2* z, const float2* __restrict__ tw, int tid) {
+    if constexpr (S >= 0) {
+        constexpr int R = (S + 1) % 4 == 0 ? 4 : (S + 1) % 4;
+        constexpr int NG = (1 << LOG2L) >> R;
+        static_assert(NG % 256 == 0, "whole groups per thread");
+#pragma unroll 2
+        for (int u = 0; u < NG / 256; ++u) fft_dif_group<LOG2L, S, R>(z, tw, tid + 256 * u);
+        __syncthreads();
+        fft_dif16_pass<LOG2L, S - R>(z, tw, tid);
+    }
+}
+
+// Middle group t of a convolution: forward stages S .. S + R - 1 (the last forward pass), the
+// product with G at the natural indices the group then holds, and inverse stages S + R - 1 .. S
+// (the first inverse pass, which has the same groups): two LDS passes and two barriers fewer.
+template <int LOG2L, int S, int R>
+__host__ __device__ __forceinline__ void fft_mid_group(float2* z, const float2* __restrict__ tw,
+                                                       const float2* __restrict__ G, int t) {
+    constexpr int span = 1 << S, NP = 1 << R;
+    const int pos = t & (span - 1);
+    const int i0 = ((t >> S) << (S + R)) + pos;
+    const int q = fpad(i0);
+    float2 x[NP];
+#pragma unroll
+    for (int k = 0; k < NP; ++k) x[k] = z[q + k * span + ((k * span) >> 5)];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {  // forward, as fft_dit_group (S >= 4 here: no unit twiddles)
+        const int h = 1 << r;
+        float2 w[8];
+#pragma unroll
+        for (int m = 0; m < (h > 1 ? h / 2 : 1); ++m) w[m] = tw[(pos + span * m) << (LOG2L - S - r - 1)];
+#pragma unroll
+        for (int m = h / 2; m < h && h > 1; ++m) w[m] = make_float2(w[m - h / 2].y, -w[m - h / 2].x);
+#pragma unroll
+        for (int k = 0; k < NP; ++k) {
+            if (k & h) continue;
+            const float2 y = cmulf(w[k & (h - 1)], x[k + h]);
+            x[k + h] = make_float2(x[k].x - y.x, x[k].y - y.y);
+            x[k] = make_float2(x[k].x + y.x, x[k].y + y.y);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NP; ++k) x[k] = cmulf(x[k], G[i0 + k * span]);
+#pragma unroll
+    for (int r = R - 1; r >= 0; --r) {  // inverse, as fft_dif_group with sb = S
+        const int h = 1 << r;
+        float2 w[8];
+#pragma unroll
+        for (int m = 0; m < (h > 1 ? h / 2 : 1); ++m) {
+            const float2 c = tw[(pos + span * m) << (LOG2L - S - r - 1)];
+            w[m] = make_float2(c.x, -c.y);
+        }
+#pragma unroll
+        for (int m = h / 2; m < h && h > 1; ++m) w[m] = make_float2(-w[m - h / 2].y, w[m - h / 2].x);
+#pragma unroll
+        for (int k = 0; k < NP; ++k) {
+            if (k & h) continue;
+            const float2 a = x[k], b = x[k + h];
+            x[k] = make_float2(a.x + b.x, a.y + b.y);
+            x[k + h] = cmulf(make_float2(a.x - b.x, a.y - b.y), w[k & (h - 1)]);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NP; ++k) z[q + k * span + ((k * span) >> 5)] = x[k];
+}
+
+// Circular convolution with the transform G of a real kernel: z (bit-reversed order) ->
+// forward radix-16 passes -> product with G -> inverse passes -> z (bit-reversed order,
+// unnormalised); the last forward and first inverse passes run as one (fft_mid_group)
+template <int LOG2L, int S>
+__device__ __forceinline__ void fft_conv16_pass(float2* z, const float2* __restrict__ tw,
+                                                const float2* __restrict__ G, int tid) {
+    constexpr int R = LOG2L - S >= 4 ? 4 : LOG2L - S;
+    constexpr int NG = (1 << LOG2L) >> R;
+    static_assert(NG % 256 == 0, "whole groups per thread");
+    if constexpr (S + R == LOG2L) {
+        static_assert(S % 4 == 0 && S >= 4, "radix-16 passes below the middle one");
+#pragma unroll 2
+        for (int u = 0; u < NG / 256; ++u) fft_mid_group<LOG2L, S, R>(z, tw, G, tid + 256 * u);
+        __syncthreads();
+        fft_dif16_pass<LOG2L, S - 1>(z, tw, tid);
+    } else {
+#pragma unroll 2
+        for (int u = 0; u < NG / 256; ++u) fft_dit_group<LOG2L, S, R>(z, tw, tid + 256 * u);
+        __syncthreads();
+        fft_conv16_pass<LOG2L, S + R>(z, tw, G, tid);
+    }
+}
+
+template <int LOG2L>
+__device__ __forceinline__ void fft_convolve_r16_t(float2* z, const float2* __restrict__ tw,
+                                                   const float2* __restrict__ G, int tid) {
+    static_assert(LOG2L >= 12, "at least 256 radix-16 groups per pass");
+    fft_conv16_pass<LOG2L, 0>(z, tw, G, tid);
+}
+
+// The transforms of fft_dit_r4_t / fft_dif_inv_r4_t with radix-16 passes (L >= 2^12: at least
+// one 16-point group per thread)
+template <int LOG2L>
+__device__ __forceinline__ void fft_dit_r16_t(float2* z, const float2* __restrict__ tw, int tid) {
+    static_assert(LOG2L >= 12, "at least 256 radix-16 groups per pass");
+    fft_dit16_pass<LOG2L, 0>(z, tw, tid);
+}
+
+template <int LOG2L>
+__device__ __forceinline__ void fft_dif_inv_r16_t(float2* z, const float2* __restrict__ tw, int tid) {
+    static_assert(LOG2L >= 12, "at least 256 radix-16 groups per pass");
+    fft_dif16_pass<LOG2L, LOG2L - 1>(z, tw, tid);
 }
 
 }  // namespace rb

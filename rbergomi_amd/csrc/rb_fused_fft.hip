@@ -6,10 +6,11 @@
 //   1. Philox4x32-10 W1 of both paths (FP32 Box-Muller, stream 0), kept in registers: thread t
 //      owns the 4-step quads q = t + 256 j of both paths (genGaussianMT,
 //      rBergLib/rBergomiMT.cpp:169-172).
-//   2. Per unit (H): z = W1_a + i W1_b zero-padded to L = 2^k >= 2N - 1 in LDS, radix-4 FFT,
-//      times FFT(a, Gamma_1, ..., Gamma_{N-1}) (the first-cell weight a folded in), inverse FFT:
-//      Gamma is real, so the real and imaginary parts are the two paths' convolutions
-//      (updateWtilde's FFTW convolution, :83-109).
+//   2. Per unit (H): z = W1_a + i W1_b zero-padded to L = 2^k >= 2N - 1 in LDS, radix-16 FFT,
+//      times FFT(a, Gamma_1, ..., Gamma_{N-1}) (the first-cell weight a folded in), inverse FFT,
+//      the product fused into the pass between the transforms: Gamma is real, so the real and
+//      imaginary parts are the two paths' convolutions (updateWtilde's FFTW convolution,
+//      :83-109).
 //   3. Wtilde = sc (Y + b W1perp) for the thread's steps, W1perp regenerated from the same
 //      counters (stream 1), written back to LDS in step order; then per (H, eta, T) group
 //      sqrt(v_i) = exp2(c1 Wtilde_{i-1} - al P_i - be) and the sums of v and sqrt(v) W1 over
@@ -109,11 +110,7 @@ __global__ void __launch_bounds__(256, (QPT >= 8 || (MC && QPT >= 4)) ? 1 : 2) k
                 }
                 for (int i = 4 * nq + tid; i < L; i += 256) z[fpad(int(__brev(unsigned(i)) >> shift))] = make_float2(0.f, 0.f);
                 __syncthreads();
-                fft_dit_r4_t<fft_log2L<QPT>()>(z, A.tw, tid);
-                const float2* G = A.ghat + long(u) * L;
-                for (int k = tid; k < L; k += 256) z[fpad(k)] = cmulf(z[fpad(k)], G[k]);
-                __syncthreads();
-                fft_dif_inv_r4_t<fft_log2L<QPT>()>(z, A.tw, tid);
+                fft_convolve_r16_t<fft_log2L<QPT>()>(z, A.tw, A.ghat + long(u) * L, tid);
                 // ---- 3. Wtilde of the thread's steps (both paths), then back to LDS in step order
                 const float invL = 1.0f / float(L);
                 float wt[2][QPT][4];
