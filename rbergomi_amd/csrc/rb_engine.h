@@ -88,6 +88,12 @@ Layout make_layout(const Request& r);
 PhiloxKey philox_key(const uint64_t* seed, int n);
 // FFT in FP64 on the host (radix-2, a.size() a power of two; sign -1 forward, unnormalised)
 void fft_host(std::vector<std::complex<double>>& a, int sign);
+// i with its low `bits` bits in reverse order
+inline int bit_reverse(int i, int bits) {
+    int r = 0;
+    for (int b = 0; b < bits; ++b) r |= ((i >> b) & 1) << (bits - 1 - b);
+    return r;
+}
 
 // Per-kernel timing accumulated over the launches of one run (HIP events on the run's stream).
 struct KernelTime {

@@ -206,226 +206,198 @@ __device__ __forceinline__ void fft_dif_inv_r4_t(float2* z, const float2* __rest
     fft_dif_pass<LOG2L, LOG2L - 1>(z, tw, tid);
 }
 
-// ---- radix 2^R passes (R <= 4): the 2^R points of a butterfly group stay in registers for R
-// radix-2 stages, so a transform of L = 2^12..2^14 takes 3-4 LDS passes and barriers instead of
-// 6-7.  Element i0 + k span of a group sits in slot fpad(i0) + k span + (k span) / 32 for every
-// power-of-two span (a group never straddles a pad in a way that breaks this).
+// ---- convolution of two paths with radix-16 passes, L = 2^12..2^14 (FP32).  The stages run in
+// passes over groups of 16 points held in registers (3-5 LDS passes and barriers per transform
+// instead of 6-7 for radix 4), and the pass either side of the spectral product runs as one.
+// Element i0 + k span of a group sits in slot fpad(i0) + k span + (k span) / 32 for every
+// power-of-two span.  Two orders:
+//  - natural (fft_convolve_r16): forward by decimation in frequency from natural order, inverse
+//    by decimation in time back to natural order, the spectrum and G in bit-reversed order
+//    between them.  The top stage (distance L/2) of each transform is left to the caller:
+//    forward while the input is written (its upper half is the zero padding, so the stage gives
+//    x_i and x_i W_L^i), inverse while the output is read (y_i = x_i + conj(W_L^i) x_{i + L/2},
+//    only i < L/2 being needed).
+//  - bit-reversed (fft_convolve_r16_br): forward by decimation in time from bit-reversed input to
+//    natural order, G in natural order, inverse by decimation in frequency to bit-reversed output.
 
-// Forward group t of stages S .. S + R - 1 (decimation in time): stage S + r pairs points k and
-// k + 2^r of the group (bit r of k clear) with the twiddle W_{2 span 2^r}^{pos + span m},
-// m = k mod 2^r; the upper half of those twiddles is -i times the lower half.
-template <int LOG2L, int S, int R>
-__host__ __device__ __forceinline__ void fft_dit_group(float2* z, const float2* __restrict__ tw, int t) {
-    constexpr int span = 1 << S, G = 1 << R;
-    const int pos = t & (span - 1);
-    const int q = fpad(((t >> S) << (S + R)) + pos);
-    float2 x[G];
+// Stage SB + r of a group's points (distance 2^r in the group, 2^(SB + r) in the transform),
+// twiddle W = W_{2^(SB + r + 1)}^{pos + 2^SB m}, m = k mod 2^r, conjugated for the inverse (CONJ):
+// decimation in time maps (a, b) to (a + W b, a - W b), decimation in frequency to
+// (a + b, (a - b) W).  The upper half of a stage's twiddles is W_4 = -i (conjugate: +i) times the
+// lower half.
+template <int LOG2L, int SB, int R, int r, bool DIT, bool CONJ>
+__host__ __device__ __forceinline__ void fft_stage(float2 (&x)[1 << R], const float2* __restrict__ tw, int pos) {
+    constexpr int h = 1 << r, G = 1 << R, span = 1 << SB;
+    constexpr int NL = h > 1 ? h / 2 : 1;  // twiddles read from the table
+    float2 w[h];
+    if constexpr (SB == 0 && r == 0) {
+        w[0] = make_float2(1.f, 0.f);
+    } else {
 #pragma unroll
-    for (int k = 0; k < G; ++k) x[k] = z[q + k * span + ((k * span) >> 5)];
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const int h = 1 << r;
-        float2 w[8];
-        if (S == 0 && r <= 1) {  // W_2^0 = 1, W_4^1 = -i
-            w[0] = make_float2(1.f, 0.f);
-            w[1] = make_float2(0.f, -1.f);
-        } else {
-#pragma unroll
-            for (int m = 0; m < (h > 1 ? h / 2 : 1); ++m) w[m] = tw[(pos + span * m) << (LOG2L - S - r - 1)];
-#pragma unroll
-            for (int m = h / 2; m < h && h > 1; ++m) w[m] = make_float2(w[m - h / 2].y, -w[m - h / 2].x);
+        for (int m = 0; m < NL; ++m) {
+            const float2 c = tw[(pos + span * m) << (LOG2L - SB - r - 1)];
+            w[m] = CONJ ? make_float2(c.x, -c.y) : c;
         }
 #pragma unroll
-        for (int k = 0; k < G; ++k) {
-            if (k & h) continue;
-            const int m = k & (h - 1);
-            float2 y;
-            if (S == 0 && r == 0)
-                y = x[k + h];
-            else if (S == 0 && r == 1 && m == 1)
-                y = make_float2(x[k + h].y, -x[k + h].x);
-            else
-                y = cmulf(w[m], x[k + h]);
-            x[k + h] = make_float2(x[k].x - y.x, x[k].y - y.y);
-            x[k] = make_float2(x[k].x + y.x, x[k].y + y.y);
-        }
+        for (int m = NL; m < h; ++m)
+            w[m] = CONJ ? make_float2(-w[m - NL].y, w[m - NL].x) : make_float2(w[m - NL].y, -w[m - NL].x);
     }
 #pragma unroll
-    for (int k = 0; k < G; ++k) z[q + k * span + ((k * span) >> 5)] = x[k];
-}
-
-// Inverse group t of stages S down to S - R + 1 (decimation in frequency, conjugate twiddles):
-// stage sb + r (sb = S - R + 1, distance span1 2^r, span1 = 2^sb) maps points (k, k + 2^r) to
-// (a + b, (a - b) conj W_{2 span1 2^r}^{pos + span1 m}); the upper half of those twiddles is i
-// times the lower half.
-template <int LOG2L, int S, int R>
-__host__ __device__ __forceinline__ void fft_dif_group(float2* z, const float2* __restrict__ tw, int t) {
-    constexpr int sb = S - R + 1, span1 = 1 << sb, G = 1 << R;
-    const int pos = t & (span1 - 1);
-    const int q = fpad(((t >> sb) << (S + 1)) + pos);
-    float2 x[G];
-#pragma unroll
-    for (int k = 0; k < G; ++k) x[k] = z[q + k * span1 + ((k * span1) >> 5)];
-#pragma unroll
-    for (int r = R - 1; r >= 0; --r) {
-        const int h = 1 << r;
-        float2 w[8];
-        if (sb == 0 && r <= 1) {  // conj W_2^0 = 1, conj W_4^1 = i
-            w[0] = make_float2(1.f, 0.f);
-            w[1] = make_float2(0.f, 1.f);
+    for (int k = 0; k < G; ++k) {
+        if (k & h) continue;
+        const int m = k & (h - 1);
+        // unit twiddles at the first two stages of a transform (pos = 0): no multiply
+        const bool one = SB == 0 && m == 0 && r <= 1;
+        const bool rot = SB == 0 && m == 1 && r == 1;
+        const float2 a = x[k], b = x[k + h];
+        if constexpr (DIT) {
+            float2 y;
+            if (one)
+                y = b;
+            else if (rot)
+                y = CONJ ? make_float2(-b.y, b.x) : make_float2(b.y, -b.x);
+            else
+                y = cmulf(w[m], b);
+            x[k] = make_float2(a.x + y.x, a.y + y.y);
+            x[k + h] = make_float2(a.x - y.x, a.y - y.y);
         } else {
-#pragma unroll
-            for (int m = 0; m < (h > 1 ? h / 2 : 1); ++m) {
-                const float2 c = tw[(pos + span1 * m) << (LOG2L - sb - r - 1)];
-                w[m] = make_float2(c.x, -c.y);
-            }
-#pragma unroll
-            for (int m = h / 2; m < h && h > 1; ++m) w[m] = make_float2(-w[m - h / 2].y, w[m - h / 2].x);
-        }
-#pragma unroll
-        for (int k = 0; k < G; ++k) {
-            if (k & h) continue;
-            const int m = k & (h - 1);
-            const float2 a = x[k], b = x[k + h];
             const float2 d = make_float2(a.x - b.x, a.y - b.y);
             x[k] = make_float2(a.x + b.x, a.y + b.y);
-            if (sb == 0 && r == 0)
+            if (one)
                 x[k + h] = d;
-            else if (sb == 0 && r == 1 && m == 1)
-                x[k + h] = make_float2(-d.y, d.x);
+            else if (rot)
+                x[k + h] = CONJ ? make_float2(-d.y, d.x) : make_float2(d.y, -d.x);
             else
                 x[k + h] = cmulf(d, w[m]);
         }
     }
-#pragma unroll
-    for (int k = 0; k < G; ++k) z[q + k * span1 + ((k * span1) >> 5)] = x[k];
 }
 
-// Stages S .. LOG2L - 1 of the forward transform: radix-16 passes, the remainder (radix 2, 4 or
-// 8) last, where the spans are largest
-template <int LOG2L, int S>
-__device__ __forceinline__ void fft_dit16_pass(float2* z, const float2* __restrict__ tw, int tid) {
-    if constexpr (S < LOG2L) {
-        constexpr int R = LOG2L - S >= 4 ? 4 : LOG2L - S;
-        constexpr int NG = (1 << LOG2L) >> R;
-        static_assert(NG % 256 == 0, "whole groups per thread");
-#pragma unroll 2
-        for (int u = 0; u < NG / 256; ++u) fft_dit_group<LOG2L, S, R>(z, tw, tid + 256 * u);
-        __syncthreads();
-        fft_dit16_pass<LOG2L, S + R>(z, tw, tid);
-    }
+// stages SB .. SB + R - 1 of a group: decimation in time upwards, in frequency downwards
+template <int LOG2L, int SB, int R, bool CONJ>
+__host__ __device__ __forceinline__ void fft_stages_dit(float2 (&x)[1 << R], const float2* __restrict__ tw, int pos) {
+    fft_stage<LOG2L, SB, R, 0, true, CONJ>(x, tw, pos);
+    if constexpr (R > 1) fft_stage<LOG2L, SB, R, 1, true, CONJ>(x, tw, pos);
+    if constexpr (R > 2) fft_stage<LOG2L, SB, R, 2, true, CONJ>(x, tw, pos);
+    if constexpr (R > 3) fft_stage<LOG2L, SB, R, 3, true, CONJ>(x, tw, pos);
+}
+template <int LOG2L, int SB, int R, bool CONJ>
+__host__ __device__ __forceinline__ void fft_stages_dif(float2 (&x)[1 << R], const float2* __restrict__ tw, int pos) {
+    if constexpr (R > 3) fft_stage<LOG2L, SB, R, 3, false, CONJ>(x, tw, pos);
+    if constexpr (R > 2) fft_stage<LOG2L, SB, R, 2, false, CONJ>(x, tw, pos);
+    if constexpr (R > 1) fft_stage<LOG2L, SB, R, 1, false, CONJ>(x, tw, pos);
+    fft_stage<LOG2L, SB, R, 0, false, CONJ>(x, tw, pos);
 }
 
-// Stages S down to 0 of the inverse transform: the remainder first, then radix-16 passes
-template <int LOG2L, int S>
-__device__ __forceinline__ void fft_dif16_pass(float2* z, const float2* __restrict__ tw, int tid) {
-    if constexpr (S >= 0) {
-        constexpr int R = (S + 1) % 4 == 0 ? 4 : (S + 1) % 4;
-        constexpr int NG = (1 << LOG2L) >> R;
-        static_assert(NG % 256 == 0, "whole groups per thread");
-#pragma unroll 2
-        for (int u = 0; u < NG / 256; ++u) fft_dif_group<LOG2L, S, R>(z, tw, tid + 256 * u);
-        __syncthreads();
-        fft_dif16_pass<LOG2L, S - R>(z, tw, tid);
-    }
-}
+enum FftPassKind {
+    FFT_FWD_DIF,  // forward, decimation in frequency
+    FFT_INV_DIT,  // inverse, decimation in time
+    FFT_MID_NAT,  // forward DIF, product with G, inverse DIT (natural order)
+    FFT_FWD_DIT,  // forward, decimation in time
+    FFT_INV_DIF,  // inverse, decimation in frequency
+    FFT_MID_BR,   // forward DIT, product with G, inverse DIF (bit-reversed order)
+};
 
-// Middle group t of a convolution: forward stages S .. S + R - 1 (the last forward pass), the
-// product with G at the natural indices the group then holds, and inverse stages S + R - 1 .. S
-// (the first inverse pass, which has the same groups): two LDS passes and two barriers fewer.
-template <int LOG2L, int S, int R>
-__host__ __device__ __forceinline__ void fft_mid_group(float2* z, const float2* __restrict__ tw,
-                                                       const float2* __restrict__ G, int t) {
-    constexpr int span = 1 << S, NP = 1 << R;
+// group t of a pass over stages SB .. SB + R - 1
+template <int LOG2L, int SB, int R, int KIND>
+__host__ __device__ __forceinline__ void fft_group(float2* z, const float2* __restrict__ tw,
+                                                   const float2* __restrict__ G, int t) {
+    constexpr int span = 1 << SB, NP = 1 << R;
     const int pos = t & (span - 1);
-    const int i0 = ((t >> S) << (S + R)) + pos;
+    const int i0 = ((t >> SB) << (SB + R)) + pos;
     const int q = fpad(i0);
     float2 x[NP];
 #pragma unroll
     for (int k = 0; k < NP; ++k) x[k] = z[q + k * span + ((k * span) >> 5)];
+    if constexpr (KIND == FFT_FWD_DIF || KIND == FFT_MID_NAT) fft_stages_dif<LOG2L, SB, R, false>(x, tw, pos);
+    if constexpr (KIND == FFT_FWD_DIT || KIND == FFT_MID_BR) fft_stages_dit<LOG2L, SB, R, false>(x, tw, pos);
+    if constexpr (KIND == FFT_MID_NAT || KIND == FFT_MID_BR) {
 #pragma unroll
-    for (int r = 0; r < R; ++r) {  // forward, as fft_dit_group (S >= 4 here: no unit twiddles)
-        const int h = 1 << r;
-        float2 w[8];
-#pragma unroll
-        for (int m = 0; m < (h > 1 ? h / 2 : 1); ++m) w[m] = tw[(pos + span * m) << (LOG2L - S - r - 1)];
-#pragma unroll
-        for (int m = h / 2; m < h && h > 1; ++m) w[m] = make_float2(w[m - h / 2].y, -w[m - h / 2].x);
-#pragma unroll
-        for (int k = 0; k < NP; ++k) {
-            if (k & h) continue;
-            const float2 y = cmulf(w[k & (h - 1)], x[k + h]);
-            x[k + h] = make_float2(x[k].x - y.x, x[k].y - y.y);
-            x[k] = make_float2(x[k].x + y.x, x[k].y + y.y);
-        }
+        for (int k = 0; k < NP; ++k) x[k] = cmulf(x[k], G[i0 + k * span]);
     }
-#pragma unroll
-    for (int k = 0; k < NP; ++k) x[k] = cmulf(x[k], G[i0 + k * span]);
-#pragma unroll
-    for (int r = R - 1; r >= 0; --r) {  // inverse, as fft_dif_group with sb = S
-        const int h = 1 << r;
-        float2 w[8];
-#pragma unroll
-        for (int m = 0; m < (h > 1 ? h / 2 : 1); ++m) {
-            const float2 c = tw[(pos + span * m) << (LOG2L - S - r - 1)];
-            w[m] = make_float2(c.x, -c.y);
-        }
-#pragma unroll
-        for (int m = h / 2; m < h && h > 1; ++m) w[m] = make_float2(-w[m - h / 2].y, w[m - h / 2].x);
-#pragma unroll
-        for (int k = 0; k < NP; ++k) {
-            if (k & h) continue;
-            const float2 a = x[k], b = x[k + h];
-            x[k] = make_float2(a.x + b.x, a.y + b.y);
-            x[k + h] = cmulf(make_float2(a.x - b.x, a.y - b.y), w[k & (h - 1)]);
-        }
-    }
+    if constexpr (KIND == FFT_INV_DIT || KIND == FFT_MID_NAT) fft_stages_dit<LOG2L, SB, R, true>(x, tw, pos);
+    if constexpr (KIND == FFT_INV_DIF || KIND == FFT_MID_BR) fft_stages_dif<LOG2L, SB, R, true>(x, tw, pos);
 #pragma unroll
     for (int k = 0; k < NP; ++k) z[q + k * span + ((k * span) >> 5)] = x[k];
 }
 
-// Circular convolution with the transform G of a real kernel: z (bit-reversed order) ->
-// forward radix-16 passes -> product with G -> inverse passes -> z (bit-reversed order,
-// unnormalised); the last forward and first inverse passes run as one (fft_mid_group)
-template <int LOG2L, int S>
-__device__ __forceinline__ void fft_conv16_pass(float2* z, const float2* __restrict__ tw,
-                                                const float2* __restrict__ G, int tid) {
-    constexpr int R = LOG2L - S >= 4 ? 4 : LOG2L - S;
+// one pass: every group, then a barrier
+template <int LOG2L, int SB, int R, int KIND>
+__device__ __forceinline__ void fft_pass(float2* z, const float2* __restrict__ tw, const float2* __restrict__ G,
+                                         int tid) {
     constexpr int NG = (1 << LOG2L) >> R;
     static_assert(NG % 256 == 0, "whole groups per thread");
-    if constexpr (S + R == LOG2L) {
-        static_assert(S % 4 == 0 && S >= 4, "radix-16 passes below the middle one");
 #pragma unroll 2
-        for (int u = 0; u < NG / 256; ++u) fft_mid_group<LOG2L, S, R>(z, tw, G, tid + 256 * u);
-        __syncthreads();
-        fft_dif16_pass<LOG2L, S - 1>(z, tw, tid);
-    } else {
-#pragma unroll 2
-        for (int u = 0; u < NG / 256; ++u) fft_dit_group<LOG2L, S, R>(z, tw, tid + 256 * u);
-        __syncthreads();
-        fft_conv16_pass<LOG2L, S + R>(z, tw, G, tid);
+    for (int u = 0; u < NG / 256; ++u) fft_group<LOG2L, SB, R, KIND>(z, tw, G, tid + 256 * u);
+    __syncthreads();
+}
+
+// ---- natural order: stages 0 .. LOG2L - 2 in passes, the middle one at the bottom
+template <int LOG2L>
+constexpr int fft_mid_stages() { return (LOG2L - 1) % 4 == 0 ? 4 : (LOG2L - 1) % 4; }
+
+template <int LOG2L, int S>  // inverse passes from stage S up to LOG2L - 2
+__device__ __forceinline__ void fft_nat_inv(float2* z, const float2* __restrict__ tw, int tid) {
+    if constexpr (S <= LOG2L - 2) {
+        fft_pass<LOG2L, S, 4, FFT_INV_DIT>(z, tw, nullptr, tid);
+        fft_nat_inv<LOG2L, S + 4>(z, tw, tid);
     }
 }
 
-template <int LOG2L>
-__device__ __forceinline__ void fft_convolve_r16_t(float2* z, const float2* __restrict__ tw,
-                                                   const float2* __restrict__ G, int tid) {
-    static_assert(LOG2L >= 12, "at least 256 radix-16 groups per pass");
-    fft_conv16_pass<LOG2L, 0>(z, tw, G, tid);
+template <int LOG2L, int S>  // forward passes from stage S down to the middle pass
+__device__ __forceinline__ void fft_nat_fwd(float2* z, const float2* __restrict__ tw, const float2* __restrict__ G,
+                                            int tid) {
+    constexpr int RM = fft_mid_stages<LOG2L>();
+    if constexpr (S >= RM) {
+        fft_pass<LOG2L, S - 3, 4, FFT_FWD_DIF>(z, tw, nullptr, tid);
+        fft_nat_fwd<LOG2L, S - 4>(z, tw, G, tid);
+    } else {
+        static_assert(S == RM - 1, "passes of four stages down to the middle pass");
+        fft_pass<LOG2L, 0, RM, FFT_MID_NAT>(z, tw, G, tid);
+        fft_nat_inv<LOG2L, RM>(z, tw, tid);
+    }
 }
 
-// The transforms of fft_dit_r4_t / fft_dif_inv_r4_t with radix-16 passes (L >= 2^12: at least
-// one 16-point group per thread)
+// Stages 0 .. LOG2L - 2 of the forward transform, the product with G (bit-reversed order) and
+// stages 0 .. LOG2L - 2 of the inverse; z holds the input after the forward top stage and is left
+// ready for the inverse top stage.  Starts after and ends with a barrier.
 template <int LOG2L>
-__device__ __forceinline__ void fft_dit_r16_t(float2* z, const float2* __restrict__ tw, int tid) {
-    static_assert(LOG2L >= 12, "at least 256 radix-16 groups per pass");
-    fft_dit16_pass<LOG2L, 0>(z, tw, tid);
+__device__ __forceinline__ void fft_convolve_r16(float2* z, const float2* __restrict__ tw,
+                                                 const float2* __restrict__ G, int tid) {
+    static_assert(LOG2L >= 12 && LOG2L <= 14, "radix-16 passes of at least 256 groups");
+    fft_nat_fwd<LOG2L, LOG2L - 2>(z, tw, G, tid);
 }
 
+// ---- bit-reversed order: forward passes from stage 0 up, the middle one at the top
+template <int LOG2L, int S>  // inverse passes from stage S down to 0
+__device__ __forceinline__ void fft_br_inv(float2* z, const float2* __restrict__ tw, int tid) {
+    if constexpr (S >= 0) {
+        fft_pass<LOG2L, S - 3, 4, FFT_INV_DIF>(z, tw, nullptr, tid);
+        fft_br_inv<LOG2L, S - 4>(z, tw, tid);
+    }
+}
+
+template <int LOG2L, int SB>  // forward passes from stage SB up to the middle pass
+__device__ __forceinline__ void fft_br_fwd(float2* z, const float2* __restrict__ tw, const float2* __restrict__ G,
+                                           int tid) {
+    constexpr int R = LOG2L - SB >= 4 ? 4 : LOG2L - SB;
+    if constexpr (SB + R == LOG2L) {
+        static_assert(SB % 4 == 0 && SB >= 4, "passes of four stages below the middle pass");
+        fft_pass<LOG2L, SB, R, FFT_MID_BR>(z, tw, G, tid);
+        fft_br_inv<LOG2L, SB - 1>(z, tw, tid);
+    } else {
+        fft_pass<LOG2L, SB, 4, FFT_FWD_DIT>(z, tw, nullptr, tid);
+        fft_br_fwd<LOG2L, SB + 4>(z, tw, G, tid);
+    }
+}
+
+// Circular convolution of z (bit-reversed order in and out, unnormalised) with the spectrum G
+// (natural order).  Starts after and ends with a barrier.
 template <int LOG2L>
-__device__ __forceinline__ void fft_dif_inv_r16_t(float2* z, const float2* __restrict__ tw, int tid) {
-    static_assert(LOG2L >= 12, "at least 256 radix-16 groups per pass");
-    fft_dif16_pass<LOG2L, LOG2L - 1>(z, tw, tid);
+__device__ __forceinline__ void fft_convolve_r16_br(float2* z, const float2* __restrict__ tw,
+                                                    const float2* __restrict__ G, int tid) {
+    static_assert(LOG2L >= 12 && LOG2L <= 14, "radix-16 passes of at least 256 groups");
+    fft_br_fwd<LOG2L, 0>(z, tw, G, tid);
 }
 
 }  // namespace rb

@@ -170,7 +170,8 @@ struct FftFusedArgs {
     const FusedPG* pgs;
     const FusedOption* opts;
     const float* ptab;           // per unit: P_j = (j - off)^(2H)
-    const float2* ghat;          // per unit [L]: FFT of (a, Gamma_1, ..., Gamma_{N-1}, 0, ...)
+    const float2* ghat;          // per unit [L]: FFT of (a, Gamma_1, ..., Gamma_{N-1}, 0, ...),
+                                 // bit-reversed order
     const float2* tw;            // [L/2]: exp(-2 pi i k / L)
     double* partial;             // [out_size][nitems][2]
     int* counter;                // next work item (zeroed before the launch)

@@ -6,11 +6,11 @@
 //   1. Philox4x32-10 W1 of both paths (FP32 Box-Muller, stream 0), kept in registers: thread t
 //      owns the 4-step quads q = t + 256 j of both paths (genGaussianMT,
 //      rBergLib/rBergomiMT.cpp:169-172).
-//   2. Per unit (H): z = W1_a + i W1_b zero-padded to L = 2^k >= 2N - 1 in LDS, radix-16 FFT,
-//      times FFT(a, Gamma_1, ..., Gamma_{N-1}) (the first-cell weight a folded in), inverse FFT,
-//      the product fused into the pass between the transforms: Gamma is real, so the real and
-//      imaginary parts are the two paths' convolutions (updateWtilde's FFTW convolution,
-//      :83-109).
+//   2. Per unit (H): z = W1_a + i W1_b zero-padded to L = 2^k >= 2N - 1 in LDS, radix-16 FFT
+//      (decimation in frequency), times FFT(a, Gamma_1, ..., Gamma_{N-1}) (the first-cell weight
+//      a folded in; bit-reversed order), inverse FFT (decimation in time): Gamma is real, so the
+//      real and imaginary parts are the two paths' convolutions (updateWtilde's FFTW convolution,
+//      :83-109).  The transforms' top stages are done while z is written and read back.
 //   3. Wtilde = sc (Y + b W1perp) for the thread's steps, W1perp regenerated from the same
 //      counters (stream 1), written back to LDS in step order; then per (H, eta, T) group
 //      sqrt(v_i) = exp2(c1 Wtilde_{i-1} - al P_i - be) and the sums of v and sqrt(v) W1 over
@@ -53,8 +53,7 @@ constexpr int fft_log2L() { return QPT == 2 ? 12 : QPT == 4 ? 13 : 14; }
 template <int QPT, bool MC>
 __global__ void __launch_bounds__(256, (QPT >= 8 || (MC && QPT >= 4)) ? 1 : 2) k_fused_fft(FftFusedArgs A) {
     extern __shared__ float2 lds2[];
-    const int L = 1 << A.log2L;
-    const int shift = 32 - A.log2L;
+    const int L = 1 << A.log2L, H2 = L >> 1;
     float2* z = lds2;                                                      // [L + L/32], slot fpad(i)
     double* red = reinterpret_cast<double*>(z + L + (L >> 5));             // [4][FFT_GR][2][FFT_NS]
     PathPG* pgd = reinterpret_cast<PathPG*>(red + 4 * FFT_GR * 2 * FFT_NS);  // [2][pgu]
@@ -99,18 +98,27 @@ __global__ void __launch_bounds__(256, (QPT >= 8 || (MC && QPT >= 4)) ? 1 : 2) k
             }
             for (int u = 0; u < A.nunits; ++u) {
                 const FusedUnit U = A.units[u];
-                // ---- 2. z in bit-reversed order, transform, product, inverse transform
+                // ---- 2. z in natural order through the forward transform's top stage (its upper
+                //         half is the zero padding: z_i = x_i, z_{i + L/2} = x_i W_L^i), the
+                //         remaining stages, product, inverse transform but its top stage
 #pragma unroll
                 for (int j = 0; j < QPT; ++j) {
                     const int q = tid + 256 * j;
                     if (q < nq)
 #pragma unroll
-                        for (int t = 0; t < 4; ++t)
-                            z[fpad(int(__brev(unsigned(4 * q + t)) >> shift))] = make_float2(w1[0][j][t], w1[1][j][t]);
+                        for (int t = 0; t < 4; ++t) {
+                            const int i = 4 * q + t;  // 4 nq <= L/2
+                            const float2 x = make_float2(w1[0][j][t], w1[1][j][t]);
+                            z[fpad(i)] = x;
+                            z[fpad(i + H2)] = cmulf(x, A.tw[i]);
+                        }
                 }
-                for (int i = 4 * nq + tid; i < L; i += 256) z[fpad(int(__brev(unsigned(i)) >> shift))] = make_float2(0.f, 0.f);
+                for (int i = 4 * nq + tid; i < H2; i += 256) {
+                    z[fpad(i)] = make_float2(0.f, 0.f);
+                    z[fpad(i + H2)] = make_float2(0.f, 0.f);
+                }
                 __syncthreads();
-                fft_convolve_r16_t<fft_log2L<QPT>()>(z, A.tw, A.ghat + long(u) * L, tid);
+                fft_convolve_r16<fft_log2L<QPT>()>(z, A.tw, A.ghat + long(u) * L, tid);
                 // ---- 3. Wtilde of the thread's steps (both paths), then back to LDS in step order
                 const float invL = 1.0f / float(L);
                 float wt[2][QPT][4];
@@ -124,7 +132,12 @@ __global__ void __launch_bounds__(256, (QPT >= 8 || (MC && QPT >= 4)) ? 1 : 2) k
 #pragma unroll
                     for (int t = 0; t < 4; ++t) {
                         const int i = 4 * q + t;
-                        const float2 y = i < N ? z[fpad(int(__brev(unsigned(i)) >> shift))] : make_float2(0.f, 0.f);
+                        float2 y = make_float2(0.f, 0.f);
+                        if (i < N) {  // the inverse top stage: z_i + conj(W_L^i) z_{i + L/2}
+                            const float2 lo = z[fpad(i)], hi = z[fpad(i + H2)], w = A.tw[i];
+                            const float2 p = cmulf(make_float2(w.x, -w.y), hi);
+                            y = make_float2(lo.x + p.x, lo.y + p.y);
+                        }
                         wt[0][j][t] = U.sc * fmaf(y.x, invL, U.b * zp[0][t]);
                         wt[1][j][t] = U.sc * fmaf(y.y, invL, U.b * zp[1][t]);
                     }

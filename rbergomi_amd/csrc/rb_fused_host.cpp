@@ -613,7 +613,11 @@ int FftFused::build(const Request& req, const Layout& lay, int device, hipStream
         g[0] = lay.units[u].a;
         for (int k = 1; k < N; ++k) g[k] = lay.gamma[u * N + k];
         fft_host(g, -1);
-        for (int k = 0; k < L; ++k) gh[u * L + k] = make_float2(float(g[k].real()), float(g[k].imag()));
+        // bit-reversed order: the spectrum's order between the device transforms
+        for (int k = 0; k < L; ++k) {
+            const int b = bit_reverse(k, log2L);
+            gh[u * L + k] = make_float2(float(g[b].real()), float(g[b].imag()));
+        }
     }
     FftFusedArgs& a = args;
     a = FftFusedArgs{};
