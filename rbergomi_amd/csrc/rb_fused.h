@@ -15,7 +15,8 @@ struct FusedUnit {
     float b;         // W1perp weight of the first-cell term
     float sc;        // sqrt(2H) N^-H
     int frag_off;    // offset of this unit's A-fragment table, in 16-byte entries
-    int p_off;       // offset of the Np-entry table P_j = (j - off)^(2H), in floats
+    int p_off;       // offset of the Np-entry table P_j = (j - off)^(2H), in floats (the GEMM
+                     // kernel's table holds P_j at j - 1, the long-path kernel's at j)
     int g_begin, g_end;    // groups of this unit (contiguous)
     int pg_begin, pg_end;  // payoff groups (group, rho) of this unit (contiguous)
     int o_begin, o_end;    // options of this unit (contiguous)

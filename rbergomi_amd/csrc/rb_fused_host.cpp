@@ -139,9 +139,10 @@ static void split16(double x, int mode, uint16_t& hi, uint16_t& lo) {
 }
 
 // Units, groups (sorted by unit), payoff groups (group, rho) and options (sorted by unit) of a
-// request for the fused kernels, with the per-unit tables P_j = (j - off)^(2H) of Np entries.
-// pgmax: payoff groups held at once; gmax: most groups of one unit.
-void fused_tables(const Request& req, const Layout& lay, int Np, int frag_stride, FusedHostReq& out,
+// request for the fused kernels, with the per-unit tables P_j = (j - off)^(2H) of Np entries,
+// P_j at index j - pshift (1 <= j < N).  pgmax: payoff groups held at once; gmax: most groups of
+// one unit.
+void fused_tables(const Request& req, const Layout& lay, int Np, int frag_stride, int pshift, FusedHostReq& out,
                   int& pgmax, int& gmax) {
     const int nu = int(lay.units.size());
     out.fu.assign(nu, FusedUnit());
@@ -160,7 +161,8 @@ void fused_tables(const Request& req, const Layout& lay, int Np, int frag_stride
         U.sc = float(lay.units[u].sc);
         U.frag_off = u * frag_stride;
         U.p_off = u * Np;
-        for (int j = 1; j < req.N; ++j) out.ptab[size_t(u) * Np + j] = float(std::pow(double(j) - off, 2.0 * H));
+        for (int j = 1; j < req.N; ++j)
+            out.ptab[size_t(u) * Np + j - pshift] = float(std::pow(double(j) - off, 2.0 * H));
         U.g_begin = int(out.fg.size());
         U.pg_begin = int(out.fpg.size());
         U.o_begin = int(out.fo.size());
@@ -256,7 +258,8 @@ int plan_fused_request(const Request& req, const Layout& lay, int mode, FusedHos
         for (int s = 0; s < FUSED_MAX_UPW; ++s) a.wave_rb[w][s] = s < int(wrb[w].size()) ? wrb[w][s] : -1;
     }
     int pgmax = 1, gmax = 1;
-    fused_tables(req, lay, Np, a.NK * 128, out, pgmax, gmax);
+    // the GEMM kernel reads P_{j} at j - 1: a lane's four rows 8c + 4h + 0..3 are one aligned float4
+    fused_tables(req, lay, Np, a.NK * 128, 1, out, pgmax, gmax);
     const int nu = int(lay.units.size());
     // ---- A fragments: entry (d, lane, slot j) = G[d + m - 8h - j], G[0] = a, G[t>0] = Gamma_t
     out.frag.assign(size_t(nu) * a.NK * 128, uint4());
@@ -565,7 +568,7 @@ int FftFused::plan(const Request& req, const Layout& lay, size_t lds_limit, size
     const int Np = (N + 31) / 32 * 32;
     FusedHostReq tabs;
     int pgmax = 1, gmax = 1;
-    fused_tables(req, lay, Np, 0, tabs, pgmax, gmax);
+    fused_tables(req, lay, Np, 0, 0, tabs, pgmax, gmax);
     FftFusedArgs a{};
     a.log2L = log2L;
     a.out_size = int(lay.options.size());
@@ -598,7 +601,7 @@ int FftFused::build(const Request& req, const Layout& lay, int device, hipStream
     const int Np = (N + 31) / 32 * 32;
     FusedHostReq tabs;
     int pgmax = 1, gmax = 1;
-    fused_tables(req, lay, Np, 0, tabs, pgmax, gmax);
+    fused_tables(req, lay, Np, 0, 0, tabs, pgmax, gmax);
     int pgu = 1;
     for (const FusedUnit& U : tabs.fu) pgu = std::max(pgu, U.pg_end - U.pg_begin);
     // twiddles and FFT(a, Gamma_1, ..., Gamma_{N-1}) per unit, in FP64, stored in FP32

@@ -35,7 +35,7 @@ int plan_fused_request(const Request& req, const Layout& lay, int mode, FusedHos
                        std::string& err, int pb = 0);
 
 // Units, groups, payoff groups, options and P_j tables (Np entries per unit) of a request
-void fused_tables(const Request& req, const Layout& lay, int Np, int frag_stride, FusedHostReq& out,
+void fused_tables(const Request& req, const Layout& lay, int Np, int frag_stride, int pshift, FusedHostReq& out,
                   int& pgmax, int& gmax);
 
 // A long-path request (N > 1024, Romano-Touzi or plain MC) on the single-launch FFT kernel.

@@ -50,6 +50,14 @@ struct SplitT<2> {
 };
 
 // W1 of path column `col`, step 16 ks + kk, from the LDS tile (the value the product used)
+// a float4 from global memory (16-byte aligned); the explicit address space gives a global load
+// where the pointer's provenance is not visible to the compiler
+__device__ __forceinline__ float4 load_global_f4(const float* p) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    const f4 v = *(const __attribute__((address_space(1))) f4*)(p);
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
 template <int MODE>
 __device__ __forceinline__ float tile_w1(const uint4* chunk, int kk, int col) {
     if constexpr (MODE == 2) {
@@ -246,7 +254,7 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
                         }
                     }
                     if (pbw >= 0) {
-                        const float* P = A.ptab + U.p_off;
+                        const float* P = A.ptab + U.p_off;  // P_j at index j - 1
                         const long p = p0 + 32 * pbw + col;
 #pragma unroll
                         for (int s = 0; s < UPW; ++s) {
@@ -281,13 +289,24 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
                                     wq[4 * c + 3] = h == 0 ? y[c] : (c < 3 ? y[c < 3 ? c + 1 : 3] : ze[0]);
                                 }
                             }
+                            // P_j of the lane's rows j = 32 rb + 8 c + 4 h + t + 1, t = 0..3: one
+                            // aligned float4 per c, loaded ahead of the row loop
+                            float pv[16];
+#pragma unroll
+                            for (int c = 0; c < 4; ++c) {
+                                const float4 v = load_global_f4(P + 32 * rb + 8 * c + 4 * h);
+                                pv[4 * c + 0] = v.x;
+                                pv[4 * c + 1] = v.y;
+                                pv[4 * c + 2] = v.z;
+                                pv[4 * c + 3] = v.w;
+                            }
 #pragma unroll
                             for (int r = 0; r < 16; ++r) {
                                 const int j = 32 * rb + mfma32_row(r, lane) + 1;  // v_j uses Wtilde_{j-1}
                                 if (j < A.N) {
                                     const int ks = j >> 4, kk = j & 15;
                                     const float w1 = tile_w1<MODE>(Bf + ((pbw * NK + ks) * 2) * 64, kk, col);
-                                    const float pj = P[j];
+                                    const float pj = pv[r];
                                     const float x = acc[s][r];
 #pragma unroll
                                     for (int gg = 0; gg < GR; ++gg) {
