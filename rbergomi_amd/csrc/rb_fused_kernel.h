@@ -49,7 +49,6 @@ struct SplitT<2> {
     }
 };
 
-// W1 of path column `col`, step 16 ks + kk, from the LDS tile (the value the product used)
 // a float4 from global memory (16-byte aligned); the explicit address space gives a global load
 // where the pointer's provenance is not visible to the compiler
 __device__ __forceinline__ float4 load_global_f4(const float* p) {
@@ -58,6 +57,7 @@ __device__ __forceinline__ float4 load_global_f4(const float* p) {
     return make_float4(v.x, v.y, v.z, v.w);
 }
 
+// W1 of path column `col`, step 16 ks + kk, from the LDS tile (the value the product used)
 template <int MODE>
 __device__ __forceinline__ float tile_w1(const uint4* chunk, int kk, int col) {
     if constexpr (MODE == 2) {
@@ -66,6 +66,29 @@ __device__ __forceinline__ float tile_w1(const uint4* chunk, int kk, int col) {
         typedef typename SplitT<MODE>::T T;
         const uint4* Bp = chunk + 32 * (kk >> 3) + col;
         return float(reinterpret_cast<const T*>(Bp)[kk & 7]) + float(reinterpret_cast<const T*>(Bp + 64)[kk & 7]);
+    }
+}
+
+// W1 of path column `col`, steps 4 Q .. 4 Q + 3, from the LDS tile of path block `blk` (the
+// values the product used; tile_w1 of the same steps): one 8-byte read of hi and one of lo, or
+// one 16-byte read of FP32 operands.  Quad Q sits in chunk Q / 4, lane group (Q mod 4) / 2, half
+// Q mod 2 of the lane's 16 bytes.
+template <int MODE>
+__device__ __forceinline__ void tile_w1_quad(const uint4* Bf, int blk, int NK, int Q, int col, float w[4]) {
+    const uint4* chunk = Bf + ((blk * NK + (Q >> 2)) * 2) * 64 + 32 * ((Q & 3) >> 1) + col;
+    if constexpr (MODE == 2) {
+        const float4 v = __builtin_bit_cast(float4, chunk[64 * (Q & 1)]);
+        w[0] = v.x;
+        w[1] = v.y;
+        w[2] = v.z;
+        w[3] = v.w;
+    } else {
+        typedef typename SplitT<MODE>::T T;
+        typedef T T4 __attribute__((ext_vector_type(4)));
+        const T4 hi = __builtin_bit_cast(T4, reinterpret_cast<const uint2*>(chunk)[Q & 1]);
+        const T4 lo = __builtin_bit_cast(T4, reinterpret_cast<const uint2*>(chunk + 64)[Q & 1]);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) w[t] = float(hi[t]) + float(lo[t]);
     }
 }
 
@@ -289,9 +312,11 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
                                     wq[4 * c + 3] = h == 0 ? y[c] : (c < 3 ? y[c < 3 ? c + 1 : 3] : ze[0]);
                                 }
                             }
-                            // P_j of the lane's rows j = 32 rb + 8 c + 4 h + t + 1, t = 0..3: one
-                            // aligned float4 per c, loaded ahead of the row loop
-                            float pv[16];
+                            // P_j and W1_j of the lane's rows j = 32 rb + 8 c + 4 h + t + 1, t = 0..3,
+                            // read ahead of the row loop: P as one aligned float4 per c, W1 as
+                            // elements 1..3 of quad 8 rb + 2 c + h and element 0 of the next quad
+                            // (past the tile only for rows j >= Np, which the loop skips)
+                            float pv[16], wv[16];
 #pragma unroll
                             for (int c = 0; c < 4; ++c) {
                                 const float4 v = load_global_f4(P + 32 * rb + 8 * c + 4 * h);
@@ -299,13 +324,20 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
                                 pv[4 * c + 1] = v.y;
                                 pv[4 * c + 2] = v.z;
                                 pv[4 * c + 3] = v.w;
+                                const int Q = 8 * rb + 2 * c + h;
+                                float qa[4], qb[4];
+                                tile_w1_quad<MODE>(Bf, pbw, NK, Q, col, qa);
+                                tile_w1_quad<MODE>(Bf, pbw, NK, Q + 1 < 4 * NK ? Q + 1 : Q, col, qb);
+                                wv[4 * c + 0] = qa[1];
+                                wv[4 * c + 1] = qa[2];
+                                wv[4 * c + 2] = qa[3];
+                                wv[4 * c + 3] = qb[0];
                             }
 #pragma unroll
                             for (int r = 0; r < 16; ++r) {
                                 const int j = 32 * rb + mfma32_row(r, lane) + 1;  // v_j uses Wtilde_{j-1}
                                 if (j < A.N) {
-                                    const int ks = j >> 4, kk = j & 15;
-                                    const float w1 = tile_w1<MODE>(Bf + ((pbw * NK + ks) * 2) * 64, kk, col);
+                                    const float w1 = wv[r];
                                     const float pj = pv[r];
                                     const float x = acc[s][r];
 #pragma unroll
