@@ -199,9 +199,9 @@ void fused_tables(const Request& req, const Layout& lay, int Np, int frag_stride
         U.g_end = int(out.fg.size());
         U.pg_end = int(out.fpg.size());
         U.o_end = int(out.fo.size());
-        // payoff groups held at once: those of one pass over FUSED_GREG groups (Romano-Touzi);
-        // plain Monte Carlo passes over 6 or 8 groups, so every window of FUSED_GREG groups
-        for (int g0 = U.g_begin; g0 < U.g_end; g0 += req.method == METHOD_MC ? 1 : FUSED_GREG) {
+        // payoff groups held at once: those of one pass, over 6 or 8 groups, so every window of
+        // FUSED_GREG groups
+        for (int g0 = U.g_begin; g0 < U.g_end; ++g0) {
             const int g1 = std::min(U.g_end, g0 + FUSED_GREG) - 1;
             pgmax = std::max(pgmax, out.fg[g1].pg_end - out.fg[g0].pg_begin);
         }
@@ -439,7 +439,7 @@ int FusedBatch::build(const std::vector<const FusedHostReq*>& reqs, int mode_, i
     {
         int gm = 1;
         for (int q = 0; q < nreq; ++q) gm = std::max(gm, reqs[q]->gmax);
-        greg = mc ? fused_mc_greg(gm) : FUSED_GREG;
+        greg = fused_greg(gm);
     }
     total_out = int(out_off[nreq]);
     args.bf_u4 = bf;

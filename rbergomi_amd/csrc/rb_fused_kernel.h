@@ -255,7 +255,7 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
                         }
                     }
                 }
-                // ---- 3b. integrals of every group of the unit, FUSED_GREG groups per pass
+                // ---- 3b. integrals of every group of the unit, GR groups per pass
                 for (int g0 = U.g_begin; g0 < U.g_end; g0 += GR) {
                     const int gn_ = U.g_end - g0 < GR ? U.g_end - g0 : GR;
                     const int gn = REQ_IN_LDS ? gn_ : __builtin_amdgcn_readfirstlane(gn_);
