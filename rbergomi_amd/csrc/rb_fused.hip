@@ -54,7 +54,10 @@ hipError_t launch_fused_rt(int mode, int upw, const FusedBatchArgs& a, int grid,
 
 hipError_t launch_fused(int mode, int upw, bool mc, int greg, const FusedBatchArgs& a, int grid,
                         hipStream_t st) {
-    if (!mc) return greg == 8 ? launch_fused_rt(mode, upw, a, grid, st) : launch_fused_rt6(mode, upw, a, grid, st);
+    if (!mc) {
+        if (greg == 2) return launch_fused_rt2(mode, upw, a, grid, st);
+        return greg == 8 ? launch_fused_rt(mode, upw, a, grid, st) : launch_fused_rt6(mode, upw, a, grid, st);
+    }
     return greg == 8 ? launch_fused_mc8(mode, upw, a, grid, st) : launch_fused_mc6(mode, upw, a, grid, st);
 }
 
