@@ -99,10 +99,15 @@ struct FusedBatchArgs {
 // Mills ratio M(z) = (1 - Phi(z)) / phi(z) for z >= 0, as M = t P(t), t = 1/(1 + z/3), with P a
 // degree-10 polynomial fitted here (weighted least squares at Chebyshev nodes in t) to
 // sqrt(pi/2) erfcx(z/sqrt 2); relative error <= 3.3e-7 in FP32 over z >= 0 (tests compare it
-// with scipy.special.erfcx).  M(z) -> 1/z as z -> infinity; M(inf) = 0.
+// with scipy.special.erfcx).  M(z) -> 1/z as z -> infinity.  t is the hardware reciprocal
+// refined by one Newton step (within an ulp of 1/(1 + z/3); four instructions in place of the
+// ten of a correctly rounded division); the denominator is capped at 1e30 so that z = inf gives
+// M ~ 3e-31 rather than NaN (every caller multiplies M by exp(-z^2/2), which is 0 there).
 __device__ __forceinline__ float mills_f32(float z) {
     const float c[11] = {3.333333433e-01f, 3.333306611e-01f, 2.964155674e-01f, 2.201747149e-01f, 1.412241310e-01f, -6.350896508e-02f, 2.205852419e-01f, -5.401814580e-01f, 4.626366198e-01f, -1.762671471e-01f, 2.557139099e-02f};
-    const float t = 1.0f / fmaf(z, 0.333333333f, 1.0f);
+    const float d = fminf(fmaf(z, 0.333333333f, 1.0f), 1e30f);
+    const float r = __builtin_amdgcn_rcpf(d);
+    const float t = fmaf(r, fmaf(-d, r, 1.0f), r);
     float acc = c[10];
 #pragma unroll
     for (int i = 9; i >= 0; --i) acc = fmaf(acc, t, c[i]);
