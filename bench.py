@@ -25,8 +25,9 @@ sys.path.insert(0, ROOT)
 MATURITIES = [0.1, 0.25, 0.5, 1.0, 2.0]
 LOG_MONEYNESS = np.linspace(-0.4, 0.2, 41)
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "fp16": 2500.0, "bf16": 2500.0, "fp32gemm": 157.3}
-# fp32: FP32 FFT convolution (staged); fp32gemm: Toeplitz GEMM on FP32 MFMA (fused kernel)
-PREC_CODE = {"fp64": 0, "fp32": 1, "fp16": 2, "bf16": 3, "fp32gemm": 4}
+# fp32: FP32 FFT convolution (staged); fp32gemm: Toeplitz GEMM on FP32 MFMA (fused kernel);
+# fp32fft: FP32 FFT convolution inside the single-launch kernel
+PREC_CODE = {"fp64": 0, "fp32": 1, "fp16": 2, "bf16": 3, "fp32gemm": 4, "fp32fft": 5}
 
 
 def workload(paths):
@@ -337,7 +338,8 @@ def main():
     # algorithmic flops of the Toeplitz convolution: M*N*(N-1) per pricing call
     flops_per_launch = (M / launches) * N * (N - 1)
     roof = None
-    if "volterra" in dominant or "fused" in dominant:
+    # the matrix-core roofline applies to the Toeplitz products; the FFT kernels run on the VALU
+    if ("volterra" in dominant or "fused" in dominant) and "fft" not in dominant:
         achieved = flops_per_launch / (avg_ms * 1e-3) / 1e12
         peak = PEAK_TFLOPS[args.precision]
         default = (args.paths == 2 ** 22 and args.precision == "fp16" and args.method == "rt")
@@ -366,7 +368,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": {"fp64": "f64", "fp32": "f32", "fp16": "f16x2+f32", "bf16": "bf16x2+f32",
-                  "fp32gemm": "f32"}[args.precision],
+                  "fp32gemm": "f32", "fp32fft": "f32"}[args.precision],
         "data": "synthetic (device Philox4x32-10 Gaussians)",
         "config": {"workload": "single parameter set at scale: H=0.1 eta=1.5 rho=-0.7 xi=0.04 "
                                "N=512 M=2^22, 5 maturities x 41 strikes"

@@ -72,7 +72,11 @@ enum rb_precision {
     RB_PREC_FP32 = 1,        /* Volterra convolution by FP32 FFT in LDS, the rest FP64 */
     RB_PREC_SPLIT_FP16 = 2,  /* Toeplitz GEMM as FP16 hi+lo products, FP32 accumulate */
     RB_PREC_SPLIT_BF16 = 3,  /* Toeplitz GEMM as BF16 hi+lo products, FP32 accumulate */
-    RB_PREC_FP32_GEMM = 4    /* Toeplitz GEMM on FP32 operands (v_mfma_f32_32x32x2_f32), N <= 1024 */
+    RB_PREC_FP32_GEMM = 4,   /* Toeplitz GEMM on FP32 operands (v_mfma_f32_32x32x2_f32), N <= 1024 */
+    RB_PREC_FP32_FFT = 5     /* FP32 LDS FFT convolution inside the single-launch kernel at every
+                                N <= 8192 (the long-path kernel), FP32 integrals, FP64 sums;
+                                caller or Sobol' Gaussians take the staged FFT pipeline; not
+                                accepted by rb_price_batch and the server */
 };
 enum rb_compensator {
     RB_COMP_REF = 0,   /* 0.5 eta^2 ((i-1) dt)^(2H), rBergLib/rBergomiMT.cpp:77-79  */

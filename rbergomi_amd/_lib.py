@@ -20,7 +20,7 @@ class RbConfig(ctypes.Structure):
                 ("method", ctypes.c_int), ("scheme", ctypes.c_int)]
 
 
-PREC_FP64, PREC_FP32, PREC_SPLIT_FP16, PREC_SPLIT_BF16, PREC_FP32_GEMM = 0, 1, 2, 3, 4
+PREC_FP64, PREC_FP32, PREC_SPLIT_FP16, PREC_SPLIT_BF16, PREC_FP32_GEMM, PREC_FP32_FFT = 0, 1, 2, 3, 4, 5
 COMP_REF, COMP_EXACT = 0, 1
 METHOD_RT, METHOD_MC = 0, 1
 SCHEME_HYBRID, SCHEME_CHOLESKY = 0, 1

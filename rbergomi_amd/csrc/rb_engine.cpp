@@ -394,8 +394,9 @@ int Plan::init(const Request& r, hipStream_t st, bool quadrature, std::string& e
         err = "the Cholesky scheme covers N <= 2048";
         return ST_ERR_LIMIT;
     }
-    if (split && r.N > 1024 && r.N <= 8192 && !quadrature && r.gauss == 0) {
-        // long paths on device Gaussians: the single-launch FFT kernel
+    const bool fft1 = r.precision == PREC_FP32_FFT;  // the single-launch FFT kernel at every N
+    if ((fft1 || (split && r.N > 1024)) && r.N <= 8192 && !quadrature && r.gauss == 0) {
+        // long paths on device Gaussians (or any N on request): the single-launch FFT kernel
         fused_fft.reset(new FftFused());
         const int rc = fused_fft->build(r, lay, device, stream, err);
         if (rc) return rc;
@@ -403,7 +404,7 @@ int Plan::init(const Request& r, hipStream_t st, bool quadrature, std::string& e
         chunk = r.M;
         return ST_OK;
     }
-    if (r.precision == PREC_FP32 || (split && (r.N > 1024 || quadrature || r.gauss >= 1))) {
+    if (r.precision == PREC_FP32 || ((split || fft1) && (r.N > 1024 || quadrature || r.gauss >= 1))) {
         // staged pipeline with the FP32 FFT convolution (long paths, user Gaussians)
         conv_fft = true;
     } else if (split) {

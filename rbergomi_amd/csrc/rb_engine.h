@@ -15,7 +15,8 @@
 
 namespace rb {
 
-enum Precision { PREC_FP64 = 0, PREC_FP32 = 1, PREC_SPLIT_FP16 = 2, PREC_SPLIT_BF16 = 3, PREC_FP32_GEMM = 4 };
+enum Precision { PREC_FP64 = 0, PREC_FP32 = 1, PREC_SPLIT_FP16 = 2, PREC_SPLIT_BF16 = 3, PREC_FP32_GEMM = 4,
+                 PREC_FP32_FFT = 5 };
 
 // operand mode of the fused kernel for a precision (0 FP16 hi+lo, 1 BF16 hi+lo, 2 FP32), -1 if
 // the precision does not run on the fused kernel

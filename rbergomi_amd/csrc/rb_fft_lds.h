@@ -206,7 +206,7 @@ __device__ __forceinline__ void fft_dif_inv_r4_t(float2* z, const float2* __rest
     fft_dif_pass<LOG2L, LOG2L - 1>(z, tw, tid);
 }
 
-// ---- convolution of two paths with radix-16 passes, L = 2^12..2^14 (FP32).  The stages run in
+// ---- convolution of two paths with radix-16 passes, L = 2^10..2^14 (FP32).  The stages run in
 // passes over groups of 16 points held in registers (3-5 LDS passes and barriers per transform
 // instead of 6-7 for radix 4), and the pass either side of the spectral product runs as one.
 // Element i0 + k span of a group sits in slot fpad(i0) + k span + (k span) / 32 for every
@@ -321,14 +321,19 @@ __host__ __device__ __forceinline__ void fft_group(float2* z, const float2* __re
     for (int k = 0; k < NP; ++k) z[q + k * span + ((k * span) >> 5)] = x[k];
 }
 
-// one pass: every group, then a barrier
+// one pass: every group, then a barrier (below 256 groups, L = 2^10 and 2^11, the first NG
+// threads take one group each)
 template <int LOG2L, int SB, int R, int KIND>
 __device__ __forceinline__ void fft_pass(float2* z, const float2* __restrict__ tw, const float2* __restrict__ G,
                                          int tid) {
     constexpr int NG = (1 << LOG2L) >> R;
-    static_assert(NG % 256 == 0, "whole groups per thread");
+    if constexpr (NG >= 256) {
+        static_assert(NG % 256 == 0, "whole groups per thread");
 #pragma unroll 2
-    for (int u = 0; u < NG / 256; ++u) fft_group<LOG2L, SB, R, KIND>(z, tw, G, tid + 256 * u);
+        for (int u = 0; u < NG / 256; ++u) fft_group<LOG2L, SB, R, KIND>(z, tw, G, tid + 256 * u);
+    } else {
+        if (tid < NG) fft_group<LOG2L, SB, R, KIND>(z, tw, G, tid);
+    }
     __syncthreads();
 }
 
@@ -364,7 +369,7 @@ __device__ __forceinline__ void fft_nat_fwd(float2* z, const float2* __restrict_
 template <int LOG2L>
 __device__ __forceinline__ void fft_convolve_r16(float2* z, const float2* __restrict__ tw,
                                                  const float2* __restrict__ G, int tid) {
-    static_assert(LOG2L >= 12 && LOG2L <= 14, "radix-16 passes of at least 256 groups");
+    static_assert(LOG2L >= 10 && LOG2L <= 14, "transform lengths 2^10 .. 2^14");
     fft_nat_fwd<LOG2L, LOG2L - 2>(z, tw, G, tid);
 }
 

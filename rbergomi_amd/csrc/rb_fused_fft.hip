@@ -1,5 +1,6 @@
-// rb_fused_fft.hip -- single-launch mixed-precision Romano-Touzi pricing for long paths
-// (N > 1024, where the Toeplitz GEMM's W1 tile no longer fits LDS).
+// rb_fused_fft.hip -- single-launch mixed-precision Romano-Touzi pricing by FFT convolution: for
+// long paths (N > 1024, where the Toeplitz GEMM's W1 tile no longer fits LDS), and for any
+// N <= 8192 at the FFT precision (the same kernel with one quad per thread below N = 1024).
 //
 // A persistent workgroup takes work items (pairs_per_item pairs of paths) from an atomic
 // counter and runs the whole path of the reference for each pair without touching HBM:
@@ -45,12 +46,10 @@ size_t fused_fft_lds_bytes(const FftFusedArgs& a) {
            size_t(2) * a.pgu * sizeof(PathPG) + size_t(a.out_size) * 2 * sizeof(double);
 }
 
-// QPT: quads per thread and path, ceil(ceil(N/4) / 256).  For 1024 < N <= 8192 it fixes the
-// transform length: L = 2^(11 + log2 QPT) (QPT = 2, 4, 8 for N <= 2048, 4096, 8192).
-template <int QPT>
-constexpr int fft_log2L() { return QPT == 2 ? 12 : QPT == 4 ? 13 : 14; }
-
-template <int QPT, bool MC>
+// QPT: quads per thread and path, ceil(ceil(N/4) / 256); LOG2L: the transform length,
+// L = 2^LOG2L >= 2N - 1 and at least 2^10 (LOG2L = 11 + log2 QPT for 1024 < N <= 8192; 10 or 11
+// for QPT = 1).
+template <int QPT, bool MC, int LOG2L>
 __global__ void __launch_bounds__(256, (QPT >= 8 || (MC && QPT >= 4)) ? 1 : 2) k_fused_fft(FftFusedArgs A) {
     extern __shared__ float2 lds2[];
     const int L = 1 << A.log2L, H2 = L >> 1;
@@ -118,7 +117,7 @@ __global__ void __launch_bounds__(256, (QPT >= 8 || (MC && QPT >= 4)) ? 1 : 2) k
                     z[fpad(i + H2)] = make_float2(0.f, 0.f);
                 }
                 __syncthreads();
-                fft_convolve_r16<fft_log2L<QPT>()>(z, A.tw, A.ghat + long(u) * L, tid);
+                fft_convolve_r16<LOG2L>(z, A.tw, A.ghat + long(u) * L, tid);
                 // ---- 3. Wtilde of the thread's steps (both paths), then back to LDS in step order
                 const float invL = 1.0f / float(L);
                 float wt[2][QPT][4];
@@ -293,8 +292,8 @@ hipError_t launch_fused_fft(const FftFusedArgs& a, int grid, hipStream_t st) {
     const size_t lds = fused_fft_lds_bytes(a);
     const int qpt = ((a.N + 3) / 4 + 255) / 256;
     // the kernels are compiled for the transform length of their quad count
-    const int want = qpt <= 2 ? 12 : qpt <= 4 ? 13 : 14;
-    if (a.N <= 1024 || a.log2L != want) return hipErrorInvalidValue;
+    const int want = qpt == 1 ? (a.N <= 512 ? 10 : 11) : qpt <= 2 ? 12 : qpt <= 4 ? 13 : 14;
+    if (a.N < 1 || a.log2L != want) return hipErrorInvalidValue;
     auto go = [&](auto kern) -> hipError_t {
         if (lds > 65536) {
             const hipError_t e = allow_lds(reinterpret_cast<const void*>(kern), lds);
@@ -304,13 +303,15 @@ hipError_t launch_fused_fft(const FftFusedArgs& a, int grid, hipStream_t st) {
         return hipGetLastError();
     };
     if (a.mc) {
-        if (qpt <= 2) return go(&k_fused_fft<2, true>);
-        if (qpt <= 4) return go(&k_fused_fft<4, true>);
-        if (qpt <= 8) return go(&k_fused_fft<8, true>);
+        if (qpt == 1) return a.log2L == 10 ? go(&k_fused_fft<1, true, 10>) : go(&k_fused_fft<1, true, 11>);
+        if (qpt <= 2) return go(&k_fused_fft<2, true, 12>);
+        if (qpt <= 4) return go(&k_fused_fft<4, true, 13>);
+        if (qpt <= 8) return go(&k_fused_fft<8, true, 14>);
     } else {
-        if (qpt <= 2) return go(&k_fused_fft<2, false>);
-        if (qpt <= 4) return go(&k_fused_fft<4, false>);
-        if (qpt <= 8) return go(&k_fused_fft<8, false>);
+        if (qpt == 1) return a.log2L == 10 ? go(&k_fused_fft<1, false, 10>) : go(&k_fused_fft<1, false, 11>);
+        if (qpt <= 2) return go(&k_fused_fft<2, false, 12>);
+        if (qpt <= 4) return go(&k_fused_fft<4, false, 13>);
+        if (qpt <= 8) return go(&k_fused_fft<8, false, 14>);
     }
     return hipErrorInvalidValue;
 }

@@ -562,7 +562,7 @@ int FftFused::plan(const Request& req, const Layout& lay, size_t lds_limit, size
         err = "the long-path kernel covers 1024 < N <= 8192";
         return ST_ERR_LIMIT;
     }
-    int log2L = 1;
+    int log2L = 10;  // L >= 2N - 1, at least 2^10 (the kernel's shortest transform)
     while ((1 << log2L) < 2 * N - 1) ++log2L;
     const size_t L = size_t(1) << log2L;
     const int Np = (N + 31) / 32 * 32;
@@ -591,7 +591,7 @@ int FftFused::build(const Request& req, const Layout& lay, int device, hipStream
     release();
     stream = st;
     const int N = req.N;
-    int log2L = 1;
+    int log2L = 10;  // L >= 2N - 1, at least 2^10 (the kernel's shortest transform)
     while ((1 << log2L) < 2 * N - 1) ++log2L;
     const int L = 1 << log2L;
     if (((N + 3) / 4 + 255) / 256 > 8) {

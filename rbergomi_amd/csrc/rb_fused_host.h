@@ -38,7 +38,8 @@ int plan_fused_request(const Request& req, const Layout& lay, int mode, FusedHos
 void fused_tables(const Request& req, const Layout& lay, int Np, int frag_stride, int pshift, FusedHostReq& out,
                   int& pgmax, int& gmax);
 
-// A long-path request (N > 1024, Romano-Touzi or plain MC) on the single-launch FFT kernel.
+// A request on the single-launch FFT kernel (Romano-Touzi or plain MC): long paths (N > 1024)
+// at the mixed precisions, any N <= 8192 at RB_PREC_FP32_FFT.
 struct FftFused {
     FftFusedArgs args{};
     int grid = 0, total_out = 0;

@@ -104,3 +104,41 @@ def test_fused_long_path_rejects_what_lds_cannot_hold(rb):
     with pytest.raises(rb.RbError) as e:
         rb.price_v2(0.04, [0.1], [1.5], [-0.7], [1.0] * len(K), K, 8192, 10, [1], precision=rb.PREC_SPLIT_FP16)
     assert e.value.code == 5
+
+
+@pytest.mark.parametrize("method", [0, 1])
+@pytest.mark.parametrize("N,M", [(7, 100), (32, 999), (100, 1501), (512, 2001), (513, 777), (1000, 600),
+                                 (1024, 301), (2048, 500)])
+def test_fft_precision_prices_match_fp64(rb, N, M, method):
+    # PREC_FP32_FFT runs the single-launch FFT kernel at every N (transforms of 2^10 and 2^11 below
+    # N = 1024, one quad per thread): same Philox counters as the FP64 path, FP32 transform and
+    # integrals, FP64 sums; two units, several maturities and correlations, odd M
+    H, eta, rho = [0.07, 0.3], [1.9, 1.1], [-0.9, -0.3]
+    T = [0.1, 0.5, 1.0, 2.0]
+    K = [0.95, 0.85, 1.2, 1.0]
+    print("fft precision N=%d M=%d method=%d" % (N, M, method), flush=True)
+    p64, iv64, s64 = rb.price_v2(0.04, H, eta, rho, T, K, N, M, [8, N], precision=rb.PREC_FP64, method=method)
+    pm, ivm, sm = rb.price_v2(0.04, H, eta, rho, T, K, N, M, [8, N], precision=rb.PREC_FP32_FFT, method=method)
+    err = np.max(np.abs(pm - p64))
+    print("   max |price - fp64| = %.2e, |stat - fp64| = %.2e" % (err, np.max(np.abs(sm - s64))), flush=True)
+    assert err <= 1e-4 and np.max(np.abs(sm - s64)) <= 1e-4
+    assert np.all(np.isfinite(ivm) | ~np.isfinite(iv64))
+
+
+def test_fft_precision_routing_and_batches(rb):
+    # the plan runs the single-launch FFT kernel at N = 512; caller Gaussians take the staged FFT
+    # pipeline; batches and the server run the Toeplitz-GEMM kernel only and reject the precision
+    T, K = [1.0] * 3, [0.9, 1.0, 1.1]
+    pl = rb.Plan(0.04, [0.1], [1.5], [-0.7], T, K, 512, 2000, [4], precision=rb.PREC_FP32_FFT, profile=True)
+    pl.run()
+    times = pl.kernel_times()
+    p, _, _ = pl.results()
+    pl.close()
+    assert "k_fused_fft" in times and "k_fused" not in times
+    again, _, _ = rb.price_v2(0.04, [0.1], [1.5], [-0.7], T, K, 512, 2000, [4], precision=rb.PREC_FP32_FFT)
+    assert np.array_equal(p, again)
+    from rbergomi_amd.serving import Request, price_batch
+    req = Request(0.04, [0.1], [1.5], [-0.7], T, K, 512, 1000, [1])
+    with pytest.raises(rb.RbError) as e:
+        price_batch([req], precision=rb.PREC_FP32_FFT)
+    assert e.value.code == 1
