@@ -42,7 +42,7 @@ k_volterra_fft(const double* __restrict__ Z, long zstride, long np, int N, int l
     __syncthreads();
     const float2* G = ghat + long(u) * L;  // FFT(Gamma) of this unit
     if constexpr (LOG2L >= 12) {
-        fft_convolve_r16_br<LOG2L>(z, tw, G, threadIdx.x);
+        fft_convolve_r16_br<LOG2L>(z, tw, G, FftWorkgroup{int(threadIdx.x)});
     } else {
         if constexpr (LOG2L > 0)
             fft_dit_r4_t<LOG2L>(z, tw, threadIdx.x);

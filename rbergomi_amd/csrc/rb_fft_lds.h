@@ -321,88 +321,92 @@ __host__ __device__ __forceinline__ void fft_group(float2* z, const float2* __re
     for (int k = 0; k < NP; ++k) z[q + k * span + ((k * span) >> 5)] = x[k];
 }
 
-// one pass: every group, then a barrier (below 256 groups, L = 2^10 and 2^11, the first NG
-// threads take one group each)
-template <int LOG2L, int SB, int R, int KIND>
-__device__ __forceinline__ void fft_pass(float2* z, const float2* __restrict__ tw, const float2* __restrict__ G,
-                                         int tid) {
-    constexpr int NG = (1 << LOG2L) >> R;
-    if constexpr (NG >= 256) {
-        static_assert(NG % 256 == 0, "whole groups per thread");
+// The runner of a pass: a workgroup of 256 threads takes every group of the pass, then meets at a
+// barrier (below 256 groups, L = 2^10 and 2^11, the first NG threads take one group each).  The
+// schedules below are templates on the runner, so a host runner that walks the groups in order
+// executes the same passes on the CPU.
+struct FftWorkgroup {
+    int tid;
+    template <int LOG2L, int SB, int R, int KIND>
+    __device__ __forceinline__ void pass(float2* z, const float2* __restrict__ tw, const float2* __restrict__ G) const {
+        constexpr int NG = (1 << LOG2L) >> R;
+        if constexpr (NG >= 256) {
+            static_assert(NG % 256 == 0, "whole groups per thread");
 #pragma unroll 2
-        for (int u = 0; u < NG / 256; ++u) fft_group<LOG2L, SB, R, KIND>(z, tw, G, tid + 256 * u);
-    } else {
-        if (tid < NG) fft_group<LOG2L, SB, R, KIND>(z, tw, G, tid);
+            for (int u = 0; u < NG / 256; ++u) fft_group<LOG2L, SB, R, KIND>(z, tw, G, tid + 256 * u);
+        } else {
+            if (tid < NG) fft_group<LOG2L, SB, R, KIND>(z, tw, G, tid);
+        }
+        __syncthreads();
     }
-    __syncthreads();
-}
+};
 
 // ---- natural order: stages 0 .. LOG2L - 2 in passes, the middle one at the bottom
 template <int LOG2L>
 constexpr int fft_mid_stages() { return (LOG2L - 1) % 4 == 0 ? 4 : (LOG2L - 1) % 4; }
 
-template <int LOG2L, int S>  // inverse passes from stage S up to LOG2L - 2
-__device__ __forceinline__ void fft_nat_inv(float2* z, const float2* __restrict__ tw, int tid) {
+template <int LOG2L, int S, class X>  // inverse passes from stage S up to LOG2L - 2
+__host__ __device__ __forceinline__ void fft_nat_inv(float2* z, const float2* __restrict__ tw, const X& x) {
     if constexpr (S <= LOG2L - 2) {
-        fft_pass<LOG2L, S, 4, FFT_INV_DIT>(z, tw, nullptr, tid);
-        fft_nat_inv<LOG2L, S + 4>(z, tw, tid);
+        x.template pass<LOG2L, S, 4, FFT_INV_DIT>(z, tw, nullptr);
+        fft_nat_inv<LOG2L, S + 4>(z, tw, x);
     }
 }
 
-template <int LOG2L, int S>  // forward passes from stage S down to the middle pass
-__device__ __forceinline__ void fft_nat_fwd(float2* z, const float2* __restrict__ tw, const float2* __restrict__ G,
-                                            int tid) {
+template <int LOG2L, int S, class X>  // forward passes from stage S down to the middle pass
+__host__ __device__ __forceinline__ void fft_nat_fwd(float2* z, const float2* __restrict__ tw,
+                                                     const float2* __restrict__ G, const X& x) {
     constexpr int RM = fft_mid_stages<LOG2L>();
     if constexpr (S >= RM) {
-        fft_pass<LOG2L, S - 3, 4, FFT_FWD_DIF>(z, tw, nullptr, tid);
-        fft_nat_fwd<LOG2L, S - 4>(z, tw, G, tid);
+        x.template pass<LOG2L, S - 3, 4, FFT_FWD_DIF>(z, tw, nullptr);
+        fft_nat_fwd<LOG2L, S - 4>(z, tw, G, x);
     } else {
         static_assert(S == RM - 1, "passes of four stages down to the middle pass");
-        fft_pass<LOG2L, 0, RM, FFT_MID_NAT>(z, tw, G, tid);
-        fft_nat_inv<LOG2L, RM>(z, tw, tid);
+        x.template pass<LOG2L, 0, RM, FFT_MID_NAT>(z, tw, G);
+        fft_nat_inv<LOG2L, RM>(z, tw, x);
     }
 }
 
 // Stages 0 .. LOG2L - 2 of the forward transform, the product with G (bit-reversed order) and
 // stages 0 .. LOG2L - 2 of the inverse; z holds the input after the forward top stage and is left
-// ready for the inverse top stage.  Starts after and ends with a barrier.
-template <int LOG2L>
-__device__ __forceinline__ void fft_convolve_r16(float2* z, const float2* __restrict__ tw,
-                                                 const float2* __restrict__ G, int tid) {
+// ready for the inverse top stage.  With FftWorkgroup: starts after and ends with a barrier.
+template <int LOG2L, class X>
+__host__ __device__ __forceinline__ void fft_convolve_r16(float2* z, const float2* __restrict__ tw,
+                                                          const float2* __restrict__ G, const X& x) {
     static_assert(LOG2L >= 10 && LOG2L <= 14, "transform lengths 2^10 .. 2^14");
-    fft_nat_fwd<LOG2L, LOG2L - 2>(z, tw, G, tid);
+    fft_nat_fwd<LOG2L, LOG2L - 2>(z, tw, G, x);
 }
 
 // ---- bit-reversed order: forward passes from stage 0 up, the middle one at the top
-template <int LOG2L, int S>  // inverse passes from stage S down to 0
-__device__ __forceinline__ void fft_br_inv(float2* z, const float2* __restrict__ tw, int tid) {
+template <int LOG2L, int S, class X>  // inverse passes from stage S down to 0
+__host__ __device__ __forceinline__ void fft_br_inv(float2* z, const float2* __restrict__ tw, const X& x) {
     if constexpr (S >= 0) {
-        fft_pass<LOG2L, S - 3, 4, FFT_INV_DIF>(z, tw, nullptr, tid);
-        fft_br_inv<LOG2L, S - 4>(z, tw, tid);
+        x.template pass<LOG2L, S - 3, 4, FFT_INV_DIF>(z, tw, nullptr);
+        fft_br_inv<LOG2L, S - 4>(z, tw, x);
     }
 }
 
-template <int LOG2L, int SB>  // forward passes from stage SB up to the middle pass
-__device__ __forceinline__ void fft_br_fwd(float2* z, const float2* __restrict__ tw, const float2* __restrict__ G,
-                                           int tid) {
+template <int LOG2L, int SB, class X>  // forward passes from stage SB up to the middle pass
+__host__ __device__ __forceinline__ void fft_br_fwd(float2* z, const float2* __restrict__ tw,
+                                                    const float2* __restrict__ G, const X& x) {
     constexpr int R = LOG2L - SB >= 4 ? 4 : LOG2L - SB;
     if constexpr (SB + R == LOG2L) {
         static_assert(SB % 4 == 0 && SB >= 4, "passes of four stages below the middle pass");
-        fft_pass<LOG2L, SB, R, FFT_MID_BR>(z, tw, G, tid);
-        fft_br_inv<LOG2L, SB - 1>(z, tw, tid);
+        x.template pass<LOG2L, SB, R, FFT_MID_BR>(z, tw, G);
+        fft_br_inv<LOG2L, SB - 1>(z, tw, x);
     } else {
-        fft_pass<LOG2L, SB, 4, FFT_FWD_DIT>(z, tw, nullptr, tid);
-        fft_br_fwd<LOG2L, SB + 4>(z, tw, G, tid);
+        x.template pass<LOG2L, SB, 4, FFT_FWD_DIT>(z, tw, nullptr);
+        fft_br_fwd<LOG2L, SB + 4>(z, tw, G, x);
     }
 }
 
 // Circular convolution of z (bit-reversed order in and out, unnormalised) with the spectrum G
-// (natural order).  Starts after and ends with a barrier.
-template <int LOG2L>
-__device__ __forceinline__ void fft_convolve_r16_br(float2* z, const float2* __restrict__ tw,
-                                                    const float2* __restrict__ G, int tid) {
+// (natural order).  With FftWorkgroup: starts after and ends with a barrier.
+template <int LOG2L, class X>
+__host__ __device__ __forceinline__ void fft_convolve_r16_br(float2* z, const float2* __restrict__ tw,
+                                                             const float2* __restrict__ G, const X& x) {
     static_assert(LOG2L >= 12 && LOG2L <= 14, "radix-16 passes of at least 256 groups");
-    fft_br_fwd<LOG2L, 0>(z, tw, G, tid);
+    fft_br_fwd<LOG2L, 0>(z, tw, G, x);
 }
 
 }  // namespace rb

@@ -117,7 +117,7 @@ __global__ void __launch_bounds__(256, (QPT >= 8 || (MC && QPT >= 4)) ? 1 : 2) k
                     z[fpad(i + H2)] = make_float2(0.f, 0.f);
                 }
                 __syncthreads();
-                fft_convolve_r16<LOG2L>(z, A.tw, A.ghat + long(u) * L, tid);
+                fft_convolve_r16<LOG2L>(z, A.tw, A.ghat + long(u) * L, FftWorkgroup{tid});
                 // ---- 3. Wtilde of the thread's steps (both paths), then back to LDS in step order
                 const float invL = 1.0f / float(L);
                 float wt[2][QPT][4];
