@@ -151,7 +151,7 @@ size_t fused_req_lds_bytes(const FusedReq& a);
 size_t fused_lds_bytes(const FusedBatchArgs& b);
 // mode: 0 = FP16 hi+lo split, 1 = BF16 hi+lo split, 2 = FP32 operands
 // mc: plain Monte Carlo kernel (every request of the batch), else Romano-Touzi; greg groups per
-// pass: 6 or 8, or 2 (Romano-Touzi)
+// pass: 2, 6 or 8
 hipError_t launch_fused(int mode, int upw, bool mc, int greg, const FusedBatchArgs& b, int grid,
                         hipStream_t st);
 hipError_t launch_fused_rt(int mode, int upw, const FusedBatchArgs& b, int grid, hipStream_t st);
@@ -159,10 +159,11 @@ hipError_t launch_fused_rt6(int mode, int upw, const FusedBatchArgs& b, int grid
 hipError_t launch_fused_rt2(int mode, int upw, const FusedBatchArgs& b, int grid, hipStream_t st);
 hipError_t launch_fused_mc6(int mode, int upw, const FusedBatchArgs& b, int grid, hipStream_t st);
 hipError_t launch_fused_mc8(int mode, int upw, const FusedBatchArgs& b, int grid, hipStream_t st);
+hipError_t launch_fused_mc2(int mode, int upw, const FusedBatchArgs& b, int grid, hipStream_t st);
 // groups per pass of the fused kernel for units of at most gmax groups: fewest passes, then the
-// smaller register footprint (two groups per pass for the Romano-Touzi kernel only)
-inline int fused_greg(int gmax, bool mc) {
-    if (!mc && gmax <= 2) return 2;
+// smaller register footprint
+inline int fused_greg(int gmax) {
+    if (gmax <= 2) return 2;
     return (gmax + 7) / 8 < (gmax + 5) / 6 ? 8 : 6;
 }
 hipError_t launch_zero_int(int* p, int n, hipStream_t st);
