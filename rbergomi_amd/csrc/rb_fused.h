@@ -157,13 +157,15 @@ hipError_t launch_fused(int mode, int upw, bool mc, int greg, const FusedBatchAr
 hipError_t launch_fused_rt(int mode, int upw, const FusedBatchArgs& b, int grid, hipStream_t st);
 hipError_t launch_fused_rt6(int mode, int upw, const FusedBatchArgs& b, int grid, hipStream_t st);
 hipError_t launch_fused_rt2(int mode, int upw, const FusedBatchArgs& b, int grid, hipStream_t st);
+hipError_t launch_fused_rt5(int mode, int upw, const FusedBatchArgs& b, int grid, hipStream_t st);
 hipError_t launch_fused_mc6(int mode, int upw, const FusedBatchArgs& b, int grid, hipStream_t st);
 hipError_t launch_fused_mc8(int mode, int upw, const FusedBatchArgs& b, int grid, hipStream_t st);
 hipError_t launch_fused_mc2(int mode, int upw, const FusedBatchArgs& b, int grid, hipStream_t st);
 // groups per pass of the fused kernel for units of at most gmax groups: fewest passes, then the
 // smaller register footprint
-inline int fused_greg(int gmax) {
+inline int fused_greg(int gmax, bool mc) {
     if (gmax <= 2) return 2;
+    if (!mc && gmax == 5) return 5;  // the Romano-Touzi kernel also has an exact five-group pass
     return (gmax + 7) / 8 < (gmax + 5) / 6 ? 8 : 6;
 }
 hipError_t launch_zero_int(int* p, int n, hipStream_t st);
