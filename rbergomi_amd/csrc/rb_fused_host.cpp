@@ -439,7 +439,7 @@ int FusedBatch::build(const std::vector<const FusedHostReq*>& reqs, int mode_, i
     {
         int gm = 1;
         for (int q = 0; q < nreq; ++q) gm = std::max(gm, reqs[q]->gmax);
-        greg = fused_greg(gm, mc);
+        greg = fused_greg(gm);
     }
     total_out = int(out_off[nreq]);
     args.bf_u4 = bf;
