@@ -56,10 +56,12 @@ hipError_t launch_fused(int mode, int upw, bool mc, int greg, const FusedBatchAr
                         hipStream_t st) {
     if (!mc) {
         if (greg == 2) return launch_fused_rt2(mode, upw, a, grid, st);
+        if (greg == 4) return launch_fused_rt4(mode, upw, a, grid, st);
         if (greg == 5) return launch_fused_rt5(mode, upw, a, grid, st);
         return greg == 8 ? launch_fused_rt(mode, upw, a, grid, st) : launch_fused_rt6(mode, upw, a, grid, st);
     }
     if (greg == 2) return launch_fused_mc2(mode, upw, a, grid, st);
+    if (greg == 4) return launch_fused_mc4(mode, upw, a, grid, st);
     if (greg == 5) return launch_fused_mc5(mode, upw, a, grid, st);
     return greg == 8 ? launch_fused_mc8(mode, upw, a, grid, st) : launch_fused_mc6(mode, upw, a, grid, st);
 }

@@ -1,5 +1,5 @@
 // rb_fused_mc5.hip -- the plain Monte Carlo fused kernel with five groups per pass, for units of
-// exactly five maturity groups; the stages are described in rb_fused_mc.hip.
+// five maturity groups or nine to ten (two passes); the stages are described in rb_fused_mc.hip.
 #include "rb_fused_kernel.h"
 
 namespace rb {

@@ -1,6 +1,6 @@
-// rb_fused_rt5.hip -- the Romano-Touzi fused kernel with five groups per pass, for units of exactly
-// five maturity groups (a five-maturity strike grid integrates with no dead group slot in the row
-// loop); the stages are described in rb_fused.hip.
+// rb_fused_rt5.hip -- the Romano-Touzi fused kernel with five groups per pass, for units of five
+// maturity groups (no dead group slot in the row loop) or nine to ten (two passes); the stages are
+// described in rb_fused.hip.
 #include "rb_fused_kernel.h"
 
 namespace rb {
