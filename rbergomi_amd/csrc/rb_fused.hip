@@ -55,11 +55,13 @@ hipError_t launch_fused_rt(int mode, int upw, const FusedBatchArgs& a, int grid,
 hipError_t launch_fused(int mode, int upw, bool mc, int greg, const FusedBatchArgs& a, int grid,
                         hipStream_t st) {
     if (!mc) {
+        if (greg == 1) return launch_fused_rt1(mode, upw, a, grid, st);
         if (greg == 2) return launch_fused_rt2(mode, upw, a, grid, st);
         if (greg == 4) return launch_fused_rt4(mode, upw, a, grid, st);
         if (greg == 5) return launch_fused_rt5(mode, upw, a, grid, st);
         return greg == 8 ? launch_fused_rt(mode, upw, a, grid, st) : launch_fused_rt6(mode, upw, a, grid, st);
     }
+    if (greg == 1) return launch_fused_mc1(mode, upw, a, grid, st);
     if (greg == 2) return launch_fused_mc2(mode, upw, a, grid, st);
     if (greg == 4) return launch_fused_mc4(mode, upw, a, grid, st);
     if (greg == 5) return launch_fused_mc5(mode, upw, a, grid, st);

@@ -1,5 +1,5 @@
-// rb_fused_mc2.hip -- the plain Monte Carlo fused kernel with two groups per pass, for units of one
-// or two maturity groups; the stages are described in rb_fused_mc.hip.
+// rb_fused_mc2.hip -- the plain Monte Carlo fused kernel with two groups per pass, for units of two
+// maturity groups; the stages are described in rb_fused_mc.hip.
 #include "rb_fused_kernel.h"
 
 namespace rb {

@@ -1,6 +1,5 @@
-// rb_fused_rt2.hip -- the Romano-Touzi fused kernel with two groups per pass, for units of one or
-// two maturity groups (single-maturity calibration requests): no dead group slots in the row loop;
-// the stages are described in rb_fused.hip.
+// rb_fused_rt2.hip -- the Romano-Touzi fused kernel with two groups per pass, for units of two
+// maturity groups (no dead group slot in the row loop); the stages are described in rb_fused.hip.
 #include "rb_fused_kernel.h"
 
 namespace rb {
