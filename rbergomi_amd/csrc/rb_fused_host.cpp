@@ -199,8 +199,8 @@ void fused_tables(const Request& req, const Layout& lay, int Np, int frag_stride
         U.g_end = int(out.fg.size());
         U.pg_end = int(out.fpg.size());
         U.o_end = int(out.fo.size());
-        // payoff groups held at once: those of one pass, over 6 or 8 groups, so every window of
-        // FUSED_GREG groups
+        // payoff groups held at once: those of one pass (at most FUSED_GREG groups), so every
+        // window of FUSED_GREG groups
         for (int g0 = U.g_begin; g0 < U.g_end; ++g0) {
             const int g1 = std::min(U.g_end, g0 + FUSED_GREG) - 1;
             pgmax = std::max(pgmax, out.fg[g1].pg_end - out.fg[g0].pg_begin);

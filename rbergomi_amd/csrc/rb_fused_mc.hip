@@ -11,7 +11,8 @@
 
 namespace rb {
 
-// six groups per pass (the eight-group variant is in rb_fused_mc8.hip, built in parallel)
+// six groups per pass (the one-, two-, four-, five- and eight-group variants are in their own
+// files, built in parallel)
 hipError_t launch_fused_mc6(int mode, int upw, const FusedBatchArgs& a, int grid, hipStream_t st) {
     return launch_mu<true, 6>(mode, upw, a, grid, st);
 }

@@ -1,6 +1,7 @@
-// rb_fused_rt6.hip -- the Romano-Touzi fused kernel with six groups per pass (units of up to
-// six maturity groups, or 9-12: one dead group slot fewer per row than the eight-group variant in
-// rb_fused.hip, and fewer accumulator registers); the stages are described in rb_fused.hip.
+// rb_fused_rt6.hip -- the Romano-Touzi fused kernel with six groups per pass (units of six
+// maturity groups, or eleven to twelve in two passes: fewer dead group slots per row and fewer
+// accumulator registers than the eight-group variant in rb_fused.hip); the stages are described
+// in rb_fused.hip.
 #include "rb_fused_kernel.h"
 
 namespace rb {
