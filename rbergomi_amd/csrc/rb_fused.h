@@ -151,7 +151,7 @@ size_t fused_req_lds_bytes(const FusedReq& a);
 size_t fused_lds_bytes(const FusedBatchArgs& b);
 // mode: 0 = FP16 hi+lo split, 1 = BF16 hi+lo split, 2 = FP32 operands
 // mc: plain Monte Carlo kernel (every request of the batch), else Romano-Touzi; greg groups per
-// pass: 1, 2, 4, 5, 6 or 8
+// pass: 1 to 6, or 8
 hipError_t launch_fused(int mode, int upw, bool mc, int greg, const FusedBatchArgs& b, int grid,
                         hipStream_t st);
 hipError_t launch_fused_rt(int mode, int upw, const FusedBatchArgs& b, int grid, hipStream_t st);
@@ -166,12 +166,14 @@ hipError_t launch_fused_rt4(int mode, int upw, const FusedBatchArgs& b, int grid
 hipError_t launch_fused_mc4(int mode, int upw, const FusedBatchArgs& b, int grid, hipStream_t st);
 hipError_t launch_fused_rt1(int mode, int upw, const FusedBatchArgs& b, int grid, hipStream_t st);
 hipError_t launch_fused_mc1(int mode, int upw, const FusedBatchArgs& b, int grid, hipStream_t st);
-// groups per pass of the fused kernel for units of at most gmax groups (1, 2, 4, 5, 6 or 8): the
+hipError_t launch_fused_rt3(int mode, int upw, const FusedBatchArgs& b, int grid, hipStream_t st);
+hipError_t launch_fused_mc3(int mode, int upw, const FusedBatchArgs& b, int grid, hipStream_t st);
+// groups per pass of the fused kernel for units of at most gmax groups (1 to 6, or 8): the
 // fewest passes, then the fewest groups per pass (dead group slots cost a test per row, and every
 // slot holds accumulators in registers)
 inline int fused_greg(int gmax) {
     const int passes = (gmax + FUSED_GREG - 1) / FUSED_GREG;
-    for (int g : {1, 2, 4, 5, 6})
+    for (int g : {1, 2, 3, 4, 5, 6})
         if ((gmax + g - 1) / g == passes) return g;
     return FUSED_GREG;
 }

@@ -57,12 +57,14 @@ hipError_t launch_fused(int mode, int upw, bool mc, int greg, const FusedBatchAr
     if (!mc) {
         if (greg == 1) return launch_fused_rt1(mode, upw, a, grid, st);
         if (greg == 2) return launch_fused_rt2(mode, upw, a, grid, st);
+        if (greg == 3) return launch_fused_rt3(mode, upw, a, grid, st);
         if (greg == 4) return launch_fused_rt4(mode, upw, a, grid, st);
         if (greg == 5) return launch_fused_rt5(mode, upw, a, grid, st);
         return greg == 8 ? launch_fused_rt(mode, upw, a, grid, st) : launch_fused_rt6(mode, upw, a, grid, st);
     }
     if (greg == 1) return launch_fused_mc1(mode, upw, a, grid, st);
     if (greg == 2) return launch_fused_mc2(mode, upw, a, grid, st);
+    if (greg == 3) return launch_fused_mc3(mode, upw, a, grid, st);
     if (greg == 4) return launch_fused_mc4(mode, upw, a, grid, st);
     if (greg == 5) return launch_fused_mc5(mode, upw, a, grid, st);
     return greg == 8 ? launch_fused_mc8(mode, upw, a, grid, st) : launch_fused_mc6(mode, upw, a, grid, st);

@@ -1,5 +1,5 @@
-// rb_fused_rt4.hip -- the Romano-Touzi fused kernel with four groups per pass, for units of three or
-// four maturity groups; the stages are described in rb_fused.hip.
+// rb_fused_rt4.hip -- the Romano-Touzi fused kernel with four groups per pass, for units of four
+// maturity groups; the stages are described in rb_fused.hip.
 #include "rb_fused_kernel.h"
 
 namespace rb {
