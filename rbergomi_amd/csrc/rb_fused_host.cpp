@@ -449,6 +449,28 @@ int FusedBatch::build(const std::vector<const FusedHostReq*>& reqs, int mode_, i
     hipDeviceProp_t prop;
     RB_TRY(device_props(device, prop));
     keep_pool_cached(device);
+    // Two workgroups per CU need at most half the CU's LDS each, the static request record
+    // included.  When the cross-wave partials are what keeps a launch above that, they are summed
+    // in smaller chunks of groups (the same sums in the same order, so the same bits).
+    {
+        const size_t half = prop.maxSharedMemoryPerMultiProcessor / 2;
+        const size_t rest = size_t(bf) * 16 + size_t(pg) * sizeof(PathPG) + size_t(ac) * 8 +
+                            sizeof(FusedReq) + 16;
+        const int ns = mc ? 3 : 2;
+        if (rest + size_t(wp) * 8 > half) {
+            int cap = 0;
+            for (int c = FUSED_GREG; c >= 1 && cap == 0; --c)
+                if (rest + size_t(4 * c * 32 * ns) * 8 <= half) cap = c;
+            if (cap > 0) {
+                wp = 0;
+                for (FusedReq& r : R) {
+                    r.gchunk = std::min(r.gchunk, cap);
+                    wp = std::max(wp, 4 * r.gchunk * 32 * ns);
+                }
+                args.wpart_d = wp;
+            }
+        }
+    }
     lds_need = fused_lds_bytes(args);
     if (lds_need > prop.sharedMemPerBlock) {
         err = "fused kernel needs " + std::to_string(lds_need) + " bytes of LDS (too many options or groups)";
