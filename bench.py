@@ -1,15 +1,22 @@
 """Benchmark of the rough-Bergomi Romano-Touzi pricing path on one MI355X.
 
-Workload (one "step" = one full pricing call over device-resident work buffers):
-  single parameter set at scale -- H = 0.1, eta = 1.5, rho = -0.7, xi = 0.04, N = 512 steps,
-  M = 2^22 paths, maturities {0.1, 0.25, 0.5, 1, 2} x 41 log-moneyness points in [-0.4, 0.2]
-  (205 options), Philox Gaussians generated on the device, FP64 price accumulation.
-Metric: paths*steps/s = M*N / seconds per pricing call.
+Default line -- the served mixed-precision pricer on the serving trace:
+  4096 calibration requests, each a parameter set (H ~ U[0.02, 0.45], eta ~ U[0.5, 4],
+  rho ~ U[-0.99, -0.1], xi ~ U[0.01, 0.16]) with 10-200 options over 1-5 maturities
+  T ~ U[0.02, 2] and log-moneyness U[-0.5, 0.3], a path budget M = 2^U, U ~ U[14, 21], and
+  N = max(32, ceil(256 T_max)) steps, priced by the continuous-batching server on the split-FP16
+  Toeplitz-GEMM kernel with Philox Gaussians generated on the device and FP64 price sums.
+  One "step" = the whole trace submitted at once and every result waited for (closed loop).
+Metric: paths*steps/s = sum of M*N over the trace / seconds per step, with the p50 / p99 request
+latency of the same trace under Poisson arrivals at 0.7 x the measured capacity, and the largest
+absolute price error against the FP64 path on the same Philox counters.
 
-Prints one JSON line (rank 0, N = 1 GPU only) with the roofline of the dominant kernel, timed
-live with HIP events on the plan's stream, and a CPU baseline: the CPU restatement of the
-reference's OpenMP driver (mt19937_64 Gaussians, per-path FFT convolution, options loop)
-timed on a bounded sample of the same workload on the host cores.
+The line carries the roofline of the dominant kernel (the fused pricing kernel of every batch,
+timed with HIP events on each batch's stream), the single-set workload (N = 512, M = 2^22,
+205 options) as a sub-object with its own roofline, and a CPU baseline: the CPU restatement of the
+reference's OpenMP driver (mt19937_64 Gaussians, per-path FFT convolution, options loop) timed on
+a bounded sample of the same trace on the host cores, with all the threads the host gives the job
+and with one thread.
 """
 import argparse
 import json
@@ -46,6 +53,57 @@ def load_profile(kernel):
     with open(path) as f:
         data = json.load(f)
     return data.get(kernel, {})
+
+
+def host_cpu():
+    """The host's CPU model and thread counts (lscpu, os)."""
+    model = ""
+    try:
+        import subprocess
+        for line in subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    return {"model": model, "nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+
+
+def cpu_threads():
+    """The host threads this job may use: OMP_NUM_THREADS where the host sets it (the GPU box
+    sets the share of one GPU's job), else every core of the affinity mask."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    n = len(os.sched_getaffinity(0))
+    return min(n, int(env)) if env.isdigit() and int(env) > 0 else n
+
+
+def cpu_time_requests(reqs, threads, m_scale):
+    """The oracle's restatement of the reference's OpenMP driver (compute_rt, FFT convolution)
+    over requests with M scaled by m_scale; returns (paths*steps, seconds)."""
+    from oracle import pyoracle as orc
+    work = 0
+    t0 = time.perf_counter()
+    for d in reqs:
+        M = max(1, int(round(d["M"] * m_scale)))
+        orc.compute_rt(d["xi"], d["H"], d["eta"], d["rho"], d["T"], d["K"], d["N"], M, d["seed"],
+                       num_threads=threads, conv=orc.CONV_FFT)
+        work += M * d["N"]
+    return work, time.perf_counter() - t0
+
+
+def cpu_baseline_serving(reqs, threads, n_multi=512, n_single=32, m_scale=1.0 / 256):
+    """CPU baseline of the serving trace: the first n_multi requests with every M scaled by
+    m_scale on `threads` threads, and the first n_single of them on one thread."""
+    w, dt = cpu_time_requests(reqs[:n_multi], threads, m_scale)
+    w1, dt1 = cpu_time_requests(reqs[:n_single], 1, m_scale)
+    return {"value": w / dt, "unit": "paths*steps/s", "cores": threads, "kind": "port",
+            "sample": "first %d requests of the serving trace with M scaled by 1/%d (%.3g paths*steps), "
+                      "oracle restatement of the reference's OpenMP driver (mt19937_64, per-path FFT "
+                      "convolution, options loop), %d OpenMP threads, %.1f s"
+                      % (n_multi, round(1 / m_scale), w, threads, dt),
+            "single_thread": {"value": w1 / dt1, "sample": "first %d requests, M scaled by 1/%d, %.1f s"
+                                                           % (n_single, round(1 / m_scale), dt1)},
+            "host": host_cpu()}
 
 
 def cpu_baseline(sample_paths, threads, method="rt"):
@@ -223,6 +281,148 @@ def bench_batched(rb, args):
     return out
 
 
+def poisson_latency(rb, ds, rate, seed=11):
+    """Poisson arrivals of the requests at `rate` per second through a fresh server with the
+    benchmark's settings: latency percentiles (submit to results copied) and throughput."""
+    import threading
+    from rbergomi_amd.serving import Request, Server
+    reqs = [Request(d["xi"], d["H"], d["eta"], d["rho"], d["T"], d["K"], d["N"], d["M"], d["seed"]) for d in ds]
+    gaps = np.random.default_rng(seed).exponential(1.0 / rate, len(reqs))
+    done = []
+    with Server(rb.PREC_SPLIT_FP16, max_inflight=SERVE_INFLIGHT, max_batch_requests=SERVE_MAX_BATCH,
+                max_batch_path_steps=SERVE_CAP) as srv:
+        def waiter():
+            for r in reqs:
+                while r.ticket is None:
+                    time.sleep(1e-4)
+                srv.wait(r)
+                done.append(r.latency_ms)
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        t0 = time.perf_counter()
+        t_next = t0
+        for r, g in zip(reqs, gaps):
+            t_next += g
+            dt = t_next - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            srv.submit(r)
+        th.join()
+        wall = time.perf_counter() - t0
+        rep = srv.report()
+    lat = np.array(done)
+    work = float(sum(d["M"] * d["N"] for d in ds))
+    return {"requests": len(ds), "offered_req_per_s": rate, "wall_s": wall,
+            "paths_steps_per_s": work / wall, "p50_ms": float(np.percentile(lat, 50)),
+            "p99_ms": float(np.percentile(lat, 99)), "batches": rep["batches"],
+            "peak_batches_in_flight": rep["peak_inflight"]}
+
+
+def serving_line(rb, args):
+    """The default line: the serving trace through the continuous-batching server, closed loop;
+    then its latency under Poisson arrivals and its price error against the FP64 path."""
+    from rbergomi_amd.serving import Request, Server
+    ds = serving_requests(args.requests)
+    work = float(sum(d["M"] * d["N"] for d in ds))
+    reqs = [Request(d["xi"], d["H"], d["eta"], d["rho"], d["T"], d["K"], d["N"], d["M"], d["seed"]) for d in ds]
+    with Server(rb.PREC_SPLIT_FP16, max_inflight=SERVE_INFLIGHT, max_batch_requests=SERVE_MAX_BATCH,
+                max_batch_path_steps=SERVE_CAP) as srv:
+        def step():
+            for r in reqs:
+                srv.submit(r)
+            for r in reqs:  # every result copied back: the device work of the step has finished
+                srv.wait(r)
+
+        for _ in range(args.warmup):
+            step()
+        r0 = srv.report()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        t1 = time.perf_counter()
+        r1 = srv.report()
+    s_step = (t1 - t0) / args.steps
+    assert abs((r1["path_steps"] - r0["path_steps"]) - work * args.steps) <= 1e-9 * work * args.steps
+    out = {"metric": "paths*steps/s", "value": work / s_step, "unit": "paths*steps/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": s_step * 1e3, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f16x2+f32",
+           "data": "synthetic (seeded parameter trace, device Philox4x32-10 Gaussians)",
+           "config": {"workload": "serving trace: %d calibration requests (10-200 options over 1-5 maturities, "
+                                  "M = 2^14..2^21, N = max(32, 256 T)), continuous batching, closed loop"
+                                  % len(ds),
+                      "requests": len(ds), "paths_steps_per_step": work, "precision": "fp16",
+                      "server": {"max_inflight": SERVE_INFLIGHT, "max_batch_requests": SERVE_MAX_BATCH,
+                                 "max_batch_path_steps": SERVE_CAP},
+                      "parallelism": "1 GPU"},
+           "batches_per_step": (r1["batches"] - r0["batches"]) / args.steps,
+           "peak_batches_in_flight": r1["peak_inflight"]}
+    if not args.no_latency:
+        lat = poisson_latency(rb, ds, 0.7 * len(ds) / s_step)
+        out["latency"] = lat
+        out["p50_ms"], out["p99_ms"] = lat["p50_ms"], lat["p99_ms"]
+    if not args.no_check:
+        # the served results against the FP64 path on the same Philox counters (every 8th request;
+        # each request's results are the same alone and in any mix)
+        # the same loop times the plain FP64 single-request GPU path (rb_price_v2, one request
+        # after another) on these requests: the path the served pricer is compared with
+        errs = []
+        sub = list(zip(ds, reqs))[::8]
+        t64 = time.perf_counter()
+        for dd, r in sub:
+            p64 = rb.price_v2(dd["xi"], dd["H"], dd["eta"], dd["rho"], dd["T"], dd["K"], dd["N"], dd["M"], dd["seed"])[0]
+            errs.append(float(np.max(np.abs(r.price - p64))))
+        t64 = time.perf_counter() - t64
+        w64 = float(sum(dd["M"] * dd["N"] for dd, _ in sub))
+        out["accuracy"] = {"max_abs_price_err_vs_fp64": max(errs), "median_abs_price_err_vs_fp64": float(np.median(errs)),
+                           "requests_checked": len(errs), "reference": "FP64 path, same Philox counters"}
+        out["fp64_single_request"] = {"requests": len(sub), "s": t64, "paths_steps_per_s": w64 / t64,
+                                      "served_speedup": (work / s_step) / (w64 / t64),
+                                      "what": "every 8th request of the trace through rb_price_v2 (FP64), one "
+                                              "after another, including its host-side planning and copies"}
+    return out
+
+
+def serving_roofline(rb, args):
+    """The roofline of the serving trace's dominant kernel: one closed-loop pass of the trace
+    through a server with HIP timing events around every batch's kernel, on its stream.  The events
+    keep the batches' kernels from overlapping (25% of throughput on MI355X), so this pass runs
+    after everything else the line measures and its batches are timed one kernel at a time."""
+    from rbergomi_amd.serving import Request, Server
+    ds = serving_requests(args.requests)
+    work = float(sum(d["M"] * d["N"] for d in ds))
+    reqs = [Request(d["xi"], d["H"], d["eta"], d["rho"], d["T"], d["K"], d["N"], d["M"], d["seed"]) for d in ds]
+    with Server(rb.PREC_SPLIT_FP16, max_inflight=SERVE_INFLIGHT, max_batch_requests=SERVE_MAX_BATCH,
+                max_batch_path_steps=SERVE_CAP) as srv:
+        srv.set_timing(True)
+        for r in reqs:
+            srv.submit(r)
+        for r in reqs:
+            srv.wait(r)
+        d = srv.report()
+    avg_ms = d["kernel_ms"] / d["kernel_launches"]
+    flops_launch = d["conv_flops"] / d["kernel_launches"]
+    achieved = flops_launch / (avg_ms * 1e-3) / 1e12
+    prof = load_profile("k_fused_serving")
+    roof = {"bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS["fp16"], "unit": "TFLOP/s",
+            "frac": achieved / PEAK_TFLOPS["fp16"], "traffic": prof.get("bytes_per_launch"),
+            "kernel": "k_fused (the fused pricing kernel of every batch)", "avg_launch_ms": avg_ms,
+            "launches_per_step": d["kernel_launches"], "flops_per_launch": flops_launch,
+            "timing": "HIP events on each batch's stream around its kernel, in one closed-loop pass of the "
+                      "trace after the timed steps (the events serialize the batches' kernels, so the timed "
+                      "steps run without them)",
+            "timing_pass": {"kernel_ms": d["kernel_ms"], "device_busy_ms": d["device_busy_ms"],
+                            "batches": d["batches"],
+                            "achieved_over_busy_TFLOPs": d["conv_flops"] / (d["device_busy_ms"] * 1e-3) / 1e12}}
+    if prof.get("valu_insts_per_launch"):
+        v, tr = prof["valu_insts_per_launch"], prof["trans_f32_insts_per_launch"]
+        roof["valu"] = {"insts_per_launch": v, "trans_insts_per_launch": tr,
+                        "lane_insts_per_path_step": 64.0 * v * d["kernel_launches"] / work,
+                        "issue_busy_frac": (4.0 * v + 12.0 * tr) / 1024 / (avg_ms * 1e-3 * 2.4e9),
+                        "source": "SQ_INSTS_VALU, SQ_INSTS_VALU_TRANS_F32 (rocprofv3 --pmc), %s" % prof.get("source")}
+    return roof
+
+
 def bench_serving(rb, args):
     reqs = serving_requests(args.requests)
     tuned = tune(rb, [d["N"] for d in reqs], args, (("fp16", rb.PREC_SPLIT_FP16),)) if args.autotune else None
@@ -276,39 +476,9 @@ def bench_long(rb, args):
             "rows": rows, "vs_baseline": None, "scaling": "weak", "data": "synthetic"}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--precision", default="fp16", choices=sorted(PREC_CODE))
-    ap.add_argument("--workload", default="single", choices=["single", "batched", "serving", "long"])
-    ap.add_argument("--method", default="rt", choices=["rt", "mc"],
-                    help="single workload: Romano-Touzi (computeIVRT_tensor) or plain Monte Carlo "
-                         "(computeIV_tensor)")
-    ap.add_argument("--requests", type=int, default=4096, help="serving workload size")
-    ap.add_argument("--paths", type=int, default=2 ** 22)
-    ap.add_argument("--cpu-sample-paths", type=int, default=2 ** 18)
-    ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-check", action="store_true", help="skip the FP64 comparison run")
-    ap.add_argument("--no-latency", action="store_true", help="skip the serving latency sample")
-    ap.add_argument("--autotune", action="store_true",
-                    help="batched / serving workloads: autotune the fused kernel's tile shapes first")
-    ap.add_argument("--copy-probe", type=int, default=0,
-                    help="also run the streaming copy probe over this many MiB (traffic calibration)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(paths would shard by range with one all-reduce of the option sums)", file=sys.stderr)
-        sys.exit(2)
-
-    import rbergomi_amd as rb
-
-    if args.workload != "single":
-        fn = {"batched": bench_batched, "serving": bench_serving, "long": bench_long}[args.workload]
-        print(json.dumps(fn(rb, args)), flush=True)
-        return
+def single_line(rb, args, sub=False):
+    """The single-set workload (N = 512, M = 2^22, 205 options) through a plan; sub: as a
+    sub-object of the default line (no latency sample, no CPU baseline)."""
     w = workload(args.paths)
     method = rb.METHOD_RT if args.method == "rt" else rb.METHOD_MC
     plan = rb.Plan(w["xi"], w["H"], w["eta"], w["rho"], w["T"], w["K"], w["N"], w["M"],
@@ -391,23 +561,81 @@ def main():
                            "max_abs_stat_err_vs_fp64": float(np.max(np.abs(stat - s64))),
                            "max_price_stat_ratio": float(np.max(np.abs(price - p64) / s64)),
                            "reference": "staged FP64 path, same Philox counters"}
-    if not args.no_latency and args.precision in ("fp16", "bf16", "fp32gemm") and args.method == "rt":
+    if not sub and not args.no_latency and args.precision in ("fp16", "bf16", "fp32gemm") and args.method == "rt":
         # request latency under load: 256 serving requests, Poisson arrivals at 0.7 x capacity
         lat = run_serving(rb, serving_requests(256), PREC_CODE[args.precision])
         out["latency"] = {k: lat[k] for k in ("requests", "p50_ms", "p99_ms", "offered_req_per_s",
                                                "open_loop_paths_steps_per_s", "batches")}
         out["p99_ms"] = lat["p99_ms"]
     if args.copy_probe:
-        import ctypes
-        ms_copy = ctypes.c_double()
-        nbytes = args.copy_probe << 20
-        assert rb.load().rb_probe_copy(nbytes, 5, ctypes.byref(ms_copy)) == 0
-        out["copy_probe"] = {"bytes": nbytes, "avg_ms": ms_copy.value,
-                             "GB_per_s": 2 * nbytes / (ms_copy.value * 1e-3) / 1e9}
-    if not args.no_cpu:
-        threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
-        out["cpu_baseline"] = cpu_baseline(args.cpu_sample_paths, threads, args.method)
-    print(json.dumps(out), flush=True)
+        out["copy_probe"] = copy_probe(rb, args.copy_probe)
+    if not sub and not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_sample_paths, args.cpu_threads or cpu_threads(), args.method)
+    return out
+
+
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--precision", default="fp16", choices=sorted(PREC_CODE))
+    ap.add_argument("--workload", default="trace", choices=["trace", "single", "batched", "serving", "long"],
+                    help="trace: the serving trace, closed loop (the default line); single: the single-set "
+                         "workload; batched: 256 sets in one launch; serving: Poisson latency sweep; long: "
+                         "long paths")
+    ap.add_argument("--method", default="rt", choices=["rt", "mc"],
+                    help="single workload: Romano-Touzi (computeIVRT_tensor) or plain Monte Carlo "
+                         "(computeIV_tensor)")
+    ap.add_argument("--requests", type=int, default=4096, help="serving trace size")
+    ap.add_argument("--paths", type=int, default=2 ** 22)
+    ap.add_argument("--single-steps", type=int, default=10, help="trace: timed steps of the single-set sub-line")
+    ap.add_argument("--cpu-sample-paths", type=int, default=2 ** 18)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-single", action="store_true", help="trace: skip the single-set sub-line")
+    ap.add_argument("--no-check", action="store_true", help="skip the FP64 comparison")
+    ap.add_argument("--no-latency", action="store_true", help="skip the latency sample")
+    ap.add_argument("--autotune", action="store_true",
+                    help="batched / serving workloads: autotune the fused kernel's tile shapes first")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="also run the streaming copy probe over this many MiB (traffic calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(paths would shard by range with one all-reduce of the option sums)", file=sys.stderr)
+        sys.exit(2)
+
+    import rbergomi_amd as rb
+
+    if args.workload == "trace":
+        out = serving_line(rb, args)
+        if not args.no_single:
+            sa = argparse.Namespace(**vars(args))
+            sa.steps, sa.warmup, sa.precision, sa.method, sa.paths = args.single_steps, 2, "fp16", "rt", 2 ** 22
+            out["single_set"] = single_line(rb, sa, sub=True)
+        out["roofline"] = serving_roofline(rb, args)
+        if args.copy_probe:
+            out["copy_probe"] = copy_probe(rb, args.copy_probe)
+        if not args.no_cpu:
+            out["cpu_baseline"] = cpu_baseline_serving(serving_requests(args.requests), args.cpu_threads or cpu_threads())
+        print(json.dumps(out), flush=True)
+        return
+    if args.workload != "single":
+        fn = {"batched": bench_batched, "serving": bench_serving, "long": bench_long}[args.workload]
+        print(json.dumps(fn(rb, args)), flush=True)
+        return
+    print(json.dumps(single_line(rb, args)), flush=True)
+
+
+def copy_probe(rb, mib):
+    import ctypes
+    ms_copy = ctypes.c_double()
+    nbytes = mib << 20
+    assert rb.load().rb_probe_copy(nbytes, 5, ctypes.byref(ms_copy)) == 0
+    return {"bytes": nbytes, "avg_ms": ms_copy.value, "GB_per_s": 2 * nbytes / (ms_copy.value * 1e-3) / 1e9}
 
 
 if __name__ == "__main__":

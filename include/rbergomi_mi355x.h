@@ -226,6 +226,27 @@ typedef struct rb_request {
  * rb_price_v2 and the same cfg.  The server prices Romano-Touzi requests. */
 int rb_price_batch(const rb_request* reqs, int n, const rb_config* cfg);
 
+/* A batch planned once and priced many times, e.g. by a calibration loop that re-prices the same
+ * request shapes with new parameters.  No counterpart in the reference, whose driver sets up its
+ * OpenMP team and tables on every call (rBergLib/rBergomiMT.cpp:340-355).  rb_batch_create plans
+ * the requests, uploads their tables and allocates every buffer; each rb_batch_run prices all of
+ * them (short requests in as few fused launches as one workgroup's LDS allows, long ones on the
+ * FFT kernel, all on `stream`, NULL: the batch's own) and copies the results into the requests'
+ * output arrays, which must stay valid for the batch's life.  Results equal rb_price_batch's. */
+typedef struct rb_batch rb_batch;
+int rb_batch_create(const rb_request* reqs, int n, const rb_config* cfg, void* stream, rb_batch** out);
+/* New parameters (xi, H, eta, rho, T, K, seed) and output arrays for every request, with the same
+ * N, M and option counts as at creation and the same table layout (distinct H, maturities and rho
+ * per request): the tables of the short requests are copied into the batch's device buffers, so a
+ * recorded graph keeps replaying valid launches; long-path requests are rebuilt and the graph is
+ * recorded again on the next run.  RB_ERR_ARG if the layout changes. */
+int rb_batch_update(rb_batch* b, const rb_request* reqs, int n);
+/* Graph replay (enable = 1): the next rb_batch_run records the batch's launches into a hipGraph
+ * after one eager run, and every later run replays it (no per-launch host overhead). */
+int rb_batch_set_graph(rb_batch* b, int enable);
+int rb_batch_run(rb_batch* b);
+void rb_batch_destroy(rb_batch* b);
+
 /* Tile-shape autotuning of the fused kernel.  No counterpart in the reference (which has no
  * tiles).  A tile is 32 PB paths by all steps (PB = 1, 2 or 4 path blocks of 32; the four waves
  * of a workgroup split 4 / PB per path block).  For each N[i] (1..1024) this times every valid
@@ -262,7 +283,11 @@ typedef struct rb_server_config {
                                    * it runs as a batch of its own.  Short batches bound how long a
                                    * small request waits behind large ones. */
 } rb_server_config;
+/* Romano-Touzi requests, the reference compensator. */
 int rb_server_create(const rb_server_config* cfg, rb_server** out);
+/* The same with the estimator (RB_METHOD_RT, or RB_METHOD_MC: computeIV_tensor's plain Monte
+ * Carlo) and the compensator every request of the server is priced with. */
+int rb_server_create_method(const rb_server_config* cfg, int method, int compensator, rb_server** out);
 /* Copies the inputs; the output arrays must stay valid until rb_server_wait returns for the
  * ticket.  Returns RB_ERR_LIMIT for a request that can never fit the device budget. */
 int rb_server_submit(rb_server* s, const rb_request* req, long* ticket);
@@ -272,7 +297,39 @@ int rb_server_wait(rb_server* s, long ticket, double* latency_ms);
  * priced, and the number of batches launched. */
 int rb_server_stats(rb_server* s, double* p50_ms, double* p99_ms, long* completed,
                     double* path_steps, long* batches);
+/* Timing of the batches' pricing kernels (enable = 1; off by default): each batch launched while
+ * it is on brackets its kernel with HIP timing events on its stream, giving an interval [begin,
+ * end] on the device clock for rb_server_report_get.  The events keep the batches' kernels from
+ * overlapping on the device (measured on MI355X: 25% less serving throughput), so this is a
+ * measurement mode, not for production. */
+int rb_server_set_timing(rb_server* s, int enable);
+/* What the server has done so far.  The kernel fields cover the batches launched with timing on
+ * (zero otherwise): their sum, union, time covered by two or more and the most at one instant. */
+typedef struct rb_server_report {
+    double p50_ms, p99_ms;     /* request latency percentiles (submit to results copied)        */
+    long completed;            /* requests completed                                           */
+    double path_steps;         /* sum of M*N over them                                         */
+    long batches;              /* batches launched                                             */
+    long peak_inflight;        /* most batches launched and not yet retired at once (host)     */
+    long kernel_launches;      /* pricing-kernel intervals recorded                            */
+    double kernel_ms;          /* sum of the pricing-kernel intervals                          */
+    double device_busy_ms;     /* their union                                                  */
+    double device_overlap_ms;  /* time covered by two or more of them                          */
+    double device_span_ms;     /* first begin to last end                                      */
+    long max_concurrent;       /* most intervals covering one instant                          */
+    double conv_flops;         /* algorithmic Toeplitz flops M N (N-1) per distinct H, summed  */
+} rb_server_report;
+int rb_server_report_get(rb_server* s, rb_server_report* out);
 void rb_server_destroy(rb_server* s);
+
+/* Implied vols to the reference's tolerance, on the device: IV_call of rBergLib/BlackScholes.cpp:
+ * 36-92 (S0 = 1, r = 0; K < 1 solved on the put by put-call parity) with Boost's bracketing from
+ * the guess 0.2 by the factor 2 (at most 20 evaluations) and the bracket's midpoint once its
+ * relative width is 2^-10 (eps_tolerance(11)).  Boost narrows the bracket with TOMS 748, this
+ * routine by halving it, so the values are those the reference returns to within its tolerance,
+ * not digit for digit.  NaN where the reference's solver throws.  The pricing entry points solve
+ * to 1e-14 relative instead; this one reproduces what the reference's callers see. */
+int rb_iv_reference(const double* price, const double* K, const double* T, int n, double* iv);
 
 /* Host-only planning (no device call): group index of every option I and the unit of every
  * group, as the executor deduplicates them.  group_of_option has out_size entries,

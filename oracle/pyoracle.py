@@ -48,7 +48,7 @@ def lib():
         for fn in ("orc_phi",):
             getattr(L, fn).argtypes = [d]
             getattr(L, fn).restype = d
-        for fn in ("orc_bs_call", "orc_bs_put", "orc_iv_call"):
+        for fn in ("orc_bs_call", "orc_bs_put", "orc_iv_call", "orc_iv_call_reference"):
             getattr(L, fn).argtypes = [d, d, d, d]
             getattr(L, fn).restype = d
         L.orc_rt_payoff.argtypes = [d, d, d, d]
@@ -126,6 +126,10 @@ def bs_put(S0, K, tau, sigma):
 
 def iv_call(price, S0, K, tau):
     return lib().orc_iv_call(price, S0, K, tau)
+
+
+def iv_call_reference(price, S0, K, tau):
+    return lib().orc_iv_call_reference(price, S0, K, tau)
 
 
 def rt_payoff(Iv, Is, rho, K):

@@ -290,6 +290,59 @@ double orc_iv_call(double price, double S0, double K, double tau) {
     return s;
 }
 
+double orc_iv_call_reference(double price, double S0, double K, double tau) {
+    /* IV_call / IV_put of rBergLib/BlackScholes.cpp:36-92 to the reference's tolerance.  S0 > K:
+     * the put through put-call parity (:38-41, r = 0).  Boost's bracket_and_solve_root on
+     * f(sigma) = target - BS(sigma) (decreasing, is_rising = false, :47-60): from guess = 0.2 the
+     * bracket walks by factor = 2 -- down while f < 0, up while f >= 0 -- for at most maxit = 20
+     * evaluations, then narrows until |b - a| <= 2^(1 - 11) min(|a|, |b|) (eps_tolerance(11)),
+     * and the midpoint a + (b - a)/2 is returned (:63).  Boost narrows with TOMS 748 (Alefeld,
+     * Potra and Shi 1995); this restatement halves the bracket: the result lies within the same
+     * tolerance of the root, not on Boost's iterates.  NaN where Boost throws. */
+    bool call = true;
+    double target = price;
+    if (S0 > K) {
+        call = false;
+        target = price - S0 + K;
+    }
+    const double upper = call ? S0 : K;
+    if (!(target > 0.0) || !(target < upper) || !(tau > 0.0)) return std::nan("");
+    auto f = [&](double s) { return target - (call ? orc_bs_call(S0, K, tau, s) : orc_bs_put(S0, K, tau, s)); };
+    double a = 0.2, b = 0.2, fa = f(a), fb = fa;
+    int count = 20 - 1;
+    if (fa < 0.0) {
+        while ((fa < 0.0) == (fb < 0.0)) {
+            if (count-- == 0) return std::nan("");
+            b = a;
+            fb = fa;
+            a /= 2.0;
+            fa = f(a);
+        }
+    } else {
+        while ((fa < 0.0) == (fb < 0.0)) {
+            if (count-- == 0) return std::nan("");
+            a = b;
+            fa = fb;
+            b *= 2.0;
+            fb = f(b);
+        }
+    }
+    for (int it = 0; it < 200 && !(std::fabs(b - a) <= std::ldexp(1.0, -10) * std::fmin(std::fabs(a), std::fabs(b)));
+         ++it) {
+        if (fa == 0.0) return a;
+        if (fb == 0.0) return b;
+        const double m = a + 0.5 * (b - a), fm = f(m);
+        if ((fm < 0.0) == (fa < 0.0)) {
+            a = m;
+            fa = fm;
+        } else {
+            b = m;
+            fb = fm;
+        }
+    }
+    return a + (b - a) / 2;
+}
+
 double orc_rt_payoff(double Ivdt, double IsvdW, double rho, double K) {
     const double vol = std::sqrt((1.0 - rho * rho) * Ivdt);
     const double spot = std::exp(-0.5 * rho * rho * Ivdt + rho * IsvdW);

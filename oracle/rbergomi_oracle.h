@@ -23,6 +23,10 @@
  *                        E[S'] = 1 martingale identity within 4.5 SE                 (test_oracle_golden.py)
  *   orc_compute_mc       same known prices (the reference tests both methods against them)
  *   orc_iv_call          round trip IV(BS(sigma)) = sigma; put-call parity branch     (test_oracle_math.py)
+ *   orc_iv_call_reference  within 2^-11 relative of the exact IV (the tolerance's guarantee);
+ *                        a price of BS(0.2 * 2^k) gives 0.2 * 2^k exactly (a root on the bracket);
+ *                        vols that the 20 evaluations cannot bracket give NaN, as Boost throws;
+ *                        the parity branch                                              (test_oracle_math.py)
  *   orc_philox4x32_10    Random123 known-answer vectors                                (test_oracle_math.py)
  *   orc_normals_philox   distribution moments; equals Box-Muller on orc_philox4x32_10 outputs
  *   orc_turbo_rows       unbiased against the plain RT oracle (within 4.5 combined SE); eta = 0
@@ -91,6 +95,11 @@ double orc_bs_put(double S0, double K, double tau, double sigma);
  * Safeguarded Newton to |dsigma| <= 1e-14*sigma instead of Boost's 2^-10 bracket.
  * Returns NaN when the price is outside the no-arbitrage range. */
 double orc_iv_call(double price, double S0, double K, double tau);
+
+/* implied vol to the reference's tolerance: Boost's bracketing from 0.2 by the factor 2 (at most
+ * 20 evaluations), the midpoint of a bracket of relative width 2^-10 (rBergLib/BlackScholes.cpp:
+ * 36-92); the bracket is halved where Boost runs TOMS 748.  NaN where Boost throws. */
+double orc_iv_call_reference(double price, double S0, double K, double tau);
 
 /* Romano-Touzi conditional payoff of one path for one option, given the path integrals.
  * rBergLib/rBergomiMT.cpp:383-388 */

@@ -58,5 +58,17 @@ def build(verbose=False, jobs=4):
     return LIB
 
 
+def build_interface_test():
+    """The C++ caller of include/interface.h (tests/cpp/interface_test.cpp), compiled with g++ and
+    linked to the library; the GPU test runs it."""
+    root = os.path.abspath(os.path.join(HERE, ".."))
+    src = os.path.join(root, "tests", "cpp", "interface_test.cpp")
+    out = os.path.join(root, "tests", "cpp", "interface_test")
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I" + os.path.join(root, "include"), src, "-o", out,
+           "-L" + HERE, "-lrbergomi_mi355x", "-Wl,-rpath," + HERE, "-Wl,-rpath,$ORIGIN/../../rbergomi_amd"]
+    subprocess.check_call(cmd)
+    return out
+
+
 if __name__ == "__main__":
     print(build(verbose=True))

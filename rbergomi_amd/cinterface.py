@@ -93,6 +93,18 @@ def price_v2(xi, H, eta, rho, T, K, N, M, seed, precision=_lib.PREC_FP64,
     return price, iv, stat
 
 
+def iv_reference(price, K, T):
+    """rb_iv_reference: implied vols to the reference's Boost tolerance (bracket by the factor 2
+    from 0.2, midpoint of a bracket of relative width 2^-10)."""
+    price, pp = _arr(price)
+    K, pK = _arr(np.broadcast_to(np.asarray(K, dtype=np.float64), price.shape))
+    T, pT = _arr(np.broadcast_to(np.asarray(T, dtype=np.float64), price.shape))
+    iv = np.zeros(len(price))
+    check(_lib.load().rb_iv_reference(pp, pK, pT, len(price), iv.ctypes.data_as(dp)))
+    return iv
+
+
+
 def payoff_samples_v2(xi, H, eta, rho, T, K, N, Z, method=_lib.METHOD_RT,
                       compensator=_lib.COMP_REF, scheme=0):
     """Per-path payoffs from user Gaussians -> (payoff [M, out], price, stat)"""

@@ -276,6 +276,25 @@ int rb_probe_norm_inv(const double* u, long n, double* z) {
     return e ? fail(RB_ERR_DEVICE, hipGetErrorString(e)) : RB_OK;
 }
 
+int rb_iv_reference(const double* price, const double* K, const double* T, int n, double* iv) {
+    set_last_error("");
+    if (!price || !K || !T || !iv || n < 1) return fail(RB_ERR_ARG, "bad arguments");
+    const size_t b = size_t(n) * sizeof(double);
+    double *dp = nullptr, *dk = nullptr, *dt = nullptr, *dv = nullptr;
+    hipError_t e = hipMalloc(&dp, b);
+    if (!e) e = hipMalloc(&dk, b);
+    if (!e) e = hipMalloc(&dt, b);
+    if (!e) e = hipMalloc(&dv, b);
+    if (!e) e = hipMemcpy(dp, price, b, hipMemcpyHostToDevice);
+    if (!e) e = hipMemcpy(dk, K, b, hipMemcpyHostToDevice);
+    if (!e) e = hipMemcpy(dt, T, b, hipMemcpyHostToDevice);
+    if (!e) e = launch_iv_ref_f64(dp, dk, dt, n, dv, nullptr);
+    if (!e) e = hipMemcpy(iv, dv, b, hipMemcpyDeviceToHost);
+    for (double* x : {dp, dk, dt, dv})
+        if (x) (void)hipFree(x);
+    return e ? fail(RB_ERR_DEVICE, hipGetErrorString(e)) : RB_OK;
+}
+
 int rb_payoff_samples_v2(double xi, const double* H, const double* eta, const double* rho,
                          int par_size, const double* T, const double* K, int op_size, int N,
                          long M, const double* Z, const rb_config* cfg, double* payoff,

@@ -15,6 +15,16 @@ struct PhiloxKey {
     uint32_t k0, k1;
 };
 
+// a ^ b ^ c: on the device one v_bitop3_b32 (truth table 0x96), which the compiler does not form
+// from two XORs by itself; the host pass (probes, tables) uses the operators
+__host__ __device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+#else
+    return a ^ b ^ c;
+#endif
+}
+
 // Philox4x32 with 10 rounds (Salmon, Moraes, Dror, Shaw, SC'11): multipliers 0xD2511F53,
 // 0xCD9E8D57, Weyl key increments 0x9E3779B9, 0xBB67AE85.
 __host__ __device__ __forceinline__ uint4 philox10(uint4 c, PhiloxKey key) {
@@ -24,9 +34,9 @@ __host__ __device__ __forceinline__ uint4 philox10(uint4 c, PhiloxKey key) {
         const uint64_t p0 = uint64_t(0xD2511F53u) * c.x;
         const uint64_t p1 = uint64_t(0xCD9E8D57u) * c.z;
         uint4 n;
-        n.x = uint32_t(p1 >> 32) ^ c.y ^ k0;
+        n.x = xor3(uint32_t(p1 >> 32), c.y, k0);
         n.y = uint32_t(p1);
-        n.z = uint32_t(p0 >> 32) ^ c.w ^ k1;
+        n.z = xor3(uint32_t(p0 >> 32), c.w, k1);
         n.w = uint32_t(p0);
         c = n;
         k0 += 0x9E3779B9u;
@@ -155,6 +165,64 @@ __device__ inline double implied_vol_f64(double price, double K, double tau) {
         if (ds <= 1e-14 * s || hi - lo <= 1e-15 * s) break;
     }
     return s;
+}
+
+// The reference's implied vol to the reference's tolerance (rBergLib/BlackScholes.cpp:36-92):
+// in-the-money calls (K < 1) are solved on the put through put-call parity, as IV_call does.  Its
+// solver is Boost.Math's bracket_and_solve_root on f(sigma) = target - BS(sigma), which decreases
+// in sigma (is_rising = false): from the guess 0.2 the bracket walks by the factor 2 -- down while
+// f < 0, up while f > 0 -- until f changes sign (at most maxit = 20 evaluations), then the bracket
+// is narrowed until eps_tolerance(11) holds, |b - a| <= 2^-10 min(a, b), and its midpoint is
+// returned.  Boost narrows with TOMS 748 (Alefeld, Potra and Shi, ACM TOMS 21(3), 1995); here the
+// bracket is halved, so the iterates differ from Boost's but the result carries the same
+// guarantee: the midpoint of a bracket of relative width 2^-10 around the root.  NaN where the
+// reference's solver throws (the price outside the no-arbitrage range, or no bracket within the
+// evaluations).
+__device__ inline double implied_vol_reference_f64(double price, double K, double tau) {
+    bool call = true;
+    double target = price;
+    if (1.0 > K) {
+        call = false;
+        target = price - 1.0 + K;
+    }
+    const double upper = call ? 1.0 : K;
+    if (!(target > 0.0) || !(target < upper) || !(tau > 0.0)) return __builtin_nan("");
+    double a = 0.2, b = 0.2;
+    double fa = target - bs_price_f64(K, tau, a, call), fb = fa;
+    int count = 20 - 1;
+    if (fa < 0.0) {
+        // the root lies below the guess: halve
+        while ((fa < 0.0) == (fb < 0.0)) {
+            if (count-- == 0) return __builtin_nan("");
+            b = a;
+            fb = fa;
+            a *= 0.5;
+            fa = target - bs_price_f64(K, tau, a, call);
+        }
+    } else {
+        while ((fa < 0.0) == (fb < 0.0)) {
+            if (count-- == 0) return __builtin_nan("");
+            a = b;
+            fa = fb;
+            b *= 2.0;
+            fb = target - bs_price_f64(K, tau, b, call);
+        }
+    }
+    // f(a) >= 0 > f(b) (or a root on an end point): narrow [a, b] to the tolerance
+    for (int it = 0; it < 200 && !(fabs(b - a) <= 0x1p-10 * fmin(fabs(a), fabs(b))); ++it) {
+        if (fa == 0.0) return a;
+        if (fb == 0.0) return b;
+        const double m = a + 0.5 * (b - a);
+        const double fm = target - bs_price_f64(K, tau, m, call);
+        if ((fm < 0.0) == (fa < 0.0)) {
+            a = m;
+            fa = fm;
+        } else {
+            b = m;
+            fb = fm;
+        }
+    }
+    return a + (b - a) / 2;
 }
 
 }  // namespace rb

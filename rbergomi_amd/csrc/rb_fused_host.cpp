@@ -335,13 +335,91 @@ static hipError_t alloc_async(T** p, size_t n, hipStream_t st) {
     return hipMallocAsync(reinterpret_cast<void**>(p), std::max<size_t>(n, 1) * sizeof(T), st);
 }
 
+// n elements of T from the arena (256-byte aligned), or a stream-ordered allocation without one
+template <class T>
+static hipError_t batch_alloc(DevArena* ar, T** p, size_t n, hipStream_t st) {
+    if (!ar) return alloc_async(p, n, st);
+    const size_t bytes = (std::max<size_t>(n, 1) * sizeof(T) + 255) & ~size_t(255);
+    if (ar->used + bytes > ar->size) return hipErrorOutOfMemory;
+    *p = reinterpret_cast<T*>(ar->base + ar->used);
+    ar->used += bytes;
+    return hipSuccess;
+}
+
+template <class T>
+static hipError_t batch_upload(DevArena* ar, T** p, const std::vector<T>& v, hipStream_t st) {
+    hipError_t e = batch_alloc(ar, p, v.size(), st);
+    if (e != hipSuccess || v.empty()) return e;
+    return hipMemcpyAsync(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st);
+}
+
+size_t fused_batch_lds(const std::vector<const FusedHostReq*>& reqs, size_t lds_cu, FusedBatchArgs* a, int* cap) {
+    int bf = 0, wp = 0, pg = 0, ac = 0, ns = 2;
+    for (const FusedHostReq* q : reqs) {
+        const FusedReq& r = q->r;
+        ns = r.mc ? 3 : 2;
+        bf = std::max(bf, r.PB * r.NK * 128);
+        wp = std::max(wp, 4 * r.gchunk * 32 * ns);
+        pg = std::max(pg, r.TP * r.pgmax);
+        ac = std::max(ac, r.acc_in_lds ? 2 * r.out_size : 0);
+    }
+    const size_t stat = sizeof(FusedReq) + 16;  // the kernel's static record and item index
+    const size_t half = lds_cu / 2;
+    const size_t rest = size_t(bf) * 16 + size_t(pg) * sizeof(PathPG) + size_t(ac) * 8 + stat;
+    int c_cap = 0;
+    if (rest + size_t(wp) * 8 > half) {
+        for (int c = FUSED_GREG; c >= 1 && c_cap == 0; --c)
+            if (rest + size_t(4 * c * 32 * ns) * 8 <= half) c_cap = c;
+        if (c_cap > 0) {
+            wp = 0;
+            for (const FusedHostReq* q : reqs) wp = std::max(wp, 4 * std::min(q->r.gchunk, c_cap) * 32 * ns);
+        }
+    }
+    if (a) {
+        a->bf_u4 = bf;
+        a->wpart_d = wp;
+        a->pgd_n = pg;
+        a->acc_d = ac;
+    }
+    if (cap) *cap = c_cap;
+    return rest + size_t(wp) * 8;
+}
+
+bool fused_batch_accepts(const std::vector<const FusedHostReq*>& reqs, size_t lds_block, size_t lds_cu) {
+    if (reqs.size() <= 1) return true;
+    const size_t with = fused_batch_lds(reqs, lds_cu, nullptr, nullptr);
+    if (with > lds_block) return false;
+    if (with <= lds_cu / 2) return true;
+    const std::vector<const FusedHostReq*> before(reqs.begin(), reqs.end() - 1);
+    return fused_batch_lds(before, lds_cu, nullptr, nullptr) > lds_cu / 2;
+}
+
+std::vector<std::pair<int, int>> FusedBatch::split(const std::vector<const FusedHostReq*>& reqs, size_t lds_block,
+                                                   size_t lds_cu, bool soft) {
+    std::vector<std::pair<int, int>> runs;
+    std::vector<const FusedHostReq*> cur;
+    int begin = 0;
+    for (int i = 0; i < int(reqs.size()); ++i) {
+        cur.push_back(reqs[i]);
+        const bool fits = soft ? fused_batch_accepts(cur, lds_block, lds_cu)
+                               : cur.size() == 1 || fused_batch_lds(cur, lds_cu, nullptr, nullptr) <= lds_block;
+        if (!fits) {
+            runs.push_back({begin, i});
+            begin = i;
+            cur.assign(1, reqs[i]);
+        }
+    }
+    if (!reqs.empty()) runs.push_back({begin, int(reqs.size())});
+    return runs;
+}
+
 FusedBatch::~FusedBatch() { release(); }
 
 void FusedBatch::release() {
     void* bufs[] = {d_reqs, d_fu, d_fg, d_fpg, d_fo, d_frag, d_ptab, d_item_req, d_item_local,
                     d_counter, d_partial, d_price, d_stat, d_iv, d_ivopts, d_row_off, d_row_n, d_row_M};
     for (void* b : bufs)
-        if (b) (void)hipFreeAsync(b, stream);
+        if (b && !arena) (void)hipFreeAsync(b, stream);
     d_reqs = nullptr;
     d_fu = nullptr;
     d_fg = nullptr;
@@ -370,9 +448,43 @@ size_t FusedBatch::footprint(const std::vector<const FusedHostReq*>& reqs) {
     return b;
 }
 
+// copies v into an existing device buffer of v.size() elements
+template <class T>
+static hipError_t batch_refill(T* p, const std::vector<T>& v, hipStream_t st) {
+    if (v.empty()) return hipSuccess;
+    return hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st);
+}
+
 int FusedBatch::build(const std::vector<const FusedHostReq*>& reqs, int mode_, int device,
-                      hipStream_t st, std::string& err) {
-    release();
+                      hipStream_t st, std::string& err, bool refill, bool check_only) {
+    if (refill && !d_reqs) {
+        err = "no batch to refill";
+        return ST_ERR_ARG;
+    }
+    if (!refill) release();
+    // a refill that fails its check, or only checks, leaves the batch as it was
+    struct Saved {
+        FusedBatch& b;
+        int mode, nreq, upw, greg, total_out;
+        bool mc;
+        hipStream_t stream;
+        std::vector<long> out_off;
+        FusedBatchArgs args;
+        bool keep = false;
+        ~Saved() {
+            if (keep) return;
+            b.mode = mode;
+            b.nreq = nreq;
+            b.upw = upw;
+            b.greg = greg;
+            b.total_out = total_out;
+            b.mc = mc;
+            b.stream = stream;
+            b.out_off = out_off;
+            b.args = args;
+        }
+    } saved{*this, mode, nreq, upw, greg, total_out, mc, stream, out_off, args};
+    saved.keep = !refill;
     stream = st;
     mode = mode_;
     nreq = int(reqs.size());
@@ -389,8 +501,6 @@ int FusedBatch::build(const std::vector<const FusedHostReq*>& reqs, int mode_, i
     std::vector<long> row_off, row_n, row_M;
     out_off.assign(nreq + 1, 0);
     long poff = 0;
-    size_t lds_need = 0;
-    int bf = 0, wp = 0, pg = 0, ac = 0;
     for (int q = 0; q < nreq; ++q) {
         const FusedHostReq& H = *reqs[q];
         FusedReq r = H.r;
@@ -425,14 +535,10 @@ int FusedBatch::build(const std::vector<const FusedHostReq*>& reqs, int mode_, i
         poff += long(r.out_size) * r.nitems * 2;
         out_off[q + 1] = out_off[q] + r.out_size;
         ivo.insert(ivo.end(), H.ivopts.begin(), H.ivopts.end());
-        bf = std::max(bf, r.PB * r.NK * 128);
         if (q > 0 && r.mc != R[0].mc) {
             err = "a batch holds requests of one estimator (Romano-Touzi or plain Monte Carlo)";
             return ST_ERR_ARG;
         }
-        wp = std::max(wp, 4 * r.gchunk * 32 * (r.mc ? 3 : 2));
-        pg = std::max(pg, r.TP * r.pgmax);
-        ac = std::max(ac, r.acc_in_lds ? 2 * r.out_size : 0);
         R.push_back(r);
     }
     mc = nreq > 0 && R[0].mc != 0;
@@ -442,38 +548,20 @@ int FusedBatch::build(const std::vector<const FusedHostReq*>& reqs, int mode_, i
         greg = fused_greg(gm);
     }
     total_out = int(out_off[nreq]);
-    args.bf_u4 = bf;
-    args.wpart_d = wp;
-    args.pgd_n = pg;
-    args.acc_d = ac;
     hipDeviceProp_t prop;
     RB_TRY(device_props(device, prop));
-    keep_pool_cached(device);
+    if (!arena) keep_pool_cached(device);
     // Two workgroups per CU need at most half the CU's LDS each, the static request record
-    // included.  When the cross-wave partials are what keeps a launch above that, they are summed
-    // in smaller chunks of groups (the same sums in the same order, so the same bits).
-    {
-        const size_t half = prop.maxSharedMemoryPerMultiProcessor / 2;
-        const size_t rest = size_t(bf) * 16 + size_t(pg) * sizeof(PathPG) + size_t(ac) * 8 +
-                            sizeof(FusedReq) + 16;
-        const int ns = mc ? 3 : 2;
-        if (rest + size_t(wp) * 8 > half) {
-            int cap = 0;
-            for (int c = FUSED_GREG; c >= 1 && cap == 0; --c)
-                if (rest + size_t(4 * c * 32 * ns) * 8 <= half) cap = c;
-            if (cap > 0) {
-                wp = 0;
-                for (FusedReq& r : R) {
-                    r.gchunk = std::min(r.gchunk, cap);
-                    wp = std::max(wp, 4 * r.gchunk * 32 * ns);
-                }
-                args.wpart_d = wp;
-            }
-        }
-    }
-    lds_need = fused_lds_bytes(args);
-    if (lds_need > prop.sharedMemPerBlock) {
-        err = "fused kernel needs " + std::to_string(lds_need) + " bytes of LDS (too many options or groups)";
+    // included; fused_batch_lds cuts the cross-wave partials' chunk of groups when that helps.
+    const size_t lds_cu = size_t(prop.maxSharedMemoryPerMultiProcessor ? prop.maxSharedMemoryPerMultiProcessor
+                                                                        : prop.sharedMemPerBlock);
+    int cap = 0;
+    const size_t lds_total = fused_batch_lds(reqs, lds_cu, &args, &cap);
+    if (cap > 0)
+        for (FusedReq& r : R) r.gchunk = std::min(r.gchunk, cap);
+    if (lds_total > prop.sharedMemPerBlock) {
+        err = "fused kernel needs " + std::to_string(lds_total) + " bytes of LDS (too many options or groups; "
+              "a request that fits alone may not fit with others in one launch)";
         return ST_ERR_LIMIT;
     }
     // work items, most expensive first
@@ -489,24 +577,65 @@ int FusedBatch::build(const std::vector<const FusedHostReq*>& reqs, int mode_, i
         item_local[i] = items[i].second.second;
     }
     args.total_items = long(items.size());
+    const std::vector<size_t> shape = {fu.size(), fg.size(), fpg.size(), fo.size(), frag.size(), ptab.size(),
+                                       ivo.size(), row_off.size(), item_req.size(), size_t(poff),
+                                       size_t(total_out), size_t(mode_), size_t(upw), size_t(greg), size_t(mc),
+                                       size_t(args.bf_u4), size_t(args.wpart_d), size_t(args.pgd_n),
+                                       size_t(args.acc_d)};
+    if (refill) {
+        // new parameters of the same shapes into the same buffers: the request records keep their
+        // pointers, so launches recorded in a graph stay valid
+        if (shape != shape_) {
+            err = "the new parameters change the batch's table layout (distinct H, maturities, rho, options "
+                  "or tile shapes); build a new batch";
+            return ST_ERR_ARG;
+        }
+        if (check_only) return ST_OK;
+        saved.keep = true;
+        RB_TRY(batch_refill(d_fu, fu, st));
+        RB_TRY(batch_refill(d_fg, fg, st));
+        RB_TRY(batch_refill(d_fpg, fpg, st));
+        RB_TRY(batch_refill(d_fo, fo, st));
+        RB_TRY(batch_refill(d_frag, frag, st));
+        RB_TRY(batch_refill(d_ptab, ptab, st));
+        RB_TRY(batch_refill(d_ivopts, ivo, st));
+        RB_TRY(batch_refill(d_row_off, row_off, st));
+        RB_TRY(batch_refill(d_row_n, row_n, st));
+        RB_TRY(batch_refill(d_row_M, row_M, st));
+        RB_TRY(batch_refill(d_item_req, item_req, st));
+        RB_TRY(batch_refill(d_item_local, item_local, st));
+        for (FusedReq& r : R) {
+            r.units = d_fu + size_t(r.units);
+            r.groups = d_fg + size_t(r.groups);
+            r.pgs = d_fpg + size_t(r.pgs);
+            r.opts = d_fo + size_t(r.opts);
+            r.frag = d_frag;
+            r.ptab = d_ptab;
+            r.partial = d_partial + size_t(r.partial);
+        }
+        RB_TRY(batch_refill(d_reqs, R, st));
+        RB_TRY(hipStreamSynchronize(st));
+        return ST_OK;
+    }
+    shape_ = shape;
     // upload
-    RB_TRY(upload_vec(&d_fu, fu, st));
-    RB_TRY(upload_vec(&d_fg, fg, st));
-    RB_TRY(upload_vec(&d_fpg, fpg, st));
-    RB_TRY(upload_vec(&d_fo, fo, st));
-    RB_TRY(upload_vec(&d_frag, frag, st));
-    RB_TRY(upload_vec(&d_ptab, ptab, st));
-    RB_TRY(upload_vec(&d_ivopts, ivo, st));
-    RB_TRY(upload_vec(&d_row_off, row_off, st));
-    RB_TRY(upload_vec(&d_row_n, row_n, st));
-    RB_TRY(upload_vec(&d_row_M, row_M, st));
-    RB_TRY(upload_vec(&d_item_req, item_req, st));
-    RB_TRY(upload_vec(&d_item_local, item_local, st));
-    RB_TRY(alloc_async(&d_partial, size_t(poff), st));
-    RB_TRY(alloc_async(&d_counter, 1, st));
-    RB_TRY(alloc_async(&d_price, size_t(total_out), st));
-    RB_TRY(alloc_async(&d_stat, size_t(total_out), st));
-    RB_TRY(alloc_async(&d_iv, size_t(total_out), st));
+    RB_TRY(batch_upload(arena, &d_fu, fu, st));
+    RB_TRY(batch_upload(arena, &d_fg, fg, st));
+    RB_TRY(batch_upload(arena, &d_fpg, fpg, st));
+    RB_TRY(batch_upload(arena, &d_fo, fo, st));
+    RB_TRY(batch_upload(arena, &d_frag, frag, st));
+    RB_TRY(batch_upload(arena, &d_ptab, ptab, st));
+    RB_TRY(batch_upload(arena, &d_ivopts, ivo, st));
+    RB_TRY(batch_upload(arena, &d_row_off, row_off, st));
+    RB_TRY(batch_upload(arena, &d_row_n, row_n, st));
+    RB_TRY(batch_upload(arena, &d_row_M, row_M, st));
+    RB_TRY(batch_upload(arena, &d_item_req, item_req, st));
+    RB_TRY(batch_upload(arena, &d_item_local, item_local, st));
+    RB_TRY(batch_alloc(arena, &d_partial, size_t(poff), st));
+    RB_TRY(batch_alloc(arena, &d_counter, 1, st));
+    RB_TRY(batch_alloc(arena, &d_price, size_t(total_out), st));
+    RB_TRY(batch_alloc(arena, &d_stat, size_t(total_out), st));
+    RB_TRY(batch_alloc(arena, &d_iv, size_t(total_out), st));
     for (FusedReq& r : R) {
         r.units = d_fu + size_t(r.units);
         r.groups = d_fg + size_t(r.groups);
@@ -516,29 +645,29 @@ int FusedBatch::build(const std::vector<const FusedHostReq*>& reqs, int mode_, i
         r.ptab = d_ptab;
         r.partial = d_partial + size_t(r.partial);
     }
-    RB_TRY(upload_vec(&d_reqs, R, st));
+    RB_TRY(batch_upload(arena, &d_reqs, R, st));
     args.reqs = d_reqs;
     args.item_req = d_item_req;
     args.item_local = d_item_local;
     args.counter = d_counter;
-    const size_t lds_cu = size_t(prop.maxSharedMemoryPerMultiProcessor ? prop.maxSharedMemoryPerMultiProcessor
-                                                                        : prop.sharedMemPerBlock);
-    const int per_cu = int(std::max<size_t>(1, std::min<size_t>(2, lds_cu / lds_need)));
+    const int per_cu = int(std::max<size_t>(1, std::min<size_t>(2, lds_cu / lds_total)));
     grid = int(std::min<long>(args.total_items, long(prop.multiProcessorCount) * per_cu));
     RB_TRY(hipStreamSynchronize(st));
     return ST_OK;
 }
 
-int FusedBatch::launch(hipStream_t st, std::string& err, Plan* timing) {
+int FusedBatch::launch(hipStream_t st, std::string& err, Plan* timing, hipEvent_t kbeg, hipEvent_t kend) {
     auto timed = [&](int id) { return timing ? timing->timed(id) : -1; };
     auto mark = [&](int slot) {
         if (timing) timing->mark(slot, nullptr, nullptr);
     };
     int s;
     RB_TRY(launch_zero_int(d_counter, 1, st));
+    if (kbeg) RB_TRY(hipEventRecord(kbeg, st));
     s = timed(K_FUSED);
     RB_TRY(launch_fused(mode, upw, mc, greg, args, grid, st));
     mark(s);
+    if (kend) RB_TRY(hipEventRecord(kend, st));
     s = timed(K_REDUCE);
     RB_TRY(launch_reduce_rows(d_partial, d_row_off, d_row_n, d_row_M, total_out, d_price, d_stat, st));
     mark(s);
@@ -701,16 +830,18 @@ int FftFused::build(const Request& req, const Layout& lay, int device, hipStream
     return ST_OK;
 }
 
-int FftFused::launch(hipStream_t st, std::string& err, Plan* timing) {
+int FftFused::launch(hipStream_t st, std::string& err, Plan* timing, hipEvent_t kbeg, hipEvent_t kend) {
     auto timed = [&](int id) { return timing ? timing->timed(id) : -1; };
     auto mark = [&](int slot) {
         if (timing) timing->mark(slot, nullptr, nullptr);
     };
     int s;
     RB_TRY(launch_zero_int(d_counter, 1, st));
+    if (kbeg) RB_TRY(hipEventRecord(kbeg, st));
     s = timed(K_FUSED);
     RB_TRY(launch_fused_fft(args, grid, st));
     mark(s);
+    if (kend) RB_TRY(hipEventRecord(kend, st));
     s = timed(K_REDUCE);
     RB_TRY(launch_reduce_rows(d_partial, d_row_off, d_row_n, d_row_M, total_out, d_price, d_stat, st));
     mark(s);

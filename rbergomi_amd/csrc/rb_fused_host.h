@@ -67,7 +67,8 @@ struct FftFused {
     static int plan(const Request& req, const Layout& lay, size_t lds_limit, size_t& footprint,
                     std::string& err);
     int build(const Request& req, const Layout& lay, int device, hipStream_t st, std::string& err);
-    int launch(hipStream_t st, std::string& err, Plan* timing = nullptr);
+    int launch(hipStream_t st, std::string& err, Plan* timing = nullptr, hipEvent_t kbeg = nullptr,
+               hipEvent_t kend = nullptr);
     int results(double* price, double* iv, double* stat, hipStream_t st, std::string& err);
 };
 
@@ -85,6 +86,28 @@ std::vector<std::array<int, 4>> tune_table();
 // and records the fastest; chosen[i] / best_ms[i] report it for Ns[i].
 int autotune_fused(const Request& shape, const std::vector<int>& Ns, int mode, int device,
                    std::vector<int>& chosen, std::vector<double>& best_ms, std::string& err);
+
+// Device memory handed to a batch by its caller: buffers are carved from [base, base + size) at
+// 256-byte alignment instead of being allocated, and nothing is freed on release.
+struct DevArena {
+    char* base = nullptr;
+    size_t size = 0;
+    size_t used = 0;
+};
+
+// LDS of one fused launch over `reqs`: the largest W1 tile, cross-wave partials, per-path records
+// and option sums of its requests.  When the partials are what keeps the launch above half of a
+// CU's LDS (lds_cu / 2), their chunk of groups is cut so that two workgroups share the CU (the
+// same sums in the same order, so the same bits); *cap receives that chunk cap (0: none).  The
+// LDS fields of *a are set; returns the bytes one workgroup needs, the kernel's static request
+// record included.
+size_t fused_batch_lds(const std::vector<const FusedHostReq*>& reqs, size_t lds_cu, FusedBatchArgs* a, int* cap);
+
+// Whether the last request of `reqs` may join the launch of the others: the launch must fit one
+// workgroup's LDS (lds_block), and a launch that fits two workgroups per CU (half of lds_cu) is
+// not pushed above that, since its requests then run on half the waves.  (A request refused here
+// goes to another launch, which the server runs alongside on its own stream.)
+bool fused_batch_accepts(const std::vector<const FusedHostReq*>& reqs, size_t lds_block, size_t lds_cu);
 
 // Any number of requests priced by one launch of the fused kernel.
 struct FusedBatch {
@@ -113,14 +136,27 @@ struct FusedBatch {
     long* d_row_n = nullptr;
     long* d_row_M = nullptr;
 
+    DevArena* arena = nullptr;   // when set, buffers come from it (see DevArena)
+
     ~FusedBatch();
     void release();
     // device bytes the batch would allocate
     static size_t footprint(const std::vector<const FusedHostReq*>& reqs);
+    std::vector<size_t> shape_;  // table sizes and launch shape of the last build (see refill)
+    // refill: the batch exists and reqs have the same shapes; their tables are copied into the
+    // existing device buffers (pointers, launch shape and work items unchanged).  check_only: only
+    // whether a refill with reqs would succeed (nothing changes)
     int build(const std::vector<const FusedHostReq*>& reqs, int mode, int device, hipStream_t st,
-              std::string& err);
-    // enqueue: zero the item counter, fused kernel, ordered reduction, implied vols
-    int launch(hipStream_t st, std::string& err, Plan* timing = nullptr);
+              std::string& err, bool refill = false, bool check_only = false);
+    // LDS-aware split of reqs into consecutive runs that each fit one launch on a device with
+    // lds_block bytes per workgroup and lds_cu per CU: [begin, end) index ranges
+    // (soft: also keep launches that fit two workgroups per CU there, fused_batch_accepts)
+    static std::vector<std::pair<int, int>> split(const std::vector<const FusedHostReq*>& reqs, size_t lds_block,
+                                                  size_t lds_cu, bool soft);
+    // enqueue: zero the item counter, fused kernel, ordered reduction, implied vols; kbeg / kend
+    // (may be null) are recorded on st right before and after the fused kernel
+    int launch(hipStream_t st, std::string& err, Plan* timing = nullptr, hipEvent_t kbeg = nullptr,
+               hipEvent_t kend = nullptr);
     // enqueue the copies of request q's results (caller synchronizes the stream)
     int results(int q, double* price, double* iv, double* stat, hipStream_t st, std::string& err);
 };

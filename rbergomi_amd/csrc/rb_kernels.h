@@ -54,6 +54,8 @@ hipError_t launch_reduce_f64(const double* partial, long nblk, long M, int out_s
 hipError_t launch_reduce_rows(const double* partial, const long* row_off, const long* row_n,
                               const long* row_M, int nrows, double* price, double* stat,
                               hipStream_t st);
+hipError_t launch_iv_ref_f64(const double* price, const double* K, const double* T, int n, double* iv,
+                             hipStream_t st);
 hipError_t launch_iv_f64(const double* price, const Option* opts, int n, double* iv,
                          hipStream_t st);
 hipError_t launch_philox_probe(const void* ctr, PhiloxKey key, long n, void* out, hipStream_t st);

@@ -352,6 +352,13 @@ __global__ void k_iv_f64(const double* __restrict__ price, const Option* __restr
     if (I < n) iv[I] = implied_vol_f64(price[I], opts[I].K, opts[I].T);
 }
 
+// implied vols to the reference's tolerance (implied_vol_reference_f64)
+__global__ void k_iv_ref_f64(const double* __restrict__ price, const double* __restrict__ K,
+                             const double* __restrict__ T, int n, double* __restrict__ iv) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) iv[i] = implied_vol_reference_f64(price[i], K[i], T[i]);
+}
+
 // Philox known-answer probe: out[4*t..] = Philox4x32-10(ctr[t], key)
 __global__ void k_philox_probe(const uint4* __restrict__ ctr, PhiloxKey key, long n,
                                uint4* __restrict__ out) {
@@ -484,6 +491,12 @@ hipError_t launch_reduce_rows(const double* partial, const long* row_off, const 
 hipError_t launch_iv_f64(const double* price, const Option* opts, int n, double* iv,
                          hipStream_t st) {
     hipLaunchKernelGGL(k_iv_f64, dim3((n + 63) / 64), dim3(64), 0, st, price, opts, n, iv);
+    return hipGetLastError();
+}
+
+hipError_t launch_iv_ref_f64(const double* price, const double* K, const double* T, int n, double* iv,
+                             hipStream_t st) {
+    hipLaunchKernelGGL(k_iv_ref_f64, dim3((n + 63) / 64), dim3(64), 0, st, price, K, T, n, iv);
     return hipGetLastError();
 }
 

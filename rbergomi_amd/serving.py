@@ -24,12 +24,32 @@ class RbServerConfig(ctypes.Structure):
                 ("max_batch_path_steps", ctypes.c_double)]
 
 
+class RbServerReport(ctypes.Structure):
+    _fields_ = [("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double), ("completed", ctypes.c_long),
+                ("path_steps", ctypes.c_double), ("batches", ctypes.c_long), ("peak_inflight", ctypes.c_long),
+                ("kernel_launches", ctypes.c_long), ("kernel_ms", ctypes.c_double),
+                ("device_busy_ms", ctypes.c_double), ("device_overlap_ms", ctypes.c_double),
+                ("device_span_ms", ctypes.c_double), ("max_concurrent", ctypes.c_long),
+                ("conv_flops", ctypes.c_double)]
+
+
 def _declare(L):
     if getattr(L, "_serving_declared", False):
         return
     vp = ctypes.c_void_p
     L.rb_price_batch.argtypes = [ctypes.POINTER(RbRequest), ctypes.c_int, ctypes.POINTER(RbConfig)]
+    L.rb_batch_create.argtypes = [ctypes.POINTER(RbRequest), ctypes.c_int, ctypes.POINTER(RbConfig), vp,
+                                  ctypes.POINTER(vp)]
+    L.rb_batch_update.argtypes = [vp, ctypes.POINTER(RbRequest), ctypes.c_int]
+    L.rb_batch_set_graph.argtypes = [vp, ctypes.c_int]
+    L.rb_batch_run.argtypes = [vp]
+    L.rb_batch_destroy.argtypes = [vp]
+    L.rb_batch_destroy.restype = None
     L.rb_server_create.argtypes = [ctypes.POINTER(RbServerConfig), ctypes.POINTER(vp)]
+    L.rb_server_create_method.argtypes = [ctypes.POINTER(RbServerConfig), ctypes.c_int, ctypes.c_int,
+                                          ctypes.POINTER(vp)]
+    L.rb_server_report_get.argtypes = [vp, ctypes.POINTER(RbServerReport)]
+    L.rb_server_set_timing.argtypes = [vp, ctypes.c_int]
     L.rb_server_submit.argtypes = [vp, ctypes.POINTER(RbRequest), ctypes.POINTER(ctypes.c_long)]
     L.rb_server_wait.argtypes = [vp, ctypes.c_long, dp]
     L.rb_server_stats.argtypes = [vp, dp, dp, ctypes.POINTER(ctypes.c_long), dp, ctypes.POINTER(ctypes.c_long)]
@@ -71,6 +91,52 @@ def price_batch(requests, precision=_lib.PREC_SPLIT_FP16, method=_lib.METHOD_RT,
     cfg = RbConfig(precision, compensator, method, 0)
     check(L.rb_price_batch(arr, len(requests), ctypes.byref(cfg)))
     return requests
+
+
+class Batch:
+    """A set of requests planned once and priced many times (rb_batch_*): run() fills every
+    request's price/iv/stat; update(new_requests) swaps in new parameters of the same shapes;
+    graph=True records the launches into a hipGraph and replays it."""
+
+    def __init__(self, requests, precision=_lib.PREC_SPLIT_FP16, method=_lib.METHOD_RT,
+                 compensator=_lib.COMP_REF, graph=False, stream=None):
+        L = _lib.load()
+        _declare(L)
+        self.requests = list(requests)
+        self._arr = (RbRequest * len(self.requests))(*[r.c for r in self.requests])
+        self.cfg = RbConfig(precision, compensator, method, 0)
+        self.ptr = ctypes.c_void_p()
+        check(L.rb_batch_create(self._arr, len(self.requests), ctypes.byref(self.cfg), stream,
+                                ctypes.byref(self.ptr)))
+        if graph:
+            check(L.rb_batch_set_graph(self.ptr, 1))
+
+    def run(self):
+        check(_lib.load().rb_batch_run(self.ptr))
+        return self.requests
+
+    def update(self, requests):
+        requests = list(requests)
+        arr = (RbRequest * len(requests))(*[r.c for r in requests])
+        check(_lib.load().rb_batch_update(self.ptr, arr, len(requests)))
+        self.requests, self._arr = requests, arr
+
+    def close(self):
+        if self.ptr:
+            _lib.load().rb_batch_destroy(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def autotune(shape, Ns, precision=_lib.PREC_SPLIT_FP16, method=_lib.METHOD_RT, compensator=_lib.COMP_REF):
@@ -121,13 +187,14 @@ class Server:
     """Continuous-batching server (rb_server_*)."""
 
     def __init__(self, precision=_lib.PREC_SPLIT_FP16, max_inflight=2, max_batch_requests=64,
-                 max_device_bytes=8 << 30, max_batch_path_steps=0.0):
+                 max_device_bytes=8 << 30, max_batch_path_steps=0.0, method=_lib.METHOD_RT,
+                 compensator=_lib.COMP_REF):
         L = _lib.load()
         _declare(L)
         self.ptr = ctypes.c_void_p()
         cfg = RbServerConfig(precision, max_inflight, max_batch_requests, max_device_bytes,
                              float(max_batch_path_steps))
-        check(L.rb_server_create(ctypes.byref(cfg), ctypes.byref(self.ptr)))
+        check(L.rb_server_create_method(ctypes.byref(cfg), method, compensator, ctypes.byref(self.ptr)))
         self.live = {}
 
     def submit(self, req):
@@ -152,6 +219,18 @@ class Server:
                                           ctypes.byref(ps), ctypes.byref(b)))
         return {"p50_ms": p50.value, "p99_ms": p99.value, "completed": n.value, "path_steps": ps.value,
                 "batches": b.value}
+
+    def set_timing(self, enable):
+        """Timing events around the batches' kernels from now on (a measurement mode: they keep the
+        kernels from overlapping)."""
+        check(_lib.load().rb_server_set_timing(self.ptr, 1 if enable else 0))
+
+    def report(self):
+        """rb_server_report_get as a dict: latency, work, and the device timeline of the batches'
+        pricing kernels (union, overlap, most at once)."""
+        r = RbServerReport()
+        check(_lib.load().rb_server_report_get(self.ptr, ctypes.byref(r)))
+        return {name: getattr(r, name) for name, _ in RbServerReport._fields_}
 
     def close(self):
         if self.ptr:

@@ -234,3 +234,37 @@ def test_cpp_interface_symbols_callable_via_legacy_equivalence(rb):
     r1 = rb.computeIV_RT(a["xi"], a["H"], a["eta"], a["rho"], a["T"], a["K"], 32, 5000, [1, 2], 1)
     r8 = rb.computeIV_RT(a["xi"], a["H"], a["eta"], a["rho"], a["T"], a["K"], 32, 5000, [1, 2], 8)
     assert np.array_equal(r1["price"], r8["price"])
+
+
+def test_iv_reference_mode_matches_oracle(rb, orc):
+    # rb_iv_reference (device) against the oracle's restatement of the reference's Boost-tolerance
+    # IV, at 1e-3 relative (both halve the bracket; erfc's last bits can move one halving), and
+    # within 2^-11 of the exact IV; on prices from a seeded pricing call as well
+    rng = np.random.default_rng(8)
+    n = 500
+    K = np.exp(rng.uniform(-0.5, 0.4, n))
+    T = rng.uniform(0.05, 2.0, n)
+    sig = np.exp(rng.uniform(np.log(0.02), np.log(2.5), n))
+    price = np.array([orc.bs_call(1.0, k, t, s) for k, t, s in zip(K, T, sig)])
+    otm = np.array([orc.bs_call(1.0, k, t, s) if k >= 1 else orc.bs_put(1.0, k, t, s) for k, t, s in zip(K, T, sig)])
+    keep = otm > 1e-8  # time value not lost to rounding (what either solver inverts)
+    price, K, T = price[keep], K[keep], T[keep]
+    n = len(price)
+    got = rb.iv_reference(price, K, T)
+    want = np.array([orc.iv_call_reference(p, 1.0, k, t) for p, k, t in zip(price, K, T)])
+    exact = np.array([orc.iv_call(p, 1.0, k, t) for p, k, t in zip(price, K, T)])
+    ok = np.isfinite(want)
+    assert ok.sum() > 0.9 * n and np.array_equal(np.isfinite(got), ok)
+    assert np.all(np.abs(got[ok] - want[ok]) <= 1e-3 * want[ok])
+    assert np.all(np.abs(got[ok] - exact[ok]) <= 2.0 ** -11 * exact[ok] * (1 + 1e-9))
+    # on the reference's own test inputs (testRBergLib/tests.cpp:16-44): the IVs of a pricing
+    # call's prices within the tolerance of the exact IVs it returns, and the IVs of the
+    # reference's known prices equal to the oracle's restatement
+    a = REF_ARGS
+    p, iv, st = rb.price_v2(a["xi"], a["H"], a["eta"], a["rho"], a["T"], a["K"], 256, 100000, [1, 2])
+    Ks, Ts = np.tile(a["K"], 2), np.tile(a["T"], 2)
+    ivr = rb.iv_reference(p, Ks, Ts)
+    assert np.all(np.abs(ivr - iv) <= 2.0 ** -11 * iv * (1 + 1e-9))
+    truth = rb.iv_reference(np.array(REF_PRICE), Ks, Ts)
+    want = np.array([orc.iv_call_reference(q, 1.0, k, t) for q, k, t in zip(REF_PRICE, Ks, Ts)])
+    assert np.all(np.abs(truth - want) <= 1e-3 * want)

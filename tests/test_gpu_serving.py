@@ -124,3 +124,85 @@ def test_long_path_requests_in_batches_and_server(rb):
             srv.wait(r)
     for r, q in zip(fresh, reqs):
         assert np.array_equal(r.price, q.price) and np.array_equal(r.stat, q.stat)
+
+
+def _lds_heavy_pair(method_mc=False):
+    """Two requests that each fit one workgroup's LDS alone but not together: N = 1024 with one
+    maturity (a 131 KB W1 tile) and N = 32 with 8 maturities x 2 rho (64 KB of per-path records
+    at four path blocks per tile)."""
+    from rbergomi_amd.serving import Request
+    big_tile = Request(0.04, [0.1], [1.5], [-0.7], [1.0, 1.0], [0.9, 1.1], 1024, 3000, [31, 1])
+    mats = np.repeat(np.linspace(0.1, 2.0, 8), 2)
+    many_pg = Request(0.05, [0.2, 0.2], [1.2, 1.2], [-0.8, -0.3], mats, np.tile([0.9, 1.1], 8), 32, 5000, [31, 2])
+    return [big_tile, many_pg]
+
+
+def test_lds_incompatible_requests_batch_and_serve(rb):
+    # rb_price_batch splits them into launches that fit; the server closes a batch before the
+    # request that would not fit with it.  Each result equals pricing the request alone.
+    from rbergomi_amd.serving import Request, Server, price_batch
+    reqs = _lds_heavy_pair()
+    price_batch(reqs, precision=rb.PREC_SPLIT_FP16)
+    for r in reqs:
+        p, iv, st = alone(rb, r, rb.PREC_SPLIT_FP16)
+        assert np.array_equal(p, r.price) and np.array_equal(st, r.stat)
+    fresh = [Request(r.xi, r.H, r.eta, r.rho, r.T, r.K, r.N, r.M, r.seed) for r in reqs * 3]
+    with Server(rb.PREC_SPLIT_FP16, max_inflight=2, max_batch_requests=8) as srv:
+        for r in fresh:
+            srv.submit(r)
+        for r in fresh:
+            srv.wait(r)
+        st = srv.stats()
+    assert st["batches"] >= 2
+    for r, q in zip(fresh, reqs * 3):
+        assert np.array_equal(r.price, q.price) and np.array_equal(r.stat, q.stat)
+
+
+def test_chunked_and_unchunked_partials_price_alike(rb):
+    # plain MC, N = 512, five maturities: alone, the cross-wave partials are summed in chunks of
+    # three groups (two workgroups per CU fit the LDS); in a batch whose other request brings
+    # 64 KB of per-path records, the launch is above half a CU's LDS anyway and the five groups go
+    # in one chunk.  The sums and their order are the same, so the bits are.
+    from rbergomi_amd.serving import Request, price_batch
+    T = np.repeat([0.1, 0.25, 0.5, 1.0, 2.0], 3)
+    K = np.tile([0.9, 1.0, 1.1], 5)
+    a = Request(0.04, [0.1], [1.5], [-0.7], T, K, 512, 6000, [44, 5])
+    mats = np.repeat(np.linspace(0.1, 2.0, 8), 2)
+    b = Request(0.05, [0.2, 0.2], [1.2, 1.2], [-0.8, -0.3], mats, np.tile([0.9, 1.1], 8), 32, 5000, [44, 6])
+    kw = dict(precision=rb.PREC_SPLIT_FP16, method=rb.METHOD_MC)
+    p_alone, _, s_alone = rb.price_v2(a.xi, a.H, a.eta, a.rho, a.T, a.K, a.N, a.M, a.seed, **kw)
+    price_batch([a, b], **kw)
+    assert np.array_equal(p_alone, a.price) and np.array_equal(s_alone, a.stat)
+
+
+def test_plain_mc_server_mix_equals_alone(rb):
+    from rbergomi_amd.serving import Server
+    reqs = make_requests(10, seed=12)
+    with Server(rb.PREC_SPLIT_BF16, max_inflight=3, max_batch_requests=4, method=rb.METHOD_MC,
+                compensator=rb.COMP_EXACT) as srv:
+        srv.set_timing(True)
+        for r in reqs:
+            srv.submit(r)
+        for r in reqs:
+            srv.wait(r)
+        rep = srv.report()
+    assert rep["completed"] == 10 and rep["kernel_launches"] == rep["batches"]
+    for r in reqs:
+        p, _, s = rb.price_v2(r.xi, r.H, r.eta, r.rho, r.T, r.K, r.N, r.M, r.seed, precision=rb.PREC_SPLIT_BF16,
+                              method=rb.METHOD_MC, compensator=rb.COMP_EXACT)
+        assert np.array_equal(p, r.price) and np.array_equal(s, r.stat)
+
+
+def test_server_rejects_request_beyond_one_workgroup_lds(rb):
+    # 8 maturities x 8 rho at N = 1024: 64 payoff groups of per-path records (64 KB at 32 paths
+    # per tile) and the 128 KB W1 tile cannot share one workgroup's LDS, so it is refused at
+    # submit with the reason
+    from rbergomi_amd.serving import Request, Server
+    mats = np.linspace(0.1, 2.0, 8)
+    H = [0.1] * 8
+    rho = list(np.linspace(-0.9, -0.2, 8))
+    big = Request(0.04, H, [1.5] * 8, rho, mats, np.ones(8), 1024, 1000, [1])
+    with Server() as srv:
+        with pytest.raises(rb.RbError) as e:
+            srv.submit(big)
+    assert e.value.code == 5 and "LDS" in str(e.value)

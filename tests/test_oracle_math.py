@@ -194,3 +194,34 @@ def test_normals_moments(orc):
     zz = orc.normals_philox([2024], 0, 4000, 2, 256)
     c = np.corrcoef(zz[:, 0, :].ravel(), zz[:, 1, :].ravel())[0, 1]
     assert abs(c) < 4.5 / np.sqrt(zz[:, 0, :].size)
+
+
+def test_iv_reference_tolerance_pins(orc):
+    """The reference-tolerance IV (Boost's bracket from 0.2 by the factor 2, midpoint of a bracket
+    of relative width 2^-10, rBergLib/BlackScholes.cpp:36-92): within 2^-11 relative of the
+    exact IV of the same price; a price of BS(0.2 * 2^k) sits on a bracket end and returns it
+    exactly; a vol the 20 evaluations cannot bracket gives NaN (Boost throws there)."""
+    rng = np.random.default_rng(5)
+    for _ in range(400):
+        K = float(np.exp(rng.uniform(-0.5, 0.4)))
+        T = float(rng.uniform(0.05, 2.0))
+        sig = float(np.exp(rng.uniform(np.log(0.02), np.log(2.5))))
+        price = orc.bs_call(1.0, K, T, sig)
+        otm = price if K >= 1.0 else orc.bs_put(1.0, K, T, sig)  # what either solver inverts
+        if otm < 1e-8:  # time value lost to rounding: no solver recovers the vol from such a price
+            continue
+        exact = orc.iv_call(price, 1.0, K, T)
+        ref = orc.iv_call_reference(price, 1.0, K, T)
+        assert abs(exact / sig - 1) < 1e-9
+        assert abs(ref - exact) <= 2.0 ** -11 * exact * (1 + 1e-9), (K, T, sig, ref, exact)
+    for k in (-3, -1, 0, 1, 3):
+        s = 0.2 * 2.0 ** k
+        for K in (1.0, 1.3):  # the call branch: f(0.2 * 2^k) is exactly zero there
+            assert orc.iv_call_reference(orc.bs_call(1.0, K, 1.0, s), 1.0, K, 1.0) == s
+    # sigma = 1e-7 lies 21 halvings below the guess: no bracket within 20 evaluations
+    assert np.isnan(orc.iv_call_reference(orc.bs_call(1.0, 1.0, 1.0, 1e-7), 1.0, 1.0, 1.0))
+    # but 1e-4 (11 halvings) is bracketed
+    assert abs(orc.iv_call_reference(orc.bs_call(1.0, 1.0, 1.0, 1e-4), 1.0, 1.0, 1.0) / 1e-4 - 1) < 2.0 ** -10
+    # outside the no-arbitrage range: NaN, both branches
+    assert np.isnan(orc.iv_call_reference(1.2, 1.0, 1.1, 1.0))
+    assert np.isnan(orc.iv_call_reference(0.1, 1.0, 0.8, 1.0))
