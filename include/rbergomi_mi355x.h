@@ -331,6 +331,25 @@ void rb_server_destroy(rb_server* s);
  * to 1e-14 relative instead; this one reproduces what the reference's callers see. */
 int rb_iv_reference(const double* price, const double* K, const double* T, int n, double* iv);
 
+/* ---- pricing without device allocation ----
+ * No counterpart in the reference.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): rb_price_v2 on the
+ * fused kernel (cfg->precision RB_PREC_SPLIT_FP16, RB_PREC_SPLIT_BF16 or RB_PREC_FP32_GEMM,
+ * hybrid scheme, N <= 1024) with every device buffer beyond the caller's host inputs and
+ * outputs -- tables, work items, per-item option sums, the queue counter, device results --
+ * carved from a static per-device scratch array of 64 MiB that the library holds from load until
+ * the process ends; it calls no hipMalloc, hipFree or other allocating routine.  A request whose
+ * buffers exceed the scratch (M * out_size above about 4e9: the per-item FP64 option sums take
+ * out_size * M / 64 bytes) is refused with RB_ERR_LIMIT.  Prices, implied vols and standard errors
+ * are bit-identical to rb_price_v2 with the same cfg.  The call runs on `stream` (NULL: the null
+ * stream) and returns when the results are on the host.  All calls on a device share its scratch,
+ * which keeps the last call's contents; calls are serialised within the process (calls that
+ * overlapped on one device would race on it). */
+int rb_price_noalloc(double xi, const double* H, const double* eta, const double* rho, int par_size,
+                     const double* T, const double* K, int op_size, int N, long M,
+                     const uint64_t* seed, int seed_size, const rb_config* cfg, void* stream,
+                     double* price, double* iv, double* stat, int out_size);
+
 /* Host-only planning (no device call): group index of every option I and the unit of every
  * group, as the executor deduplicates them.  group_of_option has out_size entries,
  * unit_of_group at least out_size entries. */

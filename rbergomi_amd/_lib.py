@@ -89,6 +89,7 @@ def load():
     L.rb_probe_bridge.argtypes = [dp, l, i, dp]
     L.rb_probe_integrals.argtypes = [d, dp, dp, dp, i, dp, dp, i, i, l, dp, cfgp, dp]
     L.rb_iv_reference.argtypes = [dp, dp, dp, i, dp]
+    L.rb_price_noalloc.argtypes = [d, dp, dp, dp, i, dp, dp, i, i, l, u64p, i, cfgp, vp, dp, dp, dp, i]
     _lib = L
     return _lib
 

@@ -13,7 +13,7 @@ LIB = os.path.join(HERE, "librbergomi_mi355x.so")
 
 SOURCES = ["rb_staged.hip", "rb_fused.hip", "rb_fused_rt6.hip", "rb_fused_rt2.hip", "rb_fused_rt1.hip", "rb_fused_rt5.hip", "rb_fused_rt4.hip", "rb_fused_rt3.hip", "rb_fused_mc.hip", "rb_fused_mc8.hip", "rb_fused_mc2.hip", "rb_fused_mc1.hip", "rb_fused_mc5.hip", "rb_fused_mc4.hip", "rb_fused_mc3.hip", "rb_fft.hip",
            "rb_turbo.hip", "rb_chol.hip", "rb_probe.hip", "rb_volterra64.hip", "rb_fused_fft.hip", "rb_engine.cpp", "rb_chol.cpp", "rb_sobol.cpp",
-           "rb_fused_host.cpp", "rb_capi.cpp", "rb_server.cpp"]
+           "rb_fused_host.cpp", "rb_capi.cpp", "rb_server.cpp", "rb_noalloc.hip"]
 # Packed FP32 instructions are disabled for all device code (see the package notes): the
 # compiler would otherwise form v_pk_* from ordinary float code.
 DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]

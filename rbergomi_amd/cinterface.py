@@ -104,6 +104,24 @@ def iv_reference(price, K, T):
     return iv
 
 
+def price_noalloc(xi, H, eta, rho, T, K, N, M, seed, precision=_lib.PREC_SPLIT_FP16,
+                  compensator=_lib.COMP_REF, method=_lib.METHOD_RT, stream=None):
+    """rb_price_noalloc: rb_price_v2 on the fused kernel with no device allocation (a static
+    per-device scratch array) -> (price, iv, stat); raises RbError on failure."""
+    H, pH = _arr(H)
+    eta, pe = _arr(eta)
+    rho, pr = _arr(rho)
+    T, pT = _arr(T)
+    K, pK = _arr(K)
+    s, ps = _seed(seed)
+    out = len(H) * len(T)
+    price, iv, stat = np.zeros(out), np.zeros(out), np.zeros(out)
+    cfg = _cfg(precision, compensator, method, 0)
+    check(_lib.load().rb_price_noalloc(float(xi), pH, pe, pr, len(H), pT, pK, len(T), int(N), int(M), ps,
+                                       len(s), ctypes.byref(cfg), stream, price.ctypes.data_as(dp),
+                                       iv.ctypes.data_as(dp), stat.ctypes.data_as(dp), out))
+    return price, iv, stat
+
 
 def payoff_samples_v2(xi, H, eta, rho, T, K, N, Z, method=_lib.METHOD_RT,
                       compensator=_lib.COMP_REF, scheme=0):
