@@ -385,14 +385,14 @@ def serving_line(rb, args):
 
 def serving_roofline(rb, args):
     """The roofline of the serving trace's dominant kernel: one closed-loop pass of the trace
-    through a server with HIP timing events around every batch's kernel, on its stream.  The events
-    keep the batches' kernels from overlapping (25% of throughput on MI355X), so this pass runs
-    after everything else the line measures and its batches are timed one kernel at a time."""
+    through a server with one batch in flight and HIP timing events around every batch's kernel, on
+    its stream, so that each kernel's duration is its own (in the timed steps up to four batches'
+    kernels share the device, and a kernel's start-to-end time includes its neighbours' work)."""
     from rbergomi_amd.serving import Request, Server
     ds = serving_requests(args.requests)
     work = float(sum(d["M"] * d["N"] for d in ds))
     reqs = [Request(d["xi"], d["H"], d["eta"], d["rho"], d["T"], d["K"], d["N"], d["M"], d["seed"]) for d in ds]
-    with Server(rb.PREC_SPLIT_FP16, max_inflight=SERVE_INFLIGHT, max_batch_requests=SERVE_MAX_BATCH,
+    with Server(rb.PREC_SPLIT_FP16, max_inflight=1, max_batch_requests=SERVE_MAX_BATCH,
                 max_batch_path_steps=SERVE_CAP) as srv:
         srv.set_timing(True)
         for r in reqs:
@@ -409,8 +409,7 @@ def serving_roofline(rb, args):
             "kernel": "k_fused (the fused pricing kernel of every batch)", "avg_launch_ms": avg_ms,
             "launches_per_step": d["kernel_launches"], "flops_per_launch": flops_launch,
             "timing": "HIP events on each batch's stream around its kernel, in one closed-loop pass of the "
-                      "trace after the timed steps (the events serialize the batches' kernels, so the timed "
-                      "steps run without them)",
+                      "trace with one batch in flight, after the timed steps (which run without events)",
             "timing_pass": {"kernel_ms": d["kernel_ms"], "device_busy_ms": d["device_busy_ms"],
                             "batches": d["batches"],
                             "achieved_over_busy_TFLOPs": d["conv_flops"] / (d["device_busy_ms"] * 1e-3) / 1e12}}
@@ -436,6 +435,49 @@ def bench_serving(rb, args):
            "serving": res, "p50_ms": res["p50_ms"], "p99_ms": res["p99_ms"]}
     if tuned:
         out["autotune"] = tuned
+    return out
+
+
+def bench_config1(rb, args):
+    """H = 0.07, eta = 1.9, rho = -0.9, xi = 0.235^2, T = 1, N = 64, M = 2^16, 21 strikes in
+    [0.8, 1.2]: each precision through a plan (device-resident buffers), and the CPU restatement
+    of the reference's driver on the whole workload (host threads and one thread)."""
+    xi, H, eta, rho = 0.235 ** 2, [0.07], [1.9], [-0.9]
+    T, K = [1.0] * 21, list(np.linspace(0.8, 1.2, 21))
+    N, M = 64, 2 ** 16
+    rows = {}
+    p64 = None
+    for name, code in (("fp64", rb.PREC_FP64), ("fp16", rb.PREC_SPLIT_FP16), ("bf16", rb.PREC_SPLIT_BF16),
+                       ("fp32gemm", rb.PREC_FP32_GEMM)):
+        pl = rb.Plan(xi, H, eta, rho, T, K, N, M, [20240601, 1], precision=code)
+        for _ in range(args.warmup):
+            pl.run()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            pl.run()
+        ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        p, _, st = pl.results()
+        pl.close()
+        rows[name] = {"ms": ms, "paths_steps_per_s": M * N / (ms * 1e-3)}
+        if p64 is None:
+            p64 = p
+            rows[name]["price_atm"] = float(p[10])
+            rows[name]["stat_atm"] = float(st[10])
+        else:
+            rows[name]["max_abs_err_vs_fp64"] = float(np.max(np.abs(p - p64)))
+    out = {"metric": "paths*steps/s", "unit": "paths*steps/s", "n_gpus": 1, "higher_is_better": True,
+           "value": rows["fp16"]["paths_steps_per_s"], "dtype": "f16x2+f32", "vs_baseline": None,
+           "scaling": "weak", "data": "synthetic (device Philox4x32-10 Gaussians)",
+           "config": {"workload": "config 1: N=64, M=2^16, 21 strikes, one parameter set"}, "rows": rows}
+    if not args.no_cpu:
+        from oracle import pyoracle as orc
+        cpu = {}
+        for th in (args.cpu_threads or cpu_threads(), 1):
+            t0 = time.perf_counter()
+            orc.compute_rt(xi, H, eta, rho, T, K, N, M, [2024, 1], num_threads=th, conv=orc.CONV_FFT)
+            dt = time.perf_counter() - t0
+            cpu["%d_threads" % th] = {"s": dt, "paths_steps_per_s": M * N / dt}
+        out["cpu_baseline"] = {"whole workload": cpu, "kind": "port", "host": host_cpu()}
     return out
 
 
@@ -470,7 +512,19 @@ def bench_long(rb, args):
                     row["_p64"] = p
             row.pop("_p64", None)
             rows.append(row)
+    cpu = None
+    if not args.no_cpu:
+        # the CPU restatement of the reference's driver on 2^12 paths of each (N, H = 0.05)
+        from oracle import pyoracle as orc
+        cpu = {"kind": "port", "host": host_cpu(), "threads": args.cpu_threads or cpu_threads(), "rows": {}}
+        for N in (1024, 2048, 4096):
+            t0 = time.perf_counter()
+            orc.compute_rt(0.03, [0.05], [2.3], [-0.9], [1.0] * 31, list(K), N, 2 ** 12, [4, N],
+                           num_threads=cpu["threads"], conv=orc.CONV_FFT)
+            dt = time.perf_counter() - t0
+            cpu["rows"][str(N)] = {"paths": 2 ** 12, "s": dt, "paths_steps_per_s": 2 ** 12 * N / dt}
     return {"metric": "paths*steps/s", "unit": "paths*steps/s", "n_gpus": 1, "higher_is_better": True,
+            "cpu_baseline": cpu,
             "value": max(r["fp32fft"]["paths_steps_per_s"] for r in rows), "dtype": "f32 FFT convolution, f64 rest",
             "config": {"workload": "long paths: N in {1024,2048,4096}, M = 2^18, 31 strikes"},
             "rows": rows, "vs_baseline": None, "scaling": "weak", "data": "synthetic"}
@@ -582,7 +636,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--precision", default="fp16", choices=sorted(PREC_CODE))
-    ap.add_argument("--workload", default="trace", choices=["trace", "single", "batched", "serving", "long"],
+    ap.add_argument("--workload", default="trace", choices=["trace", "single", "config1", "batched", "serving", "long"],
                     help="trace: the serving trace, closed loop (the default line); single: the single-set "
                          "workload; batched: 256 sets in one launch; serving: Poisson latency sweep; long: "
                          "long paths")
@@ -624,7 +678,8 @@ def main():
         print(json.dumps(out), flush=True)
         return
     if args.workload != "single":
-        fn = {"batched": bench_batched, "serving": bench_serving, "long": bench_long}[args.workload]
+        fn = {"batched": bench_batched, "serving": bench_serving, "long": bench_long,
+              "config1": bench_config1}[args.workload]
         print(json.dumps(fn(rb, args)), flush=True)
         return
     print(json.dumps(single_line(rb, args)), flush=True)

@@ -298,10 +298,8 @@ int rb_server_wait(rb_server* s, long ticket, double* latency_ms);
 int rb_server_stats(rb_server* s, double* p50_ms, double* p99_ms, long* completed,
                     double* path_steps, long* batches);
 /* Timing of the batches' pricing kernels (enable = 1; off by default): each batch launched while
- * it is on brackets its kernel with HIP timing events on its stream, giving an interval [begin,
- * end] on the device clock for rb_server_report_get.  The events keep the batches' kernels from
- * overlapping on the device (measured on MI355X: 25% less serving throughput), so this is a
- * measurement mode, not for production. */
+ * it is on brackets its kernel with HIP timing events on its own stream, giving an interval
+ * [begin, end] on the device clock for rb_server_report_get (kernel time, union, overlap). */
 int rb_server_set_timing(rb_server* s, int enable);
 /* What the server has done so far.  The kernel fields cover the batches launched with timing on
  * (zero otherwise): their sum, union, time covered by two or more and the most at one instant. */

@@ -221,8 +221,7 @@ class Server:
                 "batches": b.value}
 
     def set_timing(self, enable):
-        """Timing events around the batches' kernels from now on (a measurement mode: they keep the
-        kernels from overlapping)."""
+        """Timing events around the batches' kernels from now on (kernel intervals in report())."""
         check(_lib.load().rb_server_set_timing(self.ptr, 1 if enable else 0))
 
     def report(self):
