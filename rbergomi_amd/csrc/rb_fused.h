@@ -119,26 +119,26 @@ __device__ __forceinline__ float mills_f32(float z) {
 // same for K Phi(d2), so one exponential serves both normal CDFs.
 // Multiply-adds are written out (no implicit contraction), so every kernel instantiation that
 // inlines this computes the same bits.
-__device__ __forceinline__ double payoff_mixed(const PathPG& q, float lnKf, float Kf, double lnK,
-                                               double K) {
+__device__ __forceinline__ double payoff_mixed(const PathPG& q, float lnKf, float Kf, double lnK) {
 #pragma clang fp contract(off)
-    const float d1 = fmaf(q.lnSf - lnKf, q.invs, 0.5f * q.sigf);
-    if (d1 < -8.0f) {
-        // deep out of the money (Phi(d1) < 7e-16): the normal CDF in FP64
-        const double e1 = (q.lnS - lnK) / q.sig + 0.5 * q.sig;
-        return exp(q.lnS) * phi_f64(e1) - K * phi_f64(e1 - q.sig);
-    }
-    const float d2 = d1 - q.sigf;
+    const float S = q.S, sigf = q.sigf;
+    const float d1 = fmaf(q.lnSf - lnKf, q.invs, 0.5f * sigf);
+    const float d2 = d1 - sigf;
     const float E = __expf(fmaf(-0.5f * d1, d1, q.lnSf)) * 0.398942280f;
     const float a1 = mills_f32(fabsf(d1)), a2 = mills_f32(fabsf(d2));
-    float x;
-    if (d2 >= 0.f)
-        x = fmaf(E, a2 - a1, q.S - Kf);
-    else if (d1 >= 0.f)
-        x = fmaf(-E, a1 + a2, q.S);
-    else
-        x = E * (a1 - a2);
-    return double(x);
+    // the three sign cases as selects (no divergent branches in the strike loop)
+    const float x_itm = fmaf(E, a2 - a1, S - Kf);   // d2 >= 0
+    const float x_mid = fmaf(-E, a1 + a2, S);       // d1 >= 0 > d2
+    const float x_otm = E * (a1 - a2);              // d1 < 0
+    double x = double(d2 >= 0.f ? x_itm : (d1 >= 0.f ? x_mid : x_otm));
+    if (d1 < -8.0f) {
+        // deep out of the money (Phi(d1) < 7e-16): the normal CDFs in FP64, as
+        // Phi(e) = exp(-e^2/2) erfcx(-e/sqrt 2) / 2 with S' exp(-e1^2/2) = K exp(-e2^2/2)
+        const double e1 = (q.lnS - lnK) / q.sig + 0.5 * q.sig, e2 = e1 - q.sig;
+        x = 0.5 * exp(q.lnS - 0.5 * e1 * e1) *
+            (erfcx(-0.7071067811865476 * e1) - erfcx(-0.7071067811865476 * e2));
+    }
+    return x;
 }
 
 // plain Monte Carlo payoff (S_T - K)^+ in FP64 (S_T in the sig slot)

@@ -269,7 +269,7 @@ __global__ void __launch_bounds__(256, (QPT >= 8 || (MC && QPT >= 4)) ? 1 : 2) k
                     for (int c = 0; c < 2; ++c) {
                         if (pth[c] >= A.M) continue;
                         const PathPG& rec = pgd[c * A.pgu + (O.pg - U.pg_begin)];
-                        const double x = MC ? payoff_mc(rec, O.K) : payoff_mixed(rec, lnKf, Kf, O.lnK, O.K);
+                        const double x = MC ? payoff_mc(rec, O.K) : payoff_mixed(rec, lnKf, Kf, O.lnK);
                         s1 += x;
                         s2 += x * x;
                     }

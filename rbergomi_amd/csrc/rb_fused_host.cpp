@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <array>
+#include <cfloat>
 #include <cstdio>
 #include <cmath>
 #include <cstdlib>
@@ -261,6 +262,11 @@ int plan_fused_request(const Request& req, const Layout& lay, int mode, FusedHos
     // the GEMM kernel reads P_{j} at j - 1: a lane's four rows 8c + 4h + 0..3 are one aligned float4
     fused_tables(req, lay, Np, a.NK * 128, 1, out, pgmax, gmax);
     const int nu = int(lay.units.size());
+    // rows j >= N of the last row block: P = +inf at index j - 1 and al >= FLT_MIN, so the
+    // integrand exp2(c1 x - al P - be) is exactly 0 there (the kernel does not test rows)
+    for (int u = 0; u < nu; ++u)
+        for (int i = std::max(N - 1, 0); i < Np; ++i) out.ptab[size_t(u) * Np + i] = INFINITY;
+    for (FusedGroup& F : out.fg) F.al = std::max(F.al, FLT_MIN);
     // ---- A fragments: entry (d, lane, slot j) = G[d + m - 8h - j], G[0] = a, G[t>0] = Gamma_t
     out.frag.assign(size_t(nu) * a.NK * 128, uint4());
     for (int u = 0; u < nu; ++u) {
@@ -301,7 +307,11 @@ int plan_fused_request(const Request& req, const Layout& lay, int mode, FusedHos
     a.mc = req.method == METHOD_MC ? 1 : 0;
     a.M = req.M;
     a.ntiles = (req.M + a.TP - 1) / a.TP;
-    a.tiles_per_item = std::max(1, 1024 / a.TP);
+    // work items of at most 1024 paths, and at least 256 per request where its tiles allow, so a
+    // small request alone still spreads over the chip (N = 64, M = 2^16: 64 items of 1024 paths
+    // took 0.170 ms, 256 items of 256 paths 0.089 ms).  The item size depends on the request
+    // only, so its sums are added in the same order alone and in any batch.
+    a.tiles_per_item = int(std::max<long>(1, std::min<long>(1024 / a.TP, (a.ntiles + 255) / 256)));
     a.nitems = (a.ntiles + a.tiles_per_item - 1) / a.tiles_per_item;
     a.key = philox_key(req.seed.data(), int(req.seed.size()));
     a.gchunk = std::min(gmax, FUSED_GREG);

@@ -334,21 +334,19 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
                                 wv[4 * c + 2] = qa[3];
                                 wv[4 * c + 3] = qb[0];
                             }
+                            // rows j = 32 rb + row + 1 >= N read P = +inf (al > 0), so rr = 0
+                            // and they add nothing: no per-row test.  Groups outside, rows
+                            // inside: one uniform branch per group instead of one per row and
+                            // group; each group's sum still runs over r = 0..15 in order.
 #pragma unroll
-                            for (int r = 0; r < 16; ++r) {
-                                const int j = 32 * rb + mfma32_row(r, lane) + 1;  // v_j uses Wtilde_{j-1}
-                                if (j < A.N) {
-                                    const float w1 = wv[r];
-                                    const float pj = pv[r];
-                                    const float x = acc[s][r];
+                            for (int gg = 0; gg < GR; ++gg) {
+                                if (gg < gn) {
 #pragma unroll
-                                    for (int gg = 0; gg < GR; ++gg) {
-                                        if (gg < gn) {
-                                            const float rr = __builtin_amdgcn_exp2f(fmaf(x, c1[gg], -fmaf(al[gg], pj, be[gg])));
-                                            fv[gg] = fmaf(rr, rr, fv[gg]);
-                                            fs[gg] = fmaf(rr, w1, fs[gg]);
-                                            if constexpr (MC) fp[gg] = fmaf(rr, wq[r], fp[gg]);
-                                        }
+                                    for (int r = 0; r < 16; ++r) {
+                                        const float rr = __builtin_amdgcn_exp2f(fmaf(acc[s][r], c1[gg], -fmaf(al[gg], pv[r], be[gg])));
+                                        fv[gg] = fmaf(rr, rr, fv[gg]);
+                                        fs[gg] = fmaf(rr, wv[r], fs[gg]);
+                                        if constexpr (MC) fp[gg] = fmaf(rr, wq[r], fp[gg]);
                                     }
                                 }
                             }
@@ -459,7 +457,7 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
                             double s1 = 0.0, s2 = 0.0;
                             for (int pl = 0; pl < np; ++pl) {
                                 const double x = MC ? payoff_mc(pgd[pl * A.pgmax + pgl], O.K)
-                                                    : payoff_mixed(pgd[pl * A.pgmax + pgl], lnKf, Kf, O.lnK, O.K);
+                                                    : payoff_mixed(pgd[pl * A.pgmax + pgl], lnKf, Kf, O.lnK);
                                 s1 += x;
                                 s2 += x * x;
                             }
@@ -476,7 +474,7 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
                             double s1 = 0.0, s2 = 0.0;
                             for (int pl = sl; pl < np; pl += S) {
                                 const double x = MC ? payoff_mc(pgd[pl * A.pgmax + pgl], O.K)
-                                                    : payoff_mixed(pgd[pl * A.pgmax + pgl], lnKf, Kf, O.lnK, O.K);
+                                                    : payoff_mixed(pgd[pl * A.pgmax + pgl], lnKf, Kf, O.lnK);
                                 s1 += x;
                                 s2 += x * x;
                             }

@@ -67,7 +67,7 @@ __global__ void k_probe_payoff(const double* __restrict__ lnS, const double* __r
     q.invs = float(1.0 / q.sig);
     q.lnSf = float(q.lnS);
     q.sigf = float(q.sig);
-    out[i] = payoff_mixed(q, float(log(K[i])), float(K[i]), log(K[i]), K[i]);
+    out[i] = payoff_mixed(q, float(log(K[i])), float(K[i]), log(K[i]));
     mills_out[i] = mills_f32(z[i]);
 }
 
