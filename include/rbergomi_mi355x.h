@@ -383,10 +383,11 @@ int rb_probe_mfma_layout(const float* A, const float* B, int mode, float* out);
 /* Device-to-device copy of `bytes` (multiple of 16) by a plain streaming kernel, reps times;
  * *avg_ms receives the mean kernel time (HIP events).  Used to calibrate HBM traffic counters. */
 int rb_probe_copy(long bytes, int reps, double* avg_ms);
-/* The split-precision kernel's per-path payoff for given (ln S', sigma, K) -> out[i], and its
- * Mills-ratio approximation at z[i] -> mills[i] (n values). */
+/* The split-precision kernels' per-path payoff for given (ln S', sigma, K) -> out[i], and its
+ * Mills-ratio approximation at z[i] -> mills[i] (n values).  form 0: the fused GEMM kernel's
+ * (deep out-of-the-money tail through erfcx), 1: the long-path kernel's (through erfc). */
 int rb_probe_payoff(const double* lnS, const double* sig, const double* K, int n, double* out,
-                    const float* z, float* mills);
+                    const float* z, float* mills, int form);
 /* Per-path integrals (Ivdt, IsvdW) of option I from Z: out[2*(m*out_size + I) + {0,1}] */
 int rb_probe_integrals(double xi, const double* H, const double* eta, const double* rho,
                        int par_size, const double* T, const double* K, int op_size, int N,

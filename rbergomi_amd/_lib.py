@@ -79,7 +79,7 @@ def load():
     L.rb_probe_wtilde.argtypes = [dp, l, i, d, dp]
     fp = ctypes.POINTER(ctypes.c_float)
     L.rb_probe_mfma_layout.argtypes = [fp, fp, i, fp]
-    L.rb_probe_payoff.argtypes = [dp, dp, dp, i, dp, fp, fp]
+    L.rb_probe_payoff.argtypes = [dp, dp, dp, i, dp, fp, fp, i]
     L.rb_probe_copy.argtypes = [l, i, dp]
     L.rb_probe_chol.argtypes = [d, i, dp, dp, dp]
     L.rb_probe_sobol.argtypes = [i, l, l, dp]

@@ -566,7 +566,7 @@ done:
 }
 
 int rb_probe_payoff(const double* lnS, const double* sig, const double* K, int n, double* out,
-                    const float* z, float* mills) {
+                    const float* z, float* mills, int form) {
     int rc = RB_OK;
     double *d0 = nullptr, *d1 = nullptr, *d2 = nullptr, *d3 = nullptr;
     float *f0 = nullptr, *f1 = nullptr;
@@ -581,7 +581,7 @@ int rb_probe_payoff(const double* lnS, const double* sig, const double* K, int n
     PROBE_TRY(hipMemcpy(d1, sig, b, hipMemcpyHostToDevice));
     PROBE_TRY(hipMemcpy(d2, K, b, hipMemcpyHostToDevice));
     PROBE_TRY(hipMemcpy(f0, z, fb, hipMemcpyHostToDevice));
-    PROBE_TRY(launch_probe_payoff(d0, d1, d2, n, d3, f1, f0, nullptr));
+    PROBE_TRY(launch_probe_payoff(d0, d1, d2, n, d3, f1, f0, form, nullptr));
     PROBE_TRY(hipMemcpy(out, d3, b, hipMemcpyDeviceToHost));
     PROBE_TRY(hipMemcpy(mills, f1, fb, hipMemcpyDeviceToHost));
 done:

@@ -116,27 +116,34 @@ __device__ __forceinline__ float mills_f32(float z) {
 
 // conditioned Black-Scholes payoff of one path for one strike (tau = 1, S0 = S').
 // With E = S' phi(d1) = K phi(d2):  S' Phi(d1) = d1 >= 0 ? S' - E M(d1) : E M(-d1), and the
-// same for K Phi(d2), so one exponential serves both normal CDFs.
+// same for K Phi(d2), so one exponential serves both normal CDFs; the three sign cases are
+// selects (no divergent branch in the strike loop).  Deep out of the money (d1 < -8,
+// Phi(d1) < 7e-16) the normal CDFs are evaluated in FP64, in one of two forms of the same value:
+// Phi(e) = exp(-e^2/2) erfcx(-e/sqrt 2) / 2 with S' exp(-e1^2/2) = K exp(-e2^2/2), one
+// exponential (the default: fewer instructions), or S' Phi(e1) - K Phi(e2) through erfc
+// (LOWREG: fewer live registers, for the long-path kernel, which holds its Gaussians in
+// registers; the erfcx form raised its register spills from 51 to 121 at N = 4096).
 // Multiply-adds are written out (no implicit contraction), so every kernel instantiation that
 // inlines this computes the same bits.
-__device__ __forceinline__ double payoff_mixed(const PathPG& q, float lnKf, float Kf, double lnK) {
+template <bool LOWREG = false>
+__device__ __forceinline__ double payoff_mixed(const PathPG& q, float lnKf, float Kf, double lnK, double K) {
 #pragma clang fp contract(off)
     const float S = q.S, sigf = q.sigf;
     const float d1 = fmaf(q.lnSf - lnKf, q.invs, 0.5f * sigf);
     const float d2 = d1 - sigf;
     const float E = __expf(fmaf(-0.5f * d1, d1, q.lnSf)) * 0.398942280f;
     const float a1 = mills_f32(fabsf(d1)), a2 = mills_f32(fabsf(d2));
-    // the three sign cases as selects (no divergent branches in the strike loop)
     const float x_itm = fmaf(E, a2 - a1, S - Kf);   // d2 >= 0
     const float x_mid = fmaf(-E, a1 + a2, S);       // d1 >= 0 > d2
     const float x_otm = E * (a1 - a2);              // d1 < 0
     double x = double(d2 >= 0.f ? x_itm : (d1 >= 0.f ? x_mid : x_otm));
     if (d1 < -8.0f) {
-        // deep out of the money (Phi(d1) < 7e-16): the normal CDFs in FP64, as
-        // Phi(e) = exp(-e^2/2) erfcx(-e/sqrt 2) / 2 with S' exp(-e1^2/2) = K exp(-e2^2/2)
         const double e1 = (q.lnS - lnK) / q.sig + 0.5 * q.sig, e2 = e1 - q.sig;
-        x = 0.5 * exp(q.lnS - 0.5 * e1 * e1) *
-            (erfcx(-0.7071067811865476 * e1) - erfcx(-0.7071067811865476 * e2));
+        if constexpr (LOWREG)
+            x = exp(q.lnS) * phi_f64(e1) - K * phi_f64(e2);
+        else
+            x = 0.5 * exp(q.lnS - 0.5 * e1 * e1) *
+                (erfcx(-0.7071067811865476 * e1) - erfcx(-0.7071067811865476 * e2));
     }
     return x;
 }
@@ -200,6 +207,12 @@ struct FftFusedArgs {
     int* counter;                // next work item (zeroed before the launch)
 };
 size_t fused_fft_lds_bytes(const FftFusedArgs& a);
+// quads per thread of the kernel instantiation that runs N: ceil(ceil(N/4) / 256) rounded up to
+// 1, 2, 4 or 8; its threads visit steps 0 .. 1024 QPT - 1
+inline int fused_fft_qpt(int N) {
+    const int q = ((N + 3) / 4 + 255) / 256;
+    return q <= 1 ? 1 : q <= 2 ? 2 : q <= 4 ? 4 : 8;
+}
 hipError_t launch_fused_fft(const FftFusedArgs& a, int grid, hipStream_t st);
 
 }  // namespace rb

@@ -167,6 +167,7 @@ __global__ void __launch_bounds__(256, (QPT >= 8 || (MC && QPT >= 4)) ? 1 : 2) k
                             be[gg] = A.groups[g0 + gg].be;
                         }
                     }
+                    if constexpr (MC) {
 #pragma unroll
                     for (int j = 0; j < QPT; ++j) {
                         const int q = tid + 256 * j;
@@ -200,6 +201,33 @@ __global__ void __launch_bounds__(256, (QPT >= 8 || (MC && QPT >= 4)) ? 1 : 2) k
                                 }
                             }
                         }
+                    }
+                    } else {
+                    // groups outside (one uniform branch per group), steps inside with no test:
+                    // P = +inf at i = 0 and at i >= N (al > 0) makes those terms exactly 0, and
+                    // i - 1 is clamped to a written slot.  Each group's sums still run over the
+                    // steps in the same order.
+#pragma unroll
+                    for (int gg = 0; gg < FFT_GR; ++gg) {
+                        if (gg < gn) {
+#pragma unroll
+                            for (int j = 0; j < QPT; ++j) {
+                                const int q = tid + 256 * j;
+#pragma unroll
+                                for (int t = 0; t < 4; ++t) {
+                                    const int i = 4 * q + t;  // v_i uses Wtilde_{i-1}; i = 0 is v_0 = xi
+                                    const float2 x = z[fpad(i > 0 ? i - 1 : 0)];
+                                    const float e = fmaf(al[gg], P[i], be[gg]);
+                                    const float ra = __builtin_amdgcn_exp2f(fmaf(x.x, c1[gg], -e));
+                                    const float rb = __builtin_amdgcn_exp2f(fmaf(x.y, c1[gg], -e));
+                                    fv[gg][0] = fmaf(ra, ra, fv[gg][0]);
+                                    fs[gg][0] = fmaf(ra, w1[0][j][t], fs[gg][0]);
+                                    fv[gg][1] = fmaf(rb, rb, fv[gg][1]);
+                                    fs[gg][1] = fmaf(rb, w1[1][j][t], fs[gg][1]);
+                                }
+                            }
+                        }
+                    }
                     }
                     // FP64 over the workgroup: wave sums, then the four waves in order
 #pragma unroll
@@ -269,7 +297,7 @@ __global__ void __launch_bounds__(256, (QPT >= 8 || (MC && QPT >= 4)) ? 1 : 2) k
                     for (int c = 0; c < 2; ++c) {
                         if (pth[c] >= A.M) continue;
                         const PathPG& rec = pgd[c * A.pgu + (O.pg - U.pg_begin)];
-                        const double x = MC ? payoff_mc(rec, O.K) : payoff_mixed(rec, lnKf, Kf, O.lnK);
+                        const double x = MC ? payoff_mc(rec, O.K) : payoff_mixed<true>(rec, lnKf, Kf, O.lnK, O.K);
                         s1 += x;
                         s2 += x * x;
                     }
@@ -290,9 +318,9 @@ __global__ void __launch_bounds__(256, (QPT >= 8 || (MC && QPT >= 4)) ? 1 : 2) k
 
 hipError_t launch_fused_fft(const FftFusedArgs& a, int grid, hipStream_t st) {
     const size_t lds = fused_fft_lds_bytes(a);
-    const int qpt = ((a.N + 3) / 4 + 255) / 256;
+    const int qpt = fused_fft_qpt(a.N);
     // the kernels are compiled for the transform length of their quad count
-    const int want = qpt == 1 ? (a.N <= 512 ? 10 : 11) : qpt <= 2 ? 12 : qpt <= 4 ? 13 : 14;
+    const int want = qpt == 1 ? (a.N <= 512 ? 10 : 11) : qpt == 2 ? 12 : qpt == 4 ? 13 : 14;
     if (a.N < 1 || a.log2L != want) return hipErrorInvalidValue;
     auto go = [&](auto kern) -> hipError_t {
         if (lds > 65536) {

@@ -172,7 +172,8 @@ void fused_tables(const Request& req, const Layout& lay, int Np, int frag_stride
             if (G.unit != u) continue;
             FusedGroup F;
             F.c1 = float(0.5 * G.eta * G.TH * log2e);
-            F.al = float(0.25 * G.eta * G.eta * std::pow(G.dt, 2.0 * H) * log2e);
+            // at least FLT_MIN: the kernels zero a step's term by P = +inf (al P = +inf)
+            F.al = std::max(float(0.25 * G.eta * G.eta * std::pow(G.dt, 2.0 * H) * log2e), FLT_MIN);
             F.be = float(-0.5 * std::log(G.xi) * log2e);
             F.xi = G.xi;
             F.sxi = std::sqrt(G.xi);
@@ -266,7 +267,6 @@ int plan_fused_request(const Request& req, const Layout& lay, int mode, FusedHos
     // integrand exp2(c1 x - al P - be) is exactly 0 there (the kernel does not test rows)
     for (int u = 0; u < nu; ++u)
         for (int i = std::max(N - 1, 0); i < Np; ++i) out.ptab[size_t(u) * Np + i] = INFINITY;
-    for (FusedGroup& F : out.fg) F.al = std::max(F.al, FLT_MIN);
     // ---- A fragments: entry (d, lane, slot j) = G[d + m - 8h - j], G[0] = a, G[t>0] = Gamma_t
     out.frag.assign(size_t(nu) * a.NK * 128, uint4());
     for (int u = 0; u < nu; ++u) {
@@ -717,6 +717,18 @@ void FftFused::release() {
     d_ivopts = nullptr;
 }
 
+// P table of the long-path kernel: P_i at index i for the 1024 QPT steps its threads visit
+// (QPT quads of four steps per thread, QPT of the instantiation that runs N), +inf at i = 0 and
+// at i >= N so those terms vanish
+static void fft_ptab(const Request& req, const Layout& lay, FusedHostReq& tabs, int& pgmax, int& gmax) {
+    const int N = req.N, len = 1024 * fused_fft_qpt(N);
+    fused_tables(req, lay, len, 0, 0, tabs, pgmax, gmax);
+    for (size_t u = 0; u < lay.units.size(); ++u) {
+        tabs.ptab[u * len] = INFINITY;
+        for (int i = N; i < len; ++i) tabs.ptab[u * len + i] = INFINITY;
+    }
+}
+
 int FftFused::plan(const Request& req, const Layout& lay, size_t lds_limit, size_t& footprint, std::string& err) {
     const int N = req.N;
     if (N <= 1024 || ((N + 3) / 4 + 255) / 256 > 8) {
@@ -726,10 +738,9 @@ int FftFused::plan(const Request& req, const Layout& lay, size_t lds_limit, size
     int log2L = 10;  // L >= 2N - 1, at least 2^10 (the kernel's shortest transform)
     while ((1 << log2L) < 2 * N - 1) ++log2L;
     const size_t L = size_t(1) << log2L;
-    const int Np = (N + 31) / 32 * 32;
     FusedHostReq tabs;
     int pgmax = 1, gmax = 1;
-    fused_tables(req, lay, Np, 0, 0, tabs, pgmax, gmax);
+    fft_ptab(req, lay, tabs, pgmax, gmax);
     FftFusedArgs a{};
     a.log2L = log2L;
     a.out_size = int(lay.options.size());
@@ -759,10 +770,9 @@ int FftFused::build(const Request& req, const Layout& lay, int device, hipStream
         err = "the long-path kernel covers N <= 8192";
         return ST_ERR_LIMIT;
     }
-    const int Np = (N + 31) / 32 * 32;
     FusedHostReq tabs;
     int pgmax = 1, gmax = 1;
-    fused_tables(req, lay, Np, 0, 0, tabs, pgmax, gmax);
+    fft_ptab(req, lay, tabs, pgmax, gmax);
     int pgu = 1;
     for (const FusedUnit& U : tabs.fu) pgu = std::max(pgu, U.pg_end - U.pg_begin);
     // twiddles and FFT(a, Gamma_1, ..., Gamma_{N-1}) per unit, in FP64, stored in FP32

@@ -457,7 +457,7 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
                             double s1 = 0.0, s2 = 0.0;
                             for (int pl = 0; pl < np; ++pl) {
                                 const double x = MC ? payoff_mc(pgd[pl * A.pgmax + pgl], O.K)
-                                                    : payoff_mixed(pgd[pl * A.pgmax + pgl], lnKf, Kf, O.lnK);
+                                                    : payoff_mixed(pgd[pl * A.pgmax + pgl], lnKf, Kf, O.lnK, O.K);
                                 s1 += x;
                                 s2 += x * x;
                             }
@@ -474,7 +474,7 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
                             double s1 = 0.0, s2 = 0.0;
                             for (int pl = sl; pl < np; pl += S) {
                                 const double x = MC ? payoff_mc(pgd[pl * A.pgmax + pgl], O.K)
-                                                    : payoff_mixed(pgd[pl * A.pgmax + pgl], lnKf, Kf, O.lnK);
+                                                    : payoff_mixed(pgd[pl * A.pgmax + pgl], lnKf, Kf, O.lnK, O.K);
                                 s1 += x;
                                 s2 += x * x;
                             }

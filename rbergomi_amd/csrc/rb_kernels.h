@@ -64,7 +64,7 @@ hipError_t launch_probe_mfma_layout(const float* A, const float* B, float* out, 
                                     hipStream_t st);
 
 hipError_t launch_probe_payoff(const double* lnS, const double* sig, const double* K, int n,
-                               double* out, float* mills_out, const float* z, hipStream_t st);
+                               double* out, float* mills_out, const float* z, int form, hipStream_t st);
 
 hipError_t launch_probe_copy(const void* src, void* dst, long bytes, hipStream_t st);
 

@@ -57,7 +57,7 @@ hipError_t launch_probe_mfma_layout(const float* A, const float* B, float* out, 
 // Mixed-precision payoff of the fused kernel for given (ln S', sigma, K): out[i]
 __global__ void k_probe_payoff(const double* __restrict__ lnS, const double* __restrict__ sig,
                                const double* __restrict__ K, int n, double* __restrict__ out,
-                               float* __restrict__ mills_out, const float* __restrict__ z) {
+                               float* __restrict__ mills_out, const float* __restrict__ z, int form) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     PathPG q;
@@ -67,14 +67,16 @@ __global__ void k_probe_payoff(const double* __restrict__ lnS, const double* __r
     q.invs = float(1.0 / q.sig);
     q.lnSf = float(q.lnS);
     q.sigf = float(q.sig);
-    out[i] = payoff_mixed(q, float(log(K[i])), float(K[i]), log(K[i]));
+    const double lnK = log(K[i]);
+    out[i] = form ? payoff_mixed<true>(q, float(lnK), float(K[i]), lnK, K[i])
+                  : payoff_mixed<false>(q, float(lnK), float(K[i]), lnK, K[i]);
     mills_out[i] = mills_f32(z[i]);
 }
 
 hipError_t launch_probe_payoff(const double* lnS, const double* sig, const double* K, int n,
-                               double* out, float* mills_out, const float* z, hipStream_t st) {
+                               double* out, float* mills_out, const float* z, int form, hipStream_t st) {
     hipLaunchKernelGGL(k_probe_payoff, dim3((n + 255) / 256), dim3(256), 0, st, lnS, sig, K, n, out,
-                       mills_out, z);
+                       mills_out, z, form);
     return hipGetLastError();
 }
 

@@ -162,10 +162,12 @@ def test_split_precision_routes_outside_the_fused_kernel(rb):
         assert e.value.code == 5
 
 
-def test_mixed_payoff_and_mills_ratio(rb):
+@pytest.mark.parametrize("form", [0, 1])
+def test_mixed_payoff_and_mills_ratio(rb, form):
     """The split-precision payoff (FP32 with one shared Gaussian factor and a fitted Mills
-    ratio, FP64 normal CDF below d1 = -5) against the FP64 Black-Scholes formula, and the
-    Mills-ratio fit against scipy.special.erfcx."""
+    ratio, FP64 normal CDFs below d1 = -8: form 0 through erfcx as the GEMM kernel evaluates it,
+    form 1 through erfc as the long-path kernel does) against the FP64 Black-Scholes formula,
+    and the Mills-ratio fit against scipy.special.erfcx."""
     from scipy.special import erfcx
     from scipy.stats import norm
     rng = np.random.default_rng(7)
@@ -179,7 +181,8 @@ def test_mixed_payoff_and_mills_ratio(rb):
     fp = ctypes.POINTER(ctypes.c_float)
     dp = ctypes.POINTER(ctypes.c_double)
     assert rb.load().rb_probe_payoff(lnS.ctypes.data_as(dp), sig.ctypes.data_as(dp), K.ctypes.data_as(dp), n,
-                                     out.ctypes.data_as(dp), z.ctypes.data_as(fp), mills.ctypes.data_as(fp)) == 0
+                                     out.ctypes.data_as(dp), z.ctypes.data_as(fp), mills.ctypes.data_as(fp),
+                                     form) == 0
     zz = z.astype(np.float64)
     ref_m = erfcx(zz / np.sqrt(2)) * np.sqrt(np.pi / 2)
     assert np.max(np.abs(mills / ref_m - 1)) < 5e-7
@@ -192,3 +195,11 @@ def test_mixed_payoff_and_mills_ratio(rb):
     assert np.all(err <= tol + 1e-300), np.max(err / tol)
     deep = d1 < -8
     assert deep.any() and np.all(np.abs(out[deep] - ref[deep]) <= 1e-8 * ref[deep] + 1e-300)
+    # and against 40-digit arithmetic on some deep points (the FP64 difference above cancels to
+    # ~3e-9 relative; the erfcx form does not)
+    import mpmath as mp
+    mp.mp.dps = 40
+    for i in np.flatnonzero(deep)[:40]:
+        e1 = (mp.mpf(lnS[i]) - mp.log(mp.mpf(K[i]))) / mp.mpf(sig[i]) + mp.mpf(sig[i]) / 2
+        exact = mp.e ** mp.mpf(lnS[i]) * mp.ncdf(e1) - mp.mpf(K[i]) * mp.ncdf(e1 - mp.mpf(sig[i]))
+        assert abs(out[i] - float(exact)) <= (1e-10 if form == 0 else 1e-8) * float(exact)
