@@ -268,25 +268,34 @@ int plan_fused_request(const Request& req, const Layout& lay, int mode, FusedHos
     for (int u = 0; u < nu; ++u)
         for (int i = std::max(N - 1, 0); i < Np; ++i) out.ptab[size_t(u) * Np + i] = INFINITY;
     // ---- A fragments: entry (d, lane, slot j) = G[d + m - 8h - j], G[0] = a, G[t>0] = Gamma_t
+    //      (0 outside [0, N)).  The table holds each G[t] about 64 times, so the operand split
+    //      is computed once per t and the entries copied from it.
     out.frag.assign(size_t(nu) * a.NK * 128, uint4());
+    std::vector<uint16_t> ghi(N), glo(N);
+    std::vector<float> gf(N);
     for (int u = 0; u < nu; ++u) {
         const double* gam = &lay.gamma[size_t(u) * N];
-        const double diag = lay.units[u].a;
+        for (int t = 0; t < N; ++t) {
+            const double g = t == 0 ? lay.units[u].a : gam[t];
+            if (mode == 2)
+                gf[t] = float(g);
+            else
+                split16(g, mode, ghi[t], glo[t]);
+        }
         for (int f = 0; f < a.NK; ++f) {
             const int d = 16 * f - 16;
+            uint4* e = &out.frag[(size_t(u) * a.NK + f) * 128];
             for (int lane = 0; lane < 64; ++lane) {
                 const int m = lane & 31, hh = lane >> 5;
                 uint16_t hi[8], lo[8];
                 float g32[8];
                 for (int j = 0; j < 8; ++j) {
                     const int t = d + m - 8 * hh - j;
-                    const double g = t < 0 ? 0.0 : (t == 0 ? diag : (t < N ? gam[t] : 0.0));
-                    if (mode == 2)
-                        g32[j] = float(g);
-                    else
-                        split16(g, mode, hi[j], lo[j]);
+                    const bool in = t >= 0 && t < N;
+                    hi[j] = in ? ghi[t] : 0;
+                    lo[j] = in ? glo[t] : 0;
+                    g32[j] = in ? gf[t] : 0.0f;
                 }
-                uint4* e = &out.frag[(size_t(u) * a.NK + f) * 128];
                 if (mode == 2) {  // FP32 operands: slots 0..3, then 4..7
                     std::memcpy(&e[lane], g32, 16);
                     std::memcpy(&e[64 + lane], g32 + 4, 16);
