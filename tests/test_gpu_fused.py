@@ -203,3 +203,24 @@ def test_mixed_payoff_and_mills_ratio(rb, form):
         e1 = (mp.mpf(lnS[i]) - mp.log(mp.mpf(K[i]))) / mp.mpf(sig[i]) + mp.mpf(sig[i]) / 2
         exact = mp.e ** mp.mpf(lnS[i]) * mp.ncdf(e1) - mp.mpf(K[i]) * mp.ncdf(e1 - mp.mpf(sig[i]))
         assert abs(out[i] - float(exact)) <= (1e-10 if form == 0 else 1e-8) * float(exact)
+
+
+@pytest.mark.parametrize("prec,N", [("PREC_SPLIT_FP16", 45), ("PREC_SPLIT_BF16", 45), ("PREC_FP32_GEMM", 45),
+                                    ("PREC_SPLIT_FP16", 64), ("PREC_SPLIT_FP16", 1100), ("PREC_FP32_FFT", 45)])
+def test_flat_variance_limit_mixed(rb, prec, N):
+    """eta = rho = 0: v = xi on every step, so every path's conditioned payoff is
+    C_BS(1, K, T, sqrt(xi T)) and the price equals it up to FP32 rounding.  With eta = 0 the
+    group coefficient alpha is the smallest positive float, and the steps past N of a padded row
+    block (N = 45: rows 45..64; the long-path kernel's steps past N = 1100; the FFT kernel at
+    N = 45) are zeroed only through P = +inf: one row they failed to zero would add xi dt to the
+    variance and move the price by ~5e-4 at N = 45."""
+    from scipy import stats
+    T, K, xi = [0.5, 2.0, 1.0, 0.1], [1.0, 1.2, 0.8, 1.05], 0.05
+    say("flat variance limit %s N=%d" % (prec, N))
+    price, iv, stat = rb.price_v2(xi, [0.1], [0.0], [0.0], T, K, N, 3000, [1], precision=getattr(rb, prec))
+    for j in range(len(T)):
+        s = np.sqrt(xi * T[j])
+        d1 = (np.log(1 / K[j]) + 0.5 * s * s) / s
+        want = stats.norm.cdf(d1) - K[j] * stats.norm.cdf(d1 - s)
+        assert abs(price[j] - want) < 2e-6, (j, price[j], want)
+        assert abs(iv[j] - np.sqrt(xi)) < 1e-4
