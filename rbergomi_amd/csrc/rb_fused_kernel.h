@@ -315,8 +315,8 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
                             // P_j and W1_j of the lane's rows j = 32 rb + 8 c + 4 h + t + 1, t = 0..3,
                             // read ahead of the row loop: P as one aligned float4 per c, W1 as
                             // elements 1..3 of quad 8 rb + 2 c + h and element 0 of the next quad
-                            // (which lies past the tile only for row j = Np, skipped by the loop;
-                            // that read stays on the last quad)
+                            // (which lies past the tile only for row j = Np, whose term P = +inf
+                            // zeroes; that read stays on the last quad)
                             float pv[16], wv[16];
 #pragma unroll
                             for (int c = 0; c < 4; ++c) {
