@@ -104,7 +104,6 @@ template <int MODE, int UPW, bool MC, int GR>
 __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs B) {
     static_assert(GR <= FUSED_GREG, "pgmax is sized for windows of FUSED_GREG groups");
     constexpr int NS = MC ? 3 : 2;                        // sums per (path, group)
-    constexpr bool REQ_IN_LDS = MC || UPW <= 2;           // where the request record lives
     typedef SplitT<MODE> S;
     typedef typename S::V V;
     typedef typename S::V4 V4;
@@ -126,29 +125,24 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
         const long gi = s_item;
         __syncthreads();
         if (gi >= B.total_items) break;
-        // the request record, once per item (reads through the global pointer would be reloaded
-        // after every global store).  Staged in LDS, its fields are wave-uniform and the compiler
-        // keeps them in scalar registers; with four or more row blocks per wave (Romano-Touzi)
-        // that pressure spills vector registers, and a per-lane copy (kept in scratch, since its
-        // per-wave tables are indexed by the wave number) is faster: measured 14.0 vs 14.8 ms at
-        // N = 512, against 8.2 vs 8.6 ms at N = 256 for the LDS record.
-        FusedReq Aloc;
-        if constexpr (REQ_IN_LDS) {
+        // the request record, once per item, staged in LDS (reads through the global pointer
+        // would be reloaded after every global store); the loop controls read from it are made
+        // wave-uniform (scalar branches, no exec masks).  A per-lane copy of the record (kept in
+        // scratch, since its per-wave tables are indexed by the wave number: 312 B written per
+        // lane per item) was faster with four row blocks per wave in earlier builds; with the
+        // branch-free integral loop the LDS record measured 8.70 vs 9.18 ms at N = 512, M = 2^22,
+        // and the serving trace 2.23e11 vs 2.08e11 paths*steps/s.
+        {
             const int* src = reinterpret_cast<const int*>(B.reqs + B.item_req[gi]);
             int* dst = reinterpret_cast<int*>(&s_req);
             for (int i = tid; i < int(sizeof(FusedReq) / 4); i += 256) dst[i] = src[i];
             __syncthreads();
-        } else {
-            Aloc = B.reqs[B.item_req[gi]];
         }
-        const FusedReq& A = REQ_IN_LDS ? s_req : Aloc;
+        const FusedReq& A = s_req;
         const long item = B.item_local[gi];
         const int NK = A.NK;
-        // with the per-lane record, the loop-control values are marked wave-uniform (scalar
-        // branches instead of exec masks in the integral loops): 13.3 vs 14.0 ms at N = 512;
-        // with the LDS record it measured slower (11.56 vs 11.27 ms at N = 128)
-        const int pbw = REQ_IN_LDS ? A.wave_pb[wave] : __builtin_amdgcn_readfirstlane(A.wave_pb[wave]);
-        const int nuw = REQ_IN_LDS ? A.wave_nu[wave] : __builtin_amdgcn_readfirstlane(A.wave_nu[wave]);
+        const int pbw = __builtin_amdgcn_readfirstlane(A.wave_pb[wave]);
+        const int nuw = __builtin_amdgcn_readfirstlane(A.wave_nu[wave]);
         double a1 = 0.0, a2 = 0.0;  // sums of option tid (register accumulation)
         if (A.acc_in_lds)
             for (int o = tid; o < 2 * A.out_size; o += 256) accopt[o] = 0.0;
@@ -258,7 +252,7 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
                 // ---- 3b. integrals of every group of the unit, GR groups per pass
                 for (int g0 = U.g_begin; g0 < U.g_end; g0 += GR) {
                     const int gn_ = U.g_end - g0 < GR ? U.g_end - g0 : GR;
-                    const int gn = REQ_IN_LDS ? gn_ : __builtin_amdgcn_readfirstlane(gn_);
+                    const int gn = __builtin_amdgcn_readfirstlane(gn_);
                     float c1[GR], al[GR], be[GR];
                     double sv[GR], ss[GR], sp[GR];
                     // FP32 partial sums: per 16-row block (Romano-Touzi), or over all row blocks of
