@@ -168,32 +168,32 @@ __global__ void __launch_bounds__(256, (QPT >= 8 || (MC && QPT >= 4)) ? 1 : 2) k
                         }
                     }
                     if constexpr (MC) {
+                        // plain MC: steps outside, groups inside, since the Wperp Gaussians come
+                        // per quad (held across a group loop they raised the two-quad kernel's
+                        // spills from 0 to 27 VGPRs)
 #pragma unroll
-                    for (int j = 0; j < QPT; ++j) {
-                        const int q = tid + 256 * j;
-                        float wp[2][4] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
-                        if constexpr (MC) {
+                        for (int j = 0; j < QPT; ++j) {
+                            const int q = tid + 256 * j;
+                            float wp[2][4] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
 #pragma unroll
                             for (int c = 0; c < 2; ++c)
                                 if (q < nq && pth[c] < A.M) normals4_f32(A.key, uint64_t(pth[c]), 2u, uint32_t(q), wp[c]);
-                        }
 #pragma unroll
-                        for (int t = 0; t < 4; ++t) {
-                            const int i = 4 * q + t;  // v_i uses Wtilde_{i-1}; i = 0 is v_0 = xi
-                            if (i >= 1 && i < N) {
-                                const float2 x = z[fpad(i - 1)];
-                                const float pj = P[i];
+                            for (int t = 0; t < 4; ++t) {
+                                const int i = 4 * q + t;  // v_i uses Wtilde_{i-1}; i = 0 is v_0 = xi
+                                if (i >= 1 && i < N) {
+                                    const float2 x = z[fpad(i - 1)];
+                                    const float pj = P[i];
 #pragma unroll
-                                for (int gg = 0; gg < FFT_GR; ++gg) {
-                                    if (gg < gn) {
-                                        const float e = fmaf(al[gg], pj, be[gg]);
-                                        const float ra = __builtin_amdgcn_exp2f(fmaf(x.x, c1[gg], -e));
-                                        const float rb = __builtin_amdgcn_exp2f(fmaf(x.y, c1[gg], -e));
-                                        fv[gg][0] = fmaf(ra, ra, fv[gg][0]);
-                                        fs[gg][0] = fmaf(ra, w1[0][j][t], fs[gg][0]);
-                                        fv[gg][1] = fmaf(rb, rb, fv[gg][1]);
-                                        fs[gg][1] = fmaf(rb, w1[1][j][t], fs[gg][1]);
-                                        if constexpr (MC) {
+                                    for (int gg = 0; gg < FFT_GR; ++gg) {
+                                        if (gg < gn) {
+                                            const float e = fmaf(al[gg], pj, be[gg]);
+                                            const float ra = __builtin_amdgcn_exp2f(fmaf(x.x, c1[gg], -e));
+                                            const float rb = __builtin_amdgcn_exp2f(fmaf(x.y, c1[gg], -e));
+                                            fv[gg][0] = fmaf(ra, ra, fv[gg][0]);
+                                            fs[gg][0] = fmaf(ra, w1[0][j][t], fs[gg][0]);
+                                            fv[gg][1] = fmaf(rb, rb, fv[gg][1]);
+                                            fs[gg][1] = fmaf(rb, w1[1][j][t], fs[gg][1]);
                                             fp[gg][0] = fmaf(ra, wp[0][t], fp[gg][0]);
                                             fp[gg][1] = fmaf(rb, wp[1][t], fp[gg][1]);
                                         }
@@ -201,33 +201,33 @@ __global__ void __launch_bounds__(256, (QPT >= 8 || (MC && QPT >= 4)) ? 1 : 2) k
                                 }
                             }
                         }
-                    }
                     } else {
-                    // groups outside (one uniform branch per group), steps inside with no test:
-                    // P = +inf at i = 0 and at i >= N (al > 0) makes those terms exactly 0, and
-                    // i - 1 is clamped to a written slot.  Each group's sums still run over the
-                    // steps in the same order.
+                        // Romano-Touzi: groups outside (one uniform branch per group), steps inside
+                        // with no test:
+                        // P = +inf at i = 0 and at i >= N (al > 0) makes those terms exactly 0, and
+                        // i - 1 is clamped to a written slot.  Each group's sums still run over the
+                        // steps in the same order.
 #pragma unroll
-                    for (int gg = 0; gg < FFT_GR; ++gg) {
-                        if (gg < gn) {
+                        for (int gg = 0; gg < FFT_GR; ++gg) {
+                            if (gg < gn) {
 #pragma unroll
-                            for (int j = 0; j < QPT; ++j) {
-                                const int q = tid + 256 * j;
+                                for (int j = 0; j < QPT; ++j) {
+                                    const int q = tid + 256 * j;
 #pragma unroll
-                                for (int t = 0; t < 4; ++t) {
-                                    const int i = 4 * q + t;  // v_i uses Wtilde_{i-1}; i = 0 is v_0 = xi
-                                    const float2 x = z[fpad(i > 0 ? i - 1 : 0)];
-                                    const float e = fmaf(al[gg], P[i], be[gg]);
-                                    const float ra = __builtin_amdgcn_exp2f(fmaf(x.x, c1[gg], -e));
-                                    const float rb = __builtin_amdgcn_exp2f(fmaf(x.y, c1[gg], -e));
-                                    fv[gg][0] = fmaf(ra, ra, fv[gg][0]);
-                                    fs[gg][0] = fmaf(ra, w1[0][j][t], fs[gg][0]);
-                                    fv[gg][1] = fmaf(rb, rb, fv[gg][1]);
-                                    fs[gg][1] = fmaf(rb, w1[1][j][t], fs[gg][1]);
+                                    for (int t = 0; t < 4; ++t) {
+                                        const int i = 4 * q + t;  // v_i uses Wtilde_{i-1}; i = 0 is v_0 = xi
+                                        const float2 x = z[fpad(i > 0 ? i - 1 : 0)];
+                                        const float e = fmaf(al[gg], P[i], be[gg]);
+                                        const float ra = __builtin_amdgcn_exp2f(fmaf(x.x, c1[gg], -e));
+                                        const float rb = __builtin_amdgcn_exp2f(fmaf(x.y, c1[gg], -e));
+                                        fv[gg][0] = fmaf(ra, ra, fv[gg][0]);
+                                        fs[gg][0] = fmaf(ra, w1[0][j][t], fs[gg][0]);
+                                        fv[gg][1] = fmaf(rb, rb, fv[gg][1]);
+                                        fs[gg][1] = fmaf(rb, w1[1][j][t], fs[gg][1]);
+                                    }
                                 }
                             }
                         }
-                    }
                     }
                     // FP64 over the workgroup: wave sums, then the four waves in order
 #pragma unroll
