@@ -5,8 +5,9 @@
 // coefficients, the payoff groups (group, rho), the option list and the assignment of 32-row
 // output blocks to the four waves of a workgroup (balanced triangular work).
 // FusedBatch concatenates any number of such requests into one launch: one table of requests,
-// one list of work items (1024 paths of one request each) ordered by decreasing cost and handed
-// out through an atomic counter, and one ordered reduction of every request's option sums.
+// one list of work items (up to 1024 paths of one request each, at least 256 items per request
+// where its tiles allow) ordered by decreasing cost and handed out through an atomic counter,
+// and one ordered reduction of every request's option sums.
 // A request's results depend only on its own items, so they are bit-identical alone and in any
 // batch.
 #include "rb_fused_host.h"
