@@ -17,16 +17,18 @@
 
 typedef std::vector<double> Vector;
 
+/* Output index i = h * (number of (T, K) pairs) + t for parameter triple h and option t; the
+ * per-index echo vectors say which inputs produced entry i.  The member order is the ABI. */
 struct ResultSimple {
-    Vector price;  /* prices for each combination of (H, eta, rho) and (T, K) */
-    Vector iv;     /* implied vols */
-    Vector H;      /* same size as price; H[i] belongs to price[i] */
-    Vector rho;
-    Vector eta;
-    Vector T;
-    Vector K;
+    Vector price;  /* Monte Carlo price of entry i */
+    Vector iv;     /* Black implied volatility of price[i] */
+    Vector H;      /* H of entry i */
+    Vector rho;    /* rho of entry i (rho before eta, as the reference orders them) */
+    Vector eta;    /* eta of entry i */
+    Vector T;      /* maturity of entry i */
+    Vector K;      /* strike of entry i */
     double xi;
-    Vector stat;   /* statistical errors */
+    Vector stat;   /* Monte Carlo standard error of price[i] */
     int N;
     long M;
     int numThreads;
