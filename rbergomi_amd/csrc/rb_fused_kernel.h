@@ -150,21 +150,18 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
         const long t_end = t_begin + A.tiles_per_item < A.ntiles ? t_begin + A.tiles_per_item : A.ntiles;
         for (long tile = t_begin; tile < t_end; ++tile) {
             const long p0 = tile * A.TP;
-            // ---- 1. Gaussians W1 -> B fragments (hi, lo); zero past N and past M
+            // ---- 1. Gaussians W1 -> B fragments (hi, lo); quads past N are zero.  The steps
+            //         past N inside the last quad and the paths past M keep their Gaussians:
+            //         steps >= N reach only output rows >= N, whose terms P = +inf zeroes, and
+            //         the results of paths past M are dropped
             const int nq = A.Np >> 2;
             {
                 const int pl = tid & (A.TP - 1);   // TP is 32, 64 or 128
                 const int pb = pl >> 5;
                 const long p = p0 + pl;
-                const bool live = p < A.M;
                 for (int q = tid >> A.log2TP; q < nq; q += 256 >> A.log2TP) {
                     float z[4] = {0.f, 0.f, 0.f, 0.f};
-                    if (live && 4 * q < A.N) {
-                        normals4_f32(A.key, uint64_t(p), 0u, uint32_t(q), z);
-#pragma unroll
-                        for (int j = 0; j < 4; ++j)
-                            if (4 * q + j >= A.N) z[j] = 0.f;
-                    }
+                    if (4 * q < A.N) normals4_f32(A.key, uint64_t(p), 0u, uint32_t(q), z);
                     const int ks = q >> 2, hh = (q & 3) >> 1, j0 = (q & 1);
                     const int ln = 32 * hh + (pl & 31);
                     uint4* base = Bf + ((pb * NK + ks) * 2) * 64;
@@ -241,8 +238,8 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
                         const int rb = A.wave_rb[wave][s];
 #pragma unroll
                         for (int c = 0; c < 4; ++c) {
-                            float z[4] = {0.f, 0.f, 0.f, 0.f};
-                            if (p < A.M) normals4_f32(A.key, uint64_t(p), 1u, uint32_t(8 * rb + 2 * c + h), z);
+                            float z[4];  // paths past M too: their results are dropped
+                            normals4_f32(A.key, uint64_t(p), 1u, uint32_t(8 * rb + 2 * c + h), z);
 #pragma unroll
                             for (int t = 0; t < 4; ++t)
                                 acc[s][4 * c + t] = U.sc * (acc[s][4 * c + t] + U.b * z[t]);
@@ -288,14 +285,11 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
                             // for (c = 3, h = 1) the first quad of the next row block.
                             float wq[16];
                             if constexpr (MC) {
-                                float zc[4][4], ze[4] = {0.f, 0.f, 0.f, 0.f};
+                                float zc[4][4], ze[4];
 #pragma unroll
-                                for (int c = 0; c < 4; ++c) {
-#pragma unroll
-                                    for (int t = 0; t < 4; ++t) zc[c][t] = 0.f;
-                                    if (p < A.M) normals4_f32(A.key, uint64_t(p), 2u, uint32_t(8 * rb + 2 * c + h), zc[c]);
-                                }
-                                if (p < A.M) normals4_f32(A.key, uint64_t(p), 2u, uint32_t(8 * rb + 8), ze);
+                                for (int c = 0; c < 4; ++c)
+                                    normals4_f32(A.key, uint64_t(p), 2u, uint32_t(8 * rb + 2 * c + h), zc[c]);
+                                normals4_f32(A.key, uint64_t(p), 2u, uint32_t(8 * rb + 8), ze);
                                 float y[4];
 #pragma unroll
                                 for (int c = 0; c < 4; ++c) y[c] = __shfl_xor(zc[c][0], 32, 64);
@@ -395,8 +389,8 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
                             double a = G.xi, b = G.sxi * w10;   // j = 0: v_0 = xi
                             double cp = 0.0;
                             if constexpr (MC) {
-                                float z0[4] = {0.f, 0.f, 0.f, 0.f};
-                                if (p0 + pl < A.M) normals4_f32(A.key, uint64_t(p0 + pl), 2u, 0u, z0);
+                                float z0[4];
+                                normals4_f32(A.key, uint64_t(p0 + pl), 2u, 0u, z0);
                                 cp = G.sxi * double(z0[0]);
                             }
                             for (int w = 0; w < 4; ++w)
