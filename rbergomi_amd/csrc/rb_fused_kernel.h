@@ -92,6 +92,24 @@ __device__ __forceinline__ void tile_w1_quad(const uint4* Bf, int blk, int NK, i
     }
 }
 
+// hi + lo of one split W1 element: UPPER picks the upper 16 bits of both dwords, else the lower.
+// FP16: one v_fma_mix_f32 (hi x 1.0 + lo with both operands widened exactly, rounded once), the
+// same bits as float(hi) + float(lo) without the two conversions.  BF16: the widening is a shift.
+template <int MODE, bool UPPER>
+__device__ __forceinline__ float split_sum(uint32_t hi, uint32_t lo) {
+    if constexpr (MODE == 0) {
+        float r;
+        if constexpr (UPPER)
+            asm("v_fma_mix_f32 %0, %1, 1.0, %2 op_sel:[1,0,1] op_sel_hi:[1,0,1]" : "=v"(r) : "v"(hi), "v"(lo));
+        else
+            asm("v_fma_mix_f32 %0, %1, 1.0, %2 op_sel_hi:[1,0,1]" : "=v"(r) : "v"(hi), "v"(lo));
+        return r;
+    } else {
+        return UPPER ? __uint_as_float(hi & 0xffff0000u) + __uint_as_float(lo & 0xffff0000u)
+                     : __uint_as_float(hi << 16) + __uint_as_float(lo << 16);
+    }
+}
+
 __device__ __forceinline__ double half_sum_f64(double v) {
     // sum over the 32 lanes of one half-wave; every lane of the half receives the total
 #pragma unroll
@@ -273,7 +291,7 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
 #pragma unroll
                         for (int s = 0; s < UPW; ++s) {
                             if (s >= nuw) break;
-                            const int rb = A.wave_rb[wave][s];
+                            const int rb = __builtin_amdgcn_readfirstlane(A.wave_rb[wave][s]);
                             if constexpr (!MC) {
 #pragma unroll
                                 for (int gg = 0; gg < GR; ++gg) fv[gg] = fs[gg] = 0.f;
@@ -313,14 +331,43 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
                                 pv[4 * c + 1] = v.y;
                                 pv[4 * c + 2] = v.z;
                                 pv[4 * c + 3] = v.w;
-                                const int Q = 8 * rb + 2 * c + h;
-                                float qa[4], qb[4];
-                                tile_w1_quad<MODE>(Bf, pbw, NK, Q, col, qa);
-                                tile_w1_quad<MODE>(Bf, pbw, NK, Q + 1 < 4 * NK ? Q + 1 : Q, col, qb);
-                                wv[4 * c + 0] = qa[1];
-                                wv[4 * c + 1] = qa[2];
-                                wv[4 * c + 2] = qa[3];
-                                wv[4 * c + 3] = qb[0];
+                            }
+                            if constexpr (MODE == 2) {
+#pragma unroll
+                                for (int c = 0; c < 4; ++c) {
+                                    const int Q = 8 * rb + 2 * c + h;
+                                    float qa[4], qb[4];
+                                    tile_w1_quad<MODE>(Bf, pbw, NK, Q, col, qa);
+                                    tile_w1_quad<MODE>(Bf, pbw, NK, Q + 1 < 4 * NK ? Q + 1 : Q, col, qb);
+                                    wv[4 * c + 0] = qa[1];
+                                    wv[4 * c + 1] = qa[2];
+                                    wv[4 * c + 2] = qa[3];
+                                    wv[4 * c + 3] = qb[0];
+                                }
+                            } else {
+                                // split modes, byte offsets from slot col of chunk 2 rb: quad Q is
+                                // the uint2 h of slot 32 (c & 1) + col in chunk 2 rb + c / 2 (lo
+                                // 1024 B further).  Element 0 of quad Q + 1 begins the next uint2
+                                // (h = 0), slot 32 + col (h = 1, c even) or slot col of the next
+                                // chunk (h = 1, c odd); for c = 3 in the last row block that is
+                                // past the tile, and the row (j = Np) has P = +inf, so the read
+                                // takes element 0 of the lane's own slot instead
+                                const char* Wr = reinterpret_cast<const char*>(Bf + ((pbw * NK + 2 * rb) * 2) * 64 + col);
+                                const int nb_even = h ? 512 : 8, nb_odd = h ? 1536 : 8;
+                                const int nb3 = 2 * rb + 2 < NK ? nb_odd : (h ? 0 : 8);
+#pragma unroll
+                                for (int c = 0; c < 4; ++c) {
+                                    const char* slot = Wr + (c >> 1) * 2048 + (c & 1) * 512;
+                                    const uint2 ha = *reinterpret_cast<const uint2*>(slot + 8 * h);
+                                    const uint2 la = *reinterpret_cast<const uint2*>(slot + 8 * h + 1024);
+                                    const int nb = c == 3 ? nb3 : (c & 1) ? nb_odd : nb_even;
+                                    const uint32_t hb = *reinterpret_cast<const uint32_t*>(slot + nb);
+                                    const uint32_t lb = *reinterpret_cast<const uint32_t*>(slot + nb + 1024);
+                                    wv[4 * c + 0] = split_sum<MODE, true>(ha.x, la.x);
+                                    wv[4 * c + 1] = split_sum<MODE, false>(ha.y, la.y);
+                                    wv[4 * c + 2] = split_sum<MODE, true>(ha.y, la.y);
+                                    wv[4 * c + 3] = split_sum<MODE, false>(hb, lb);
+                                }
                             }
                             // rows j = 32 rb + row + 1 >= N read P = +inf (al > 0), so rr = 0
                             // and they add nothing: no per-row test.  Groups outside, rows
