@@ -15,8 +15,9 @@ struct FusedUnit {
     float b;         // W1perp weight of the first-cell term
     float sc;        // sqrt(2H) N^-H
     int frag_off;    // offset of this unit's A-fragment table, in 16-byte entries
-    int p_off;       // offset of the Np-entry table P_j = (j - off)^(2H), in floats (the GEMM
-                     // kernel's table holds P_j at j - 1, the long-path kernel's at j)
+    int p_off;       // offset of the unit's row tables, in floats.  Long-path kernel: one table
+                     // P_j = (j - off)^(2H), P_j at j.  GEMM kernel: per group of the unit (in
+                     // order, Np floats each) the row terms al P_j + be, at j - 1
     int g_begin, g_end;    // groups of this unit (contiguous)
     int pg_begin, pg_end;  // payoff groups (group, rho) of this unit (contiguous)
     int o_begin, o_end;    // options of this unit (contiguous)
@@ -75,7 +76,7 @@ struct FusedReq {
     const FusedPG* pgs;
     const FusedOption* opts;
     const uint4* frag;           // A fragments: per unit [Np/16][hi, lo][64 lanes] x 16 bytes
-    const float* ptab;           // per unit Np floats
+    const float* ptab;           // row terms al P_j + be, Np floats per group (FusedUnit::p_off)
     double* partial;             // [out_size][nitems][2]
     int wave_pb[4];
     int wave_nu[4];              // row blocks of each wave, listed in decreasing order

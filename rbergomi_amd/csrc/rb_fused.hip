@@ -17,6 +17,7 @@
 //      same counters, then for every (H, eta, T) group sqrt(v_{i+1}) = exp2(c1 Wtilde_i -
 //      al P_{i+1} - be) and the sums of v and sqrt(v) W1 (scaleWtilde/updateV/intVdt/
 //      intRootVdW, :73-80, 163-167, 427-436): FP32 terms summed per 16-row block, FP64 beyond.
+//      The row terms al P_{i+1} + be come from a per-group table (path-independent).
 //   4. Per (path, group, rho): ln S', sigma, 1/sigma, S' once; per strike the conditioned
 //      Black-Scholes payoff S' Phi(d1) - K Phi(d2) (:383-391) in FP32, with the FP64 normal
 //      CDF for deep out-of-the-money lanes (d1 < -8), for the options of up to 8 groups at a

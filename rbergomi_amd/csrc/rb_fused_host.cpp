@@ -268,6 +268,21 @@ int plan_fused_request(const Request& req, const Layout& lay, int mode, FusedHos
     // integrand exp2(c1 x - al P - be) is exactly 0 there (the kernel does not test rows)
     for (int u = 0; u < nu; ++u)
         for (int i = std::max(N - 1, 0); i < Np; ++i) out.ptab[size_t(u) * Np + i] = INFINITY;
+    // the kernel reads each group's row terms al P_j + be (Np per group, the unit's groups in
+    // order from p_off): one FP32 multiply-add rounded once, as the device's fmaf would give,
+    // taken out of the kernel's (row, group) loop.  al P = +inf stays +inf.
+    {
+        std::vector<float> q;
+        q.reserve(out.fg.size() * size_t(Np));
+        for (int u = 0; u < nu; ++u) {
+            FusedUnit& U = out.fu[u];
+            const float* P = &out.ptab[size_t(u) * Np];
+            U.p_off = int(q.size());
+            for (int g = U.g_begin; g < U.g_end; ++g)
+                for (int i = 0; i < Np; ++i) q.push_back(std::fmaf(out.fg[g].al, P[i], out.fg[g].be));
+        }
+        out.ptab.swap(q);
+    }
     // ---- A fragments: entry (d, lane, slot j) = G[d + m - 8h - j], G[0] = a, G[t>0] = Gamma_t
     //      (0 outside [0, N)).  The table holds each G[t] about 64 times, so the operand split
     //      is computed once per t and the entries copied from it.

@@ -57,6 +57,25 @@ __device__ __forceinline__ float4 load_global_f4(const float* p) {
     return make_float4(v.x, v.y, v.z, v.w);
 }
 
+// a float from global memory (a generic pointer would give a FLAT load, after which the compiler
+// waits for every outstanding vector memory load before any use of one)
+__device__ __forceinline__ float load_global_f(const float* p) {
+    return *(const __attribute__((address_space(1))) float*)(p);
+}
+
+// 16 floats of a row table at p, p + 8, p + 16, p + 24 (four aligned float4: a lane's rows
+// 8 c + 0..3, c = 0..3)
+__device__ __forceinline__ void load_rows16(const float* p, float v[16]) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const float4 x = load_global_f4(p + 8 * c);
+        v[4 * c + 0] = x.x;
+        v[4 * c + 1] = x.y;
+        v[4 * c + 2] = x.z;
+        v[4 * c + 3] = x.w;
+    }
+}
+
 // W1 of path column `col`, step 16 ks + kk, from the LDS tile (the value the product used)
 template <int MODE>
 __device__ __forceinline__ float tile_w1(const uint4* chunk, int kk, int col) {
@@ -90,6 +109,20 @@ __device__ __forceinline__ void tile_w1_quad(const uint4* Bf, int blk, int NK, i
 #pragma unroll
         for (int t = 0; t < 4; ++t) w[t] = float(hi[t]) + float(lo[t]);
     }
+}
+
+// FP16 split of two floats, packed in pairs: hi = RN16(x), lo = RN16(x - hi).  x - hi is exact in
+// FP32, and v_fma_mixlo/mixhi_f16 form it from the FP16 hi (widened exactly) and round it once to
+// FP16 into the low or high half of lo: the same bits as converting x - float(hi), without
+// widening hi back to FP32 and packing the halves.
+__device__ __forceinline__ void split2_f16(float x0, float x1, uint32_t& hi, uint32_t& lo) {
+    typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+    const h2 h = {(_Float16)x0, (_Float16)x1};
+    hi = __builtin_bit_cast(uint32_t, h);
+    uint32_t l;
+    asm("v_fma_mixlo_f16 %0, %1, -1.0, %2 op_sel_hi:[1,0,0]" : "=v"(l) : "v"(hi), "v"(x0));
+    asm("v_fma_mixhi_f16 %0, %1, -1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "+v"(l) : "v"(hi), "v"(x1));
+    lo = l;
 }
 
 // hi + lo of one split W1 element: UPPER picks the upper 16 bits of both dwords, else the lower.
@@ -186,6 +219,12 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
                     if constexpr (MODE == 2) {
                         const V4 v = {z[0], z[1], z[2], z[3]};
                         base[64 * j0 + ln] = __builtin_bit_cast(uint4, v);
+                    } else if constexpr (MODE == 0) {
+                        uint2 hi, lo;
+                        split2_f16(z[0], z[1], hi.x, lo.x);
+                        split2_f16(z[2], z[3], hi.y, lo.y);
+                        reinterpret_cast<uint2*>(base + ln)[j0] = hi;
+                        reinterpret_cast<uint2*>(base + 64 + ln)[j0] = lo;
                     } else {
                         V4 hi, lo;
 #pragma unroll
@@ -268,7 +307,7 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
                 for (int g0 = U.g_begin; g0 < U.g_end; g0 += GR) {
                     const int gn_ = U.g_end - g0 < GR ? U.g_end - g0 : GR;
                     const int gn = __builtin_amdgcn_readfirstlane(gn_);
-                    float c1[GR], al[GR], be[GR];
+                    float c1[GR];
                     double sv[GR], ss[GR], sp[GR];
                     // FP32 partial sums: per 16-row block (Romano-Touzi), or over all row blocks of
                     // the wave (plain Monte Carlo, at most 128 terms a lane; frees the registers
@@ -278,15 +317,11 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
                     for (int gg = 0; gg < GR; ++gg) {
                         sv[gg] = ss[gg] = sp[gg] = 0.0;
                         fv[gg] = fs[gg] = fp[gg] = 0.f;
-                        c1[gg] = al[gg] = be[gg] = 0.f;
-                        if (gg < gn) {
-                            c1[gg] = A.groups[g0 + gg].c1;
-                            al[gg] = A.groups[g0 + gg].al;
-                            be[gg] = A.groups[g0 + gg].be;
-                        }
+                        c1[gg] = gg < gn ? load_global_f(&A.groups[g0 + gg].c1) : 0.f;
                     }
                     if (pbw >= 0) {
-                        const float* P = A.ptab + U.p_off;  // P_j at index j - 1
+                        // row terms al P_j + be: group g0 + gg at gg Np + j - 1
+                        const float* Qt = A.ptab + U.p_off + (g0 - U.g_begin) * A.Np;
                         const long p = p0 + 32 * pbw + col;
 #pragma unroll
                         for (int s = 0; s < UPW; ++s) {
@@ -318,20 +353,12 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
                                     wq[4 * c + 3] = h == 0 ? y[c] : (c < 3 ? y[c < 3 ? c + 1 : 3] : ze[0]);
                                 }
                             }
-                            // P_j and W1_j of the lane's rows j = 32 rb + 8 c + 4 h + t + 1, t = 0..3,
-                            // read ahead of the row loop: P as one aligned float4 per c, W1 as
-                            // elements 1..3 of quad 8 rb + 2 c + h and element 0 of the next quad
-                            // (which lies past the tile only for row j = Np, whose term P = +inf
-                            // zeroes; that read stays on the last quad)
-                            float pv[16], wv[16];
-#pragma unroll
-                            for (int c = 0; c < 4; ++c) {
-                                const float4 v = load_global_f4(P + 32 * rb + 8 * c + 4 * h);
-                                pv[4 * c + 0] = v.x;
-                                pv[4 * c + 1] = v.y;
-                                pv[4 * c + 2] = v.z;
-                                pv[4 * c + 3] = v.w;
-                            }
+                            // W1_j of the lane's rows j = 32 rb + 8 c + 4 h + t + 1, t = 0..3, read
+                            // ahead of the row loop: elements 1..3 of quad 8 rb + 2 c + h and
+                            // element 0 of the next quad (which lies past the tile only for row
+                            // j = Np, whose row term is +inf; that read stays on the last quad)
+                            float wv[16];
+                            const float* Qr = Qt + 32 * rb + 4 * h;
                             if constexpr (MODE == 2) {
 #pragma unroll
                                 for (int c = 0; c < 4; ++c) {
@@ -369,16 +396,25 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
                                     wv[4 * c + 3] = split_sum<MODE, false>(hb, lb);
                                 }
                             }
-                            // rows j = 32 rb + row + 1 >= N read P = +inf (al > 0), so rr = 0
-                            // and they add nothing: no per-row test.  Groups outside, rows
-                            // inside: one uniform branch per group instead of one per row and
-                            // group; each group's sum still runs over r = 0..15 in order.
+                            // rows j = 32 rb + row + 1 >= N have the row term +inf (P = +inf,
+                            // al > 0), so rr = 0 and they add nothing: no per-row test.  Groups
+                            // outside, rows inside: one uniform branch per group instead of one
+                            // per row and group; each group's sum still runs over r = 0..15 in
+                            // order.  The lane's 16 row terms of a group are four aligned float4
+                            // loads, those of the next group issued before this group's rows (two
+                            // buffers by the parity of the group).  The loads are issued on every
+                            // path (groups past gn reload the last one), so the waits before a
+                            // group's rows count only the loads of the group after it.
+                            float qv[2][16];
+                            load_rows16(Qr, qv[0]);
 #pragma unroll
                             for (int gg = 0; gg < GR; ++gg) {
-                                if (gg < gn) {
+                                if (gg + 1 < GR)
+                                    load_rows16(Qr + (gg + 1 < gn ? gg + 1 : gn - 1) * A.Np, qv[(gg + 1) & 1]);
+                                if (gg == 0 || gg < gn) {   // a window has at least one group
 #pragma unroll
                                     for (int r = 0; r < 16; ++r) {
-                                        const float rr = __builtin_amdgcn_exp2f(fmaf(acc[s][r], c1[gg], -fmaf(al[gg], pv[r], be[gg])));
+                                        const float rr = __builtin_amdgcn_exp2f(fmaf(acc[s][r], c1[gg], -qv[gg & 1][r]));
                                         fv[gg] = fmaf(rr, rr, fv[gg]);
                                         fs[gg] = fmaf(rr, wv[r], fs[gg]);
                                         if constexpr (MC) fp[gg] = fmaf(rr, wq[r], fp[gg]);
