@@ -50,15 +50,42 @@ struct FusedOption {
 };
 
 // per (path, payoff group) values shared by all strikes (plain Monte Carlo: lnS = ln S_T and
-// sig holds S_T itself)
+// sig holds S_T itself, the FP32 fields are unused).  The FP32 fields give the conditioned
+// Black-Scholes d1, d2 scaled by s = sqrt(log2(e) / 2), so that K phi(d2) is one exp2:
+// s d2 = A - ln K ik, s d1 = s d2 + sg, K phi(d2) = exp2(log2(K / sqrt(2 pi)) - (s d2)^2).
 struct PathPG {
     double lnS;      // ln S' = -rho^2 Iv / 2 + rho Is
     double sig;      // sqrt((1 - rho^2) Iv)
-    float S;
-    float invs;
-    float lnSf;
-    float sigf;
+    float S;         // S'
+    float ik;        // s / sig
+    float A;         // s (ln S' / sig - sig / 2)
+    float sg;        // s sig
 };
+
+constexpr double PAYOFF_S = 0.8493218002880191;   // sqrt(log2(e) / 2)
+
+// the FP32 fields of a Romano-Touzi record from its FP64 ln S' and sigma
+__device__ __forceinline__ void pathpg_f32(PathPG& q) {
+    q.S = float(exp(q.lnS));
+    q.ik = float(PAYOFF_S / q.sig);
+    q.A = float(PAYOFF_S * (q.lnS / q.sig - 0.5 * q.sig));
+    q.sg = float(PAYOFF_S * q.sig);
+}
+
+// per-strike FP32 constants of the mixed payoff
+struct OptF {
+    float lnK;       // ln K
+    float K;
+    float LK;        // log2(K / sqrt(2 pi))
+};
+
+__device__ __forceinline__ OptF opt_f32(double lnK, double K) {
+    OptF o;
+    o.lnK = float(lnK);
+    o.K = float(K);
+    o.LK = float(lnK * 1.4426950408889634 - 1.3257480647361595);
+    return o;
+}
 
 // One request of a batch: its tables, sizes and the split of its output row blocks over the
 // four waves of a workgroup.
@@ -99,26 +126,27 @@ struct FusedBatchArgs {
 
 // Mills ratio M(z) = (1 - Phi(z)) / phi(z) for z >= 0, as M = t P(t), t = 1/(1 + z/3), with P a
 // degree-10 polynomial fitted here (weighted least squares at Chebyshev nodes in t) to
-// sqrt(pi/2) erfcx(z/sqrt 2); relative error <= 3.3e-7 in FP32 over z >= 0 (tests compare it
-// with scipy.special.erfcx).  M(z) -> 1/z as z -> infinity.  t is the hardware reciprocal
-// refined by one Newton step (within an ulp of 1/(1 + z/3); four instructions in place of the
-// ten of a correctly rounded division); the denominator is capped at 1e30 so that z = inf gives
-// M ~ 3e-31 rather than NaN (every caller multiplies M by exp(-z^2/2), which is 0 there).
-__device__ __forceinline__ float mills_f32(float z) {
+// sqrt(pi/2) erfcx(z/sqrt 2); relative error <= 3.5e-7 in FP32 over z >= 0 with t rounded
+// correctly, <= 4.9e-7 with t from the hardware reciprocal (within an ulp; tests compare it
+// with scipy.special.erfcx).  M(z) -> 1/z as z -> infinity; z = inf gives t = 0 and M = 0.
+// mills_den takes the denominator 1 + z/3 itself.
+__device__ __forceinline__ float mills_den(float d) {
     const float c[11] = {3.333333433e-01f, 3.333306611e-01f, 2.964155674e-01f, 2.201747149e-01f, 1.412241310e-01f, -6.350896508e-02f, 2.205852419e-01f, -5.401814580e-01f, 4.626366198e-01f, -1.762671471e-01f, 2.557139099e-02f};
-    const float d = fminf(fmaf(z, 0.333333333f, 1.0f), 1e30f);
-    const float r = __builtin_amdgcn_rcpf(d);
-    const float t = fmaf(r, fmaf(-d, r, 1.0f), r);
+    const float t = __builtin_amdgcn_rcpf(d);
     float acc = c[10];
 #pragma unroll
     for (int i = 9; i >= 0; --i) acc = fmaf(acc, t, c[i]);
     return acc * t;
 }
 
+__device__ __forceinline__ float mills_f32(float z) { return mills_den(fmaf(z, 0.333333333f, 1.0f)); }
+
 // conditioned Black-Scholes payoff of one path for one strike (tau = 1, S0 = S').
 // With E = S' phi(d1) = K phi(d2):  S' Phi(d1) = d1 >= 0 ? S' - E M(d1) : E M(-d1), and the
 // same for K Phi(d2), so one exponential serves both normal CDFs; the three sign cases are
-// selects (no divergent branch in the strike loop).  Deep out of the money (d1 < -8,
+// selects (no divergent branch in the strike loop).  In FP32 from the record's scaled fields
+// (u = s d, s = sqrt(log2(e) / 2)): u2 = A - ln K ik, u1 = u2 + sg, E = exp2(LK - u2^2), and
+// the Mills-ratio denominators 1 + |u| / (3 s).  Deep out of the money (d1 < -8,
 // Phi(d1) < 7e-16) the normal CDFs are evaluated in FP64, in one of two forms of the same value:
 // Phi(e) = exp(-e^2/2) erfcx(-e/sqrt 2) / 2 with S' exp(-e1^2/2) = K exp(-e2^2/2), one
 // exponential (the default: fewer instructions), or S' Phi(e1) - K Phi(e2) through erfc
@@ -126,27 +154,38 @@ __device__ __forceinline__ float mills_f32(float z) {
 // registers; the erfcx form raised its register spills from 51 to 121 at N = 4096).
 // Multiply-adds are written out (no implicit contraction), so every kernel instantiation that
 // inlines this computes the same bits.
-template <bool LOWREG = false>
-__device__ __forceinline__ double payoff_mixed(const PathPG& q, float lnKf, float Kf, double lnK, double K) {
+// the FP32 part; deep: d1 < -8, where the caller takes payoff_deep instead
+__device__ __forceinline__ double payoff_f32(const PathPG& q, const OptF& o, bool& deep) {
 #pragma clang fp contract(off)
-    const float S = q.S, sigf = q.sigf;
-    const float d1 = fmaf(q.lnSf - lnKf, q.invs, 0.5f * sigf);
-    const float d2 = d1 - sigf;
-    const float E = __expf(fmaf(-0.5f * d1, d1, q.lnSf)) * 0.398942280f;
-    const float a1 = mills_f32(fabsf(d1)), a2 = mills_f32(fabsf(d2));
-    const float x_itm = fmaf(E, a2 - a1, S - Kf);   // d2 >= 0
-    const float x_mid = fmaf(-E, a1 + a2, S);       // d1 >= 0 > d2
-    const float x_otm = E * (a1 - a2);              // d1 < 0
-    double x = double(d2 >= 0.f ? x_itm : (d1 >= 0.f ? x_mid : x_otm));
-    if (d1 < -8.0f) {
-        const double e1 = (q.lnS - lnK) / q.sig + 0.5 * q.sig, e2 = e1 - q.sig;
-        if constexpr (LOWREG)
-            x = exp(q.lnS) * phi_f64(e1) - K * phi_f64(e2);
-        else
-            x = 0.5 * exp(q.lnS - 0.5 * e1 * e1) *
-                (erfcx(-0.7071067811865476 * e1) - erfcx(-0.7071067811865476 * e2));
-    }
-    return x;
+    const float u2 = fmaf(-o.lnK, q.ik, q.A);
+    const float u1 = u2 + q.sg;
+    const float E = __builtin_amdgcn_exp2f(fmaf(-u2, u2, o.LK));
+    const float k3 = float(1.0 / (3.0 * PAYOFF_S));
+    const float a1 = mills_den(fmaf(fabsf(u1), k3, 1.0f)), a2 = mills_den(fmaf(fabsf(u2), k3, 1.0f));
+    const float x_itm = fmaf(E, a2 - a1, q.S - o.K);   // d2 >= 0
+    const float x_mid = fmaf(-E, a1 + a2, q.S);        // d1 >= 0 > d2
+    const float x_otm = -E * (a2 - a1);                // d1 < 0: E (a1 - a2), same bits
+    deep = u1 < float(-8.0 * PAYOFF_S);
+    return double(u2 >= 0.f ? x_itm : (u1 >= 0.f ? x_mid : x_otm));
+}
+
+// the FP64 deep out-of-the-money form
+template <bool LOWREG = false>
+__device__ __forceinline__ double payoff_deep(const PathPG& q, double lnK, double K) {
+#pragma clang fp contract(off)
+    const double e1 = (q.lnS - lnK) / q.sig + 0.5 * q.sig, e2 = e1 - q.sig;
+    if constexpr (LOWREG)
+        return exp(q.lnS) * phi_f64(e1) - K * phi_f64(e2);
+    else
+        return 0.5 * exp(q.lnS - 0.5 * e1 * e1) *
+               (erfcx(-0.7071067811865476 * e1) - erfcx(-0.7071067811865476 * e2));
+}
+
+template <bool LOWREG = false>
+__device__ __forceinline__ double payoff_mixed(const PathPG& q, const OptF& o, double lnK, double K) {
+    bool deep;
+    const double x = payoff_f32(q, o, deep);
+    return deep ? payoff_deep<LOWREG>(q, lnK, K) : x;
 }
 
 // plain Monte Carlo payoff (S_T - K)^+ in FP64 (S_T in the sig slot)

@@ -269,9 +269,7 @@ __global__ void __launch_bounds__(256, (QPT >= 8 || (MC && QPT >= 4)) ? 1 : 2) k
                                 rec.lnS = rho * Is + sqrt(1.0 - rho * rho) * Ip - 0.5 * Iv;
                                 rec.sig = exp(rec.lnS);
                                 rec.S = float(rec.sig);
-                                rec.invs = 0.f;
-                                rec.lnSf = float(rec.lnS);
-                                rec.sigf = 0.f;
+                                rec.ik = rec.A = rec.sg = 0.f;
                                 pgd[c * A.pgu + (pg - U.pg_begin)] = rec;
                             }
                         }
@@ -280,10 +278,7 @@ __global__ void __launch_bounds__(256, (QPT >= 8 || (MC && QPT >= 4)) ? 1 : 2) k
                             PathPG rec;
                             rec.sig = sqrt((1.0 - rho * rho) * Iv);
                             rec.lnS = -0.5 * rho * rho * Iv + rho * Is;
-                            rec.S = float(exp(rec.lnS));
-                            rec.invs = float(1.0 / rec.sig);
-                            rec.lnSf = float(rec.lnS);
-                            rec.sigf = float(rec.sig);
+                            pathpg_f32(rec);
                             pgd[c * A.pgu + (pg - U.pg_begin)] = rec;
                         }
                     }
@@ -292,12 +287,12 @@ __global__ void __launch_bounds__(256, (QPT >= 8 || (MC && QPT >= 4)) ? 1 : 2) k
                 // ---- 5. payoffs of the unit's options for both paths (one thread per option)
                 for (int o = U.o_begin + tid; o < U.o_end; o += 256) {
                     const FusedOption O = A.opts[o];
-                    const float lnKf = float(O.lnK), Kf = float(O.K);
+                    const OptF of = opt_f32(O.lnK, O.K);
                     double s1 = 0.0, s2 = 0.0;
                     for (int c = 0; c < 2; ++c) {
                         if (pth[c] >= A.M) continue;
                         const PathPG& rec = pgd[c * A.pgu + (O.pg - U.pg_begin)];
-                        const double x = MC ? payoff_mc(rec, O.K) : payoff_mixed<true>(rec, lnKf, Kf, O.lnK, O.K);
+                        const double x = MC ? payoff_mc(rec, O.K) : payoff_mixed<true>(rec, of, O.lnK, O.K);
                         s1 += x;
                         s2 += x * x;
                     }

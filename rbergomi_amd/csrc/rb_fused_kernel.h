@@ -492,9 +492,7 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
                                     q.lnS = rho * Is + sqrt(1.0 - rho * rho) * Ip - 0.5 * Iv;
                                     q.sig = exp(q.lnS);
                                     q.S = float(q.sig);
-                                    q.invs = 0.f;
-                                    q.lnSf = float(q.lnS);
-                                    q.sigf = 0.f;
+                                    q.ik = q.A = q.sg = 0.f;
                                     pgd[pl * A.pgmax + (pg - pg_lo)] = q;
                                 }
                                 continue;
@@ -504,10 +502,7 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
                                 PathPG q;
                                 q.sig = sqrt((1.0 - rho * rho) * Iv);
                                 q.lnS = -0.5 * rho * rho * Iv + rho * Is;
-                                q.S = float(exp(q.lnS));
-                                q.invs = float(1.0 / q.sig);
-                                q.lnSf = float(q.lnS);
-                                q.sigf = float(q.sig);
+                                pathpg_f32(q);
                                 pgd[pl * A.pgmax + (pg - pg_lo)] = q;
                             }
                         }
@@ -524,11 +519,11 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
                         for (int oi = o_lo + tid; oi < o_hi; oi += 256) {
                             const FusedOption O = A.opts[oi];
                             const int pgl = O.pg - pg_lo;
-                            const float lnKf = float(O.lnK), Kf = float(O.K);
+                            const OptF of = opt_f32(O.lnK, O.K);
                             double s1 = 0.0, s2 = 0.0;
                             for (int pl = 0; pl < np; ++pl) {
                                 const double x = MC ? payoff_mc(pgd[pl * A.pgmax + pgl], O.K)
-                                                    : payoff_mixed(pgd[pl * A.pgmax + pgl], lnKf, Kf, O.lnK, O.K);
+                                                    : payoff_mixed(pgd[pl * A.pgmax + pgl], of, O.lnK, O.K);
                                 s1 += x;
                                 s2 += x * x;
                             }
@@ -541,11 +536,11 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
                         if (oi >= o_lo && oi < o_hi) {
                             const FusedOption O = A.opts[oi];
                             const int pgl = O.pg - pg_lo;
-                            const float lnKf = float(O.lnK), Kf = float(O.K);
+                            const OptF of = opt_f32(O.lnK, O.K);
                             double s1 = 0.0, s2 = 0.0;
                             for (int pl = sl; pl < np; pl += S) {
                                 const double x = MC ? payoff_mc(pgd[pl * A.pgmax + pgl], O.K)
-                                                    : payoff_mixed(pgd[pl * A.pgmax + pgl], lnKf, Kf, O.lnK, O.K);
+                                                    : payoff_mixed(pgd[pl * A.pgmax + pgl], of, O.lnK, O.K);
                                 s1 += x;
                                 s2 += x * x;
                             }

@@ -63,13 +63,10 @@ __global__ void k_probe_payoff(const double* __restrict__ lnS, const double* __r
     PathPG q;
     q.lnS = lnS[i];
     q.sig = sig[i];
-    q.S = float(exp(q.lnS));
-    q.invs = float(1.0 / q.sig);
-    q.lnSf = float(q.lnS);
-    q.sigf = float(q.sig);
+    pathpg_f32(q);
     const double lnK = log(K[i]);
-    out[i] = form ? payoff_mixed<true>(q, float(lnK), float(K[i]), lnK, K[i])
-                  : payoff_mixed<false>(q, float(lnK), float(K[i]), lnK, K[i]);
+    const OptF of = opt_f32(lnK, K[i]);
+    out[i] = form ? payoff_mixed<true>(q, of, lnK, K[i]) : payoff_mixed<false>(q, of, lnK, K[i]);
     mills_out[i] = mills_f32(z[i]);
 }
 
