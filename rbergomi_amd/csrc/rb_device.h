@@ -71,6 +71,9 @@ __device__ __forceinline__ void box_muller_f64(uint32_t x0, uint32_t x1, double&
 // x 2^-32 + 2^-33: the product is exact, so it rounds once where (x + 0.5) rounds once, and the
 // two agree for every 32-bit x (checked exhaustively on the host).
 __device__ __forceinline__ void box_muller_f32(uint32_t x0, uint32_t x1, float& z0, float& z1) {
+    // the normals are rounded products: no contraction into a caller's add (the BF16 split
+    // z - hi would otherwise become fma(r, cos, -hi) in some instantiations and not in others)
+#pragma clang fp contract(off)
     const float u0 = fmaf(float(x0), 2.3283064365386963e-10f, 1.1641532182693481e-10f);
     const float u1 = fmaf(float(x1), 2.3283064365386963e-10f, 1.1641532182693481e-10f);
     // ln u0 = log2(u0) * ln 2

@@ -3,6 +3,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "rb_fused.h"
 #include "rb_kernels.h"
 #include "rb_mfma.h"
@@ -55,6 +57,13 @@ __device__ __forceinline__ float4 load_global_f4(const float* p) {
     typedef float f4 __attribute__((ext_vector_type(4)));
     const f4 v = *(const __attribute__((address_space(1))) f4*)(p);
     return make_float4(v.x, v.y, v.z, v.w);
+}
+
+// 16 bytes from global memory (see load_global_f)
+__device__ __forceinline__ uint4 load_global_u4(const uint4* p) {
+    typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+    const u4 v = *(const __attribute__((address_space(1))) u4*)(p);
+    return make_uint4(v.x, v.y, v.z, v.w);
 }
 
 // a float from global memory (a generic pointer would give a FLAT load, after which the compiler
@@ -201,18 +210,18 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
         const long t_end = t_begin + A.tiles_per_item < A.ntiles ? t_begin + A.tiles_per_item : A.ntiles;
         for (long tile = t_begin; tile < t_end; ++tile) {
             const long p0 = tile * A.TP;
-            // ---- 1. Gaussians W1 -> B fragments (hi, lo); quads past N are zero.  The steps
-            //         past N inside the last quad and the paths past M keep their Gaussians:
-            //         steps >= N reach only output rows >= N, whose terms P = +inf zeroes, and
-            //         the results of paths past M are dropped
+            // ---- 1. Gaussians W1 -> B fragments (hi, lo).  Every quad of the padded steps is
+            //         generated, and the paths past M keep their Gaussians: steps >= N reach only
+            //         output rows >= N (the Toeplitz operand is zero past N - 1), whose terms
+            //         the row term +inf zeroes, and the results of paths past M are dropped.
+            //         Four quads per trip, so four Philox chains interleave.
             const int nq = A.Np >> 2;
             {
                 const int pl = tid & (A.TP - 1);   // TP is 32, 64 or 128
                 const int pb = pl >> 5;
                 const long p = p0 + pl;
-                for (int q = tid >> A.log2TP; q < nq; q += 256 >> A.log2TP) {
-                    float z[4] = {0.f, 0.f, 0.f, 0.f};
-                    if (4 * q < A.N) normals4_f32(A.key, uint64_t(p), 0u, uint32_t(q), z);
+                const int step = 256 >> A.log2TP;  // divides nq (a multiple of 8)
+                auto put = [&](int q, const float z[4]) {
                     const int ks = q >> 2, hh = (q & 3) >> 1, j0 = (q & 1);
                     const int ln = 32 * hh + (pl & 31);
                     uint4* base = Bf + ((pb * NK + ks) * 2) * 64;
@@ -235,6 +244,19 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
                         reinterpret_cast<uint2*>(base + ln)[j0] = __builtin_bit_cast(uint2, hi);
                         reinterpret_cast<uint2*>(base + 64 + ln)[j0] = __builtin_bit_cast(uint2, lo);
                     }
+                };
+                int q = tid >> A.log2TP;
+                for (; q + 3 * step < nq; q += 4 * step) {
+                    float z[4][4];
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) normals4_f32(A.key, uint64_t(p), 0u, uint32_t(q + v * step), z[v]);
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) put(q + v * step, z[v]);
+                }
+                for (; q < nq; q += step) {
+                    float z[4];
+                    normals4_f32(A.key, uint64_t(p), 0u, uint32_t(q), z);
+                    put(q, z);
                 }
             }
             __syncthreads();
@@ -249,44 +271,76 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
                     for (int r = 0; r < 16; ++r) acc[s][r] = 0.f;
                 if (pbw >= 0) {
                     const uint4* F = A.frag + U.frag_off;
-                    // phase k: row blocks 0..k-1 of the wave are active (blocks sorted descending)
+                    int rbs[UPW];   // the wave's row blocks, decreasing (wave-uniform)
 #pragma unroll
-                    for (int k = 1; k <= UPW; ++k) {
-                        if (k > nuw) break;
-                        const int dhi = 32 * A.wave_rb[wave][k - 1];
-                        const int dlo = k < nuw ? 32 * A.wave_rb[wave][k] + 16 : -16;
-                        for (int d = dhi; d >= dlo; d -= 16) {
-                            const uint4* Fd = F + ((d + 16) >> 4) * 128;
-                            const V ahi = __builtin_bit_cast(V, Fd[lane]);
-                            const V alo = __builtin_bit_cast(V, Fd[64 + lane]);
-                            V bhi[UPW], blo[UPW];
+                    for (int s = 0; s < UPW; ++s) rbs[s] = __builtin_amdgcn_readfirstlane(A.wave_rb[wave][s]);
+                    // d runs down from 32 rbs[0] to -16 in steps of 16 across the phases; every
+                    // phase has an even number of steps (2 (rbs[k-1] - rbs[k]), or 2 rbs[k-1] + 2
+                    // in the last), so the steps go in pairs d, d - 16 with the A fragments in
+                    // two register sets X and Y: a set is reloaded for d - 32 (d - 48) as soon as
+                    // its products are issued, and the load has the other set's products to
+                    // arrive (global loads: the wait before a product counts only the newer
+                    // ones).  Loads past the last d (-16) reload d = -16.
+                    auto fa = [&](int dd, int half) {
+                        const int dc = dd > -16 ? dd : -16;
+                        return __builtin_bit_cast(V, load_global_u4(F + ((dc + 16) >> 4) * 128 + 64 * half + lane));
+                    };
+                    auto step = [&](int dd, const V& ahi, const V& alo, auto kc) {
+                        constexpr int k = decltype(kc)::value;
+                        V bhi[UPW], blo[UPW];
 #pragma unroll
-                            for (int s = 0; s < k; ++s) {
-                                const int ks = (32 * A.wave_rb[wave][s] - d) >> 4;
-                                const uint4* Bp = Bf + ((pbw * NK + ks) * 2) * 64;
-                                bhi[s] = __builtin_bit_cast(V, Bp[lane]);
-                                blo[s] = __builtin_bit_cast(V, Bp[64 + lane]);
-                            }
-                            if constexpr (MODE == 2) {
-                                // k-slots 0..3 from the first halves, 4..7 from the second
-#pragma unroll
-                                for (int j = 0; j < 4; ++j)
-#pragma unroll
-                                    for (int s = 0; s < k; ++s) acc[s] = S::mma1(ahi[j], bhi[s][j], acc[s]);
-#pragma unroll
-                                for (int j = 0; j < 4; ++j)
-#pragma unroll
-                                    for (int s = 0; s < k; ++s) acc[s] = S::mma1(alo[j], blo[s][j], acc[s]);
-                            } else {
-#pragma unroll
-                                for (int s = 0; s < k; ++s) acc[s] = S::mma(ahi, bhi[s], acc[s]);
-#pragma unroll
-                                for (int s = 0; s < k; ++s) acc[s] = S::mma(ahi, blo[s], acc[s]);
-#pragma unroll
-                                for (int s = 0; s < k; ++s) acc[s] = S::mma(alo, bhi[s], acc[s]);
-                            }
+                        for (int s = 0; s < k; ++s) {
+                            const int ks = (32 * rbs[s] - dd) >> 4;
+                            const uint4* Bp = Bf + ((pbw * NK + ks) * 2) * 64;
+                            bhi[s] = __builtin_bit_cast(V, Bp[lane]);
+                            blo[s] = __builtin_bit_cast(V, Bp[64 + lane]);
                         }
-                    }
+                        if constexpr (MODE == 2) {
+                            // k-slots 0..3 from the first halves, 4..7 from the second
+#pragma unroll
+                            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                                for (int s = 0; s < k; ++s) acc[s] = S::mma1(ahi[j], bhi[s][j], acc[s]);
+#pragma unroll
+                            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                                for (int s = 0; s < k; ++s) acc[s] = S::mma1(alo[j], blo[s][j], acc[s]);
+                        } else {
+#pragma unroll
+                            for (int s = 0; s < k; ++s) acc[s] = S::mma(ahi, bhi[s], acc[s]);
+#pragma unroll
+                            for (int s = 0; s < k; ++s) acc[s] = S::mma(ahi, blo[s], acc[s]);
+#pragma unroll
+                            for (int s = 0; s < k; ++s) acc[s] = S::mma(alo, bhi[s], acc[s]);
+                        }
+                    };
+                    int d = 32 * rbs[0];
+                    V xhi = fa(d, 0), xlo = fa(d, 1), yhi = fa(d - 16, 0), ylo = fa(d - 16, 1);
+                    // phase k: row blocks 0..k-1 of the wave are active (blocks sorted descending)
+                    auto phase = [&](auto kc) {
+                        constexpr int k = decltype(kc)::value;
+                        const int dlo = k < nuw ? 32 * rbs[k < UPW ? k : 0] + 16 : -16;
+                        for (; d >= dlo; d -= 32) {
+                            // the scheduling barriers keep each reload right after its set's
+                            // products instead of sinking it to the end of the pair
+                            step(d, xhi, xlo, kc);
+                            xhi = fa(d - 32, 0);
+                            xlo = fa(d - 32, 1);
+                            __builtin_amdgcn_sched_barrier(0);
+                            step(d - 16, yhi, ylo, kc);
+                            yhi = fa(d - 48, 0);
+                            ylo = fa(d - 48, 1);
+                            __builtin_amdgcn_sched_barrier(0);
+                        }
+                    };
+                    phase(std::integral_constant<int, 1>());
+                    if constexpr (UPW >= 2) if (nuw >= 2) phase(std::integral_constant<int, 2>());
+                    if constexpr (UPW >= 3) if (nuw >= 3) phase(std::integral_constant<int, 3>());
+                    if constexpr (UPW >= 4) if (nuw >= 4) phase(std::integral_constant<int, 4>());
+                    if constexpr (UPW >= 5) if (nuw >= 5) phase(std::integral_constant<int, 5>());
+                    if constexpr (UPW >= 6) if (nuw >= 6) phase(std::integral_constant<int, 6>());
+                    if constexpr (UPW >= 7) if (nuw >= 7) phase(std::integral_constant<int, 7>());
+                    if constexpr (UPW >= 8) if (nuw >= 8) phase(std::integral_constant<int, 8>());
                     // ---- 3a. Wtilde = sc (Y' + b W1perp), W1perp from the same counters (stream 1)
                     const long p = p0 + 32 * pbw + col;
 #pragma unroll
