@@ -276,7 +276,7 @@ def bench_batched(rb, args):
                      "speedup_vs_fp64_single_request": fp64_s / (ms * 1e-3)}
     out["value"] = out["fp16"]["paths_steps_per_s"]
     out["ms_per_step"] = out["fp16"]["ms_per_batch"]
-    out["dtype"] = "f16x2+f32"
+    out["dtype"] = "f16+f32"
     out["steps"], out["warmup"] = args.steps, 1
     return out
 
@@ -346,7 +346,7 @@ def serving_line(rb, args):
     assert abs((r1["path_steps"] - r0["path_steps"]) - work * args.steps) <= 1e-9 * work * args.steps
     out = {"metric": "paths*steps/s", "value": work / s_step, "unit": "paths*steps/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": s_step * 1e3, "higher_is_better": True,
-           "scaling": "weak", "vs_baseline": None, "dtype": "f16x2+f32",
+           "scaling": "weak", "vs_baseline": None, "dtype": "f16+f32",
            "data": "synthetic (seeded parameter trace, device Philox4x32-10 Gaussians)",
            "config": {"workload": "serving trace: %d calibration requests (10-200 options over 1-5 maturities, "
                                   "M = 2^14..2^21, N = max(32, 256 T)), continuous batching, closed loop"
@@ -428,7 +428,7 @@ def bench_serving(rb, args):
     res = run_serving(rb, reqs, rb.PREC_SPLIT_FP16)
     out = {"metric": "paths*steps/s", "value": res["open_loop_paths_steps_per_s"], "unit": "paths*steps/s",
            "n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-           "dtype": "f16x2+f32", "data": "synthetic (device Philox4x32-10 Gaussians)",
+           "dtype": "f16+f32", "data": "synthetic (device Philox4x32-10 Gaussians)",
            "config": {"workload": "serving: %d calibration requests, Poisson arrivals at 0.7 x capacity, "
                                   "10-200 options over 1-5 maturities, M = 2^14..2^21, N = max(32, 256 T)"
                                   % args.requests, "parallelism": "1 GPU"},
@@ -466,7 +466,7 @@ def bench_config1(rb, args):
         else:
             rows[name]["max_abs_err_vs_fp64"] = float(np.max(np.abs(p - p64)))
     out = {"metric": "paths*steps/s", "unit": "paths*steps/s", "n_gpus": 1, "higher_is_better": True,
-           "value": rows["fp16"]["paths_steps_per_s"], "dtype": "f16x2+f32", "vs_baseline": None,
+           "value": rows["fp16"]["paths_steps_per_s"], "dtype": "f16+f32", "vs_baseline": None,
            "scaling": "weak", "data": "synthetic (device Philox4x32-10 Gaussians)",
            "config": {"workload": "config 1: N=64, M=2^16, 21 strikes, one parameter set"}, "rows": rows}
     if not args.no_cpu:
@@ -591,7 +591,7 @@ def single_line(rb, args, sub=False):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp64": "f64", "fp32": "f32", "fp16": "f16x2+f32", "bf16": "bf16x2+f32",
+        "dtype": {"fp64": "f64", "fp32": "f32", "fp16": "f16+f32", "bf16": "bf16x2+f32",
                   "fp32gemm": "f32", "fp32fft": "f32"}[args.precision],
         "data": "synthetic (device Philox4x32-10 Gaussians)",
         "config": {"workload": "single parameter set at scale: H=0.1 eta=1.5 rho=-0.7 xi=0.04 "

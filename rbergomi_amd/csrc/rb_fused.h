@@ -196,7 +196,8 @@ __device__ __forceinline__ double payoff_mc(const PathPG& q, double K) {
 
 size_t fused_req_lds_bytes(const FusedReq& a);
 size_t fused_lds_bytes(const FusedBatchArgs& b);
-// mode: 0 = FP16 hi+lo split, 1 = BF16 hi+lo split, 2 = FP32 operands
+// mode: 0 = FP16 (Gamma split hi + lo, W1 rounded to FP16), 1 = BF16 hi + lo split of both, 2 = FP32
+// operands
 // mc: plain Monte Carlo kernel (every request of the batch), else Romano-Touzi; greg groups per
 // pass: 1 to 6, or 8
 hipError_t launch_fused(int mode, int upw, bool mc, int greg, const FusedBatchArgs& b, int grid,

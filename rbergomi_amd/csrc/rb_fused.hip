@@ -2,13 +2,15 @@
 //
 // One workgroup (4 waves) owns a tile of TP = 32*PB paths at a time and runs the whole path
 // of the reference for it without touching HBM in between:
-//   1. Philox4x32-10 Gaussians W1 (FP32 Box-Muller) -> LDS, split hi + lo (FP16 or BF16) and
-//      stored directly in the B-operand fragment layout of v_mfma_f32_32x32x16_{f16,bf16}, or
-//      kept in FP32 for v_mfma_f32_32x32x2_f32 (replaces genGaussianMT,
-//      rBergLib/rBergomiMT.cpp:169-172)
+//   1. Philox4x32-10 Gaussians W1 (FP32 Box-Muller) -> LDS in the B-operand fragment layout of
+//      v_mfma_f32_32x32x{16_f16,16_bf16,2_f32}: FP16 mode rounds W1 to FP16 (the value every
+//      later stage uses); BF16 mode splits it into hi + lo; FP32 mode keeps it (replaces
+//      genGaussianMT, rBergLib/rBergomiMT.cpp:169-172)
 //   2. Y' = L W1 on the matrix cores, L the lower-triangular Toeplitz matrix of Gamma with the
-//      first-cell weight on its diagonal, as three products hi*hi + hi*lo + lo*hi accumulated
-//      in FP32, or one FP32 product (replaces the FFTW convolution of updateWtilde, :83-109).
+//      first-cell weight on its diagonal, L split into hi + lo: L_hi W1 + L_lo W1 (FP16: two
+//      products, exact up to the FP32 accumulation), L_hi W1_hi + L_hi W1_lo + L_lo W1_hi
+//      (BF16: three), or one FP32 product (replaces the FFTW convolution of updateWtilde,
+//      :83-109).
 //      The A operand of a
 //      32x16 block depends only on its diagonal offset d, so A fragments come from a small
 //      per-H table (no N x N matrix).  A wave walks d in phases over which the set of its

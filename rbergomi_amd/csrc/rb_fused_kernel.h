@@ -85,67 +85,43 @@ __device__ __forceinline__ void load_rows16(const float* p, float v[16]) {
     }
 }
 
+// The W1 tile in LDS, per path block and 16-step chunk: 64 lane slots of 16 bytes as the MFMA B
+// operand (slot 32 x + col holds steps 16 ks + 8 x + 0..7 of path column col), then 64 more.
+// FP16 mode: W1 rounded to FP16 in the first 64 slots (the second 64 unused).  BF16 mode: hi in
+// the first 64, lo = RN(W1 - hi) in the second.  FP32 mode: steps 8 x + 0..3 in the first 64,
+// 8 x + 4..7 in the second.
+
 // W1 of path column `col`, step 16 ks + kk, from the LDS tile (the value the product used)
 template <int MODE>
 __device__ __forceinline__ float tile_w1(const uint4* chunk, int kk, int col) {
     if constexpr (MODE == 2) {
         return reinterpret_cast<const float*>(chunk + 64 * ((kk & 7) >> 2) + 32 * (kk >> 3) + col)[kk & 3];
+    } else if constexpr (MODE == 0) {
+        return float(reinterpret_cast<const _Float16*>(chunk + 32 * (kk >> 3) + col)[kk & 7]);
     } else {
-        typedef typename SplitT<MODE>::T T;
         const uint4* Bp = chunk + 32 * (kk >> 3) + col;
-        return float(reinterpret_cast<const T*>(Bp)[kk & 7]) + float(reinterpret_cast<const T*>(Bp + 64)[kk & 7]);
+        return float(reinterpret_cast<const __bf16*>(Bp)[kk & 7]) + float(reinterpret_cast<const __bf16*>(Bp + 64)[kk & 7]);
     }
 }
 
-// W1 of path column `col`, steps 4 Q .. 4 Q + 3, from the LDS tile of path block `blk` (the
-// values the product used; tile_w1 of the same steps): one 8-byte read of hi and one of lo, or
-// one 16-byte read of FP32 operands.  Quad Q sits in chunk Q / 4, lane group (Q mod 4) / 2, half
-// Q mod 2 of the lane's 16 bytes.
-template <int MODE>
-__device__ __forceinline__ void tile_w1_quad(const uint4* Bf, int blk, int NK, int Q, int col, float w[4]) {
+// FP32 operands of path column `col`, steps 4 Q .. 4 Q + 3, from the LDS tile of path block
+// `blk` (tile_w1 of the same steps): one 16-byte read.  Quad Q sits in chunk Q / 4, lane group
+// (Q mod 4) / 2, half Q mod 2.
+__device__ __forceinline__ void tile_w1_quad_f32(const uint4* Bf, int blk, int NK, int Q, int col, float w[4]) {
     const uint4* chunk = Bf + ((blk * NK + (Q >> 2)) * 2) * 64 + 32 * ((Q & 3) >> 1) + col;
-    if constexpr (MODE == 2) {
-        const float4 v = __builtin_bit_cast(float4, chunk[64 * (Q & 1)]);
-        w[0] = v.x;
-        w[1] = v.y;
-        w[2] = v.z;
-        w[3] = v.w;
-    } else {
-        typedef typename SplitT<MODE>::T T;
-        typedef T T4 __attribute__((ext_vector_type(4)));
-        const T4 hi = __builtin_bit_cast(T4, reinterpret_cast<const uint2*>(chunk)[Q & 1]);
-        const T4 lo = __builtin_bit_cast(T4, reinterpret_cast<const uint2*>(chunk + 64)[Q & 1]);
-#pragma unroll
-        for (int t = 0; t < 4; ++t) w[t] = float(hi[t]) + float(lo[t]);
-    }
+    const float4 v = __builtin_bit_cast(float4, chunk[64 * (Q & 1)]);
+    w[0] = v.x;
+    w[1] = v.y;
+    w[2] = v.z;
+    w[3] = v.w;
 }
 
-// FP16 split of two floats, packed in pairs: hi = RN16(x), lo = RN16(x - hi).  x - hi is exact in
-// FP32, and v_fma_mixlo/mixhi_f16 form it from the FP16 hi (widened exactly) and round it once to
-// FP16 into the low or high half of lo: the same bits as converting x - float(hi), without
-// widening hi back to FP32 and packing the halves.
-__device__ __forceinline__ void split2_f16(float x0, float x1, uint32_t& hi, uint32_t& lo) {
-    typedef _Float16 h2 __attribute__((ext_vector_type(2)));
-    const h2 h = {(_Float16)x0, (_Float16)x1};
-    hi = __builtin_bit_cast(uint32_t, h);
-    uint32_t l;
-    asm("v_fma_mixlo_f16 %0, %1, -1.0, %2 op_sel_hi:[1,0,0]" : "=v"(l) : "v"(hi), "v"(x0));
-    asm("v_fma_mixhi_f16 %0, %1, -1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "+v"(l) : "v"(hi), "v"(x1));
-    lo = l;
-}
-
-// hi + lo of one split W1 element: UPPER picks the upper 16 bits of both dwords, else the lower.
-// FP16: one v_fma_mix_f32 (hi x 1.0 + lo with both operands widened exactly, rounded once), the
-// same bits as float(hi) + float(lo) without the two conversions.  BF16: the widening is a shift.
+// one W1 element of the 16-bit modes from the dwords hi (and lo) of the tile: UPPER picks the
+// upper 16 bits.  FP16: the FP16 value (lo unused).  BF16: hi + lo, the widening a shift.
 template <int MODE, bool UPPER>
-__device__ __forceinline__ float split_sum(uint32_t hi, uint32_t lo) {
+__device__ __forceinline__ float tile_elem(uint32_t hi, uint32_t lo) {
     if constexpr (MODE == 0) {
-        float r;
-        if constexpr (UPPER)
-            asm("v_fma_mix_f32 %0, %1, 1.0, %2 op_sel:[1,0,1] op_sel_hi:[1,0,1]" : "=v"(r) : "v"(hi), "v"(lo));
-        else
-            asm("v_fma_mix_f32 %0, %1, 1.0, %2 op_sel_hi:[1,0,1]" : "=v"(r) : "v"(hi), "v"(lo));
-        return r;
+        return float(__builtin_bit_cast(_Float16, uint16_t(UPPER ? hi >> 16 : hi & 0xffffu)));
     } else {
         return UPPER ? __uint_as_float(hi & 0xffff0000u) + __uint_as_float(lo & 0xffff0000u)
                      : __uint_as_float(hi << 16) + __uint_as_float(lo << 16);
@@ -210,11 +186,11 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
         const long t_end = t_begin + A.tiles_per_item < A.ntiles ? t_begin + A.tiles_per_item : A.ntiles;
         for (long tile = t_begin; tile < t_end; ++tile) {
             const long p0 = tile * A.TP;
-            // ---- 1. Gaussians W1 -> B fragments (hi, lo).  Every quad of the padded steps is
-            //         generated, and the paths past M keep their Gaussians: steps >= N reach only
-            //         output rows >= N (the Toeplitz operand is zero past N - 1), whose terms
-            //         the row term +inf zeroes, and the results of paths past M are dropped.
-            //         Four quads per trip, so four Philox chains interleave.
+            // ---- 1. Gaussians W1 -> B fragments (FP16, BF16 hi + lo, or FP32).  Every quad of
+            //         the padded steps is generated, and the paths past M keep their Gaussians:
+            //         steps >= N reach only output rows >= N (the Toeplitz operand is zero past
+            //         N - 1), whose terms the row term +inf zeroes, and the results of paths past
+            //         M are dropped.  Four quads per trip, so four Philox chains interleave.
             const int nq = A.Np >> 2;
             {
                 const int pl = tid & (A.TP - 1);   // TP is 32, 64 or 128
@@ -229,11 +205,9 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
                         const V4 v = {z[0], z[1], z[2], z[3]};
                         base[64 * j0 + ln] = __builtin_bit_cast(uint4, v);
                     } else if constexpr (MODE == 0) {
-                        uint2 hi, lo;
-                        split2_f16(z[0], z[1], hi.x, lo.x);
-                        split2_f16(z[2], z[3], hi.y, lo.y);
-                        reinterpret_cast<uint2*>(base + ln)[j0] = hi;
-                        reinterpret_cast<uint2*>(base + 64 + ln)[j0] = lo;
+                        typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+                        const h4 hv = {(_Float16)z[0], (_Float16)z[1], (_Float16)z[2], (_Float16)z[3]};
+                        reinterpret_cast<uint2*>(base + ln)[j0] = __builtin_bit_cast(uint2, hv);
                     } else {
                         V4 hi, lo;
 #pragma unroll
@@ -293,7 +267,7 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
                             const int ks = (32 * rbs[s] - dd) >> 4;
                             const uint4* Bp = Bf + ((pbw * NK + ks) * 2) * 64;
                             bhi[s] = __builtin_bit_cast(V, Bp[lane]);
-                            blo[s] = __builtin_bit_cast(V, Bp[64 + lane]);
+                            if constexpr (MODE != 0) blo[s] = __builtin_bit_cast(V, Bp[64 + lane]);
                         }
                         if constexpr (MODE == 2) {
                             // k-slots 0..3 from the first halves, 4..7 from the second
@@ -306,10 +280,13 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
 #pragma unroll
                                 for (int s = 0; s < k; ++s) acc[s] = S::mma1(alo[j], blo[s][j], acc[s]);
                         } else {
+                            // A hi W1 (+ A hi W1lo, BF16) + A lo W1
 #pragma unroll
                             for (int s = 0; s < k; ++s) acc[s] = S::mma(ahi, bhi[s], acc[s]);
+                            if constexpr (MODE == 1) {
 #pragma unroll
-                            for (int s = 0; s < k; ++s) acc[s] = S::mma(ahi, blo[s], acc[s]);
+                                for (int s = 0; s < k; ++s) acc[s] = S::mma(ahi, blo[s], acc[s]);
+                            }
 #pragma unroll
                             for (int s = 0; s < k; ++s) acc[s] = S::mma(alo, bhi[s], acc[s]);
                         }
@@ -418,17 +395,17 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
                                 for (int c = 0; c < 4; ++c) {
                                     const int Q = 8 * rb + 2 * c + h;
                                     float qa[4], qb[4];
-                                    tile_w1_quad<MODE>(Bf, pbw, NK, Q, col, qa);
-                                    tile_w1_quad<MODE>(Bf, pbw, NK, Q + 1 < 4 * NK ? Q + 1 : Q, col, qb);
+                                    tile_w1_quad_f32(Bf, pbw, NK, Q, col, qa);
+                                    tile_w1_quad_f32(Bf, pbw, NK, Q + 1 < 4 * NK ? Q + 1 : Q, col, qb);
                                     wv[4 * c + 0] = qa[1];
                                     wv[4 * c + 1] = qa[2];
                                     wv[4 * c + 2] = qa[3];
                                     wv[4 * c + 3] = qb[0];
                                 }
                             } else {
-                                // split modes, byte offsets from slot col of chunk 2 rb: quad Q is
-                                // the uint2 h of slot 32 (c & 1) + col in chunk 2 rb + c / 2 (lo
-                                // 1024 B further).  Element 0 of quad Q + 1 begins the next uint2
+                                // 16-bit modes, byte offsets from slot col of chunk 2 rb: quad Q is
+                                // the uint2 h of slot 32 (c & 1) + col in chunk 2 rb + c / 2 (BF16
+                                // lo 1024 B further).  Element 0 of quad Q + 1 begins the next uint2
                                 // (h = 0), slot 32 + col (h = 1, c even) or slot col of the next
                                 // chunk (h = 1, c odd); for c = 3 in the last row block that is
                                 // past the tile, and the row (j = Np) has P = +inf, so the read
@@ -439,15 +416,19 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
 #pragma unroll
                                 for (int c = 0; c < 4; ++c) {
                                     const char* slot = Wr + (c >> 1) * 2048 + (c & 1) * 512;
-                                    const uint2 ha = *reinterpret_cast<const uint2*>(slot + 8 * h);
-                                    const uint2 la = *reinterpret_cast<const uint2*>(slot + 8 * h + 1024);
                                     const int nb = c == 3 ? nb3 : (c & 1) ? nb_odd : nb_even;
+                                    const uint2 ha = *reinterpret_cast<const uint2*>(slot + 8 * h);
                                     const uint32_t hb = *reinterpret_cast<const uint32_t*>(slot + nb);
-                                    const uint32_t lb = *reinterpret_cast<const uint32_t*>(slot + nb + 1024);
-                                    wv[4 * c + 0] = split_sum<MODE, true>(ha.x, la.x);
-                                    wv[4 * c + 1] = split_sum<MODE, false>(ha.y, la.y);
-                                    wv[4 * c + 2] = split_sum<MODE, true>(ha.y, la.y);
-                                    wv[4 * c + 3] = split_sum<MODE, false>(hb, lb);
+                                    uint2 la = make_uint2(0u, 0u);
+                                    uint32_t lb = 0u;
+                                    if constexpr (MODE == 1) {
+                                        la = *reinterpret_cast<const uint2*>(slot + 8 * h + 1024);
+                                        lb = *reinterpret_cast<const uint32_t*>(slot + nb + 1024);
+                                    }
+                                    wv[4 * c + 0] = tile_elem<MODE, true>(ha.x, la.x);
+                                    wv[4 * c + 1] = tile_elem<MODE, false>(ha.y, la.y);
+                                    wv[4 * c + 2] = tile_elem<MODE, true>(ha.y, la.y);
+                                    wv[4 * c + 3] = tile_elem<MODE, false>(hb, lb);
                                 }
                             }
                             // rows j = 32 rb + row + 1 >= N have the row term +inf (P = +inf,
