@@ -94,9 +94,12 @@ struct FusedReq {
     int log2TP;                  // log2(TP)
     int nunits, pgmax, out_size, gchunk, acc_in_lds;
     int log2slices;              // lanes per option in the payoff stage (register mode)
-    int tiles_per_item;
+    int tiles_per_item;          // tiles of items 0 .. items_big - 1
     int mc;                      // 1 = plain Monte Carlo (wpart holds 3 sums per entry), 0 = RT
     long M, ntiles, nitems;
+    long items_big;              // items of tiles_per_item tiles; the rest have tiles_small
+    int tiles_small;
+    int pad_;
     PhiloxKey key;
     const FusedUnit* units;
     const FusedGroup* groups;

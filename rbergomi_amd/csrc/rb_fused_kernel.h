@@ -182,8 +182,11 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
         double a1 = 0.0, a2 = 0.0;  // sums of option tid (register accumulation)
         if (A.acc_in_lds)
             for (int o = tid; o < 2 * A.out_size; o += 256) accopt[o] = 0.0;
-        const long t_begin = item * A.tiles_per_item;
-        const long t_end = t_begin + A.tiles_per_item < A.ntiles ? t_begin + A.tiles_per_item : A.ntiles;
+        const bool big = item < A.items_big;
+        const long t_begin = big ? item * A.tiles_per_item
+                                 : A.items_big * A.tiles_per_item + (item - A.items_big) * A.tiles_small;
+        const long t_len = big ? A.tiles_per_item : A.tiles_small;
+        const long t_end = t_begin + t_len < A.ntiles ? t_begin + t_len : A.ntiles;
         for (long tile = t_begin; tile < t_end; ++tile) {
             const long p0 = tile * A.TP;
             // ---- 1. Gaussians W1 -> B fragments (FP16, BF16 hi + lo, or FP32).  Every quad of
