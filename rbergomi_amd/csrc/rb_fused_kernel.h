@@ -72,6 +72,45 @@ __device__ __forceinline__ float load_global_f(const float* p) {
     return *(const __attribute__((address_space(1))) float*)(p);
 }
 
+// a wave-uniform pointer into the constant address space: the request's read-only tables read at
+// wave-uniform indices then go through scalar loads (the scalar cache, no vector registers, no
+// wait on the vector memory counter)
+template <class T>
+__device__ __forceinline__ const __attribute__((address_space(4))) T* uniform_const(const T* p) {
+    // readfirstlane returns int: each half goes back through uint32_t before widening, or a low
+    // half with bit 31 set would sign-extend into the high half
+    const uint64_t a = reinterpret_cast<uint64_t>(p);
+    const uint32_t lo = uint32_t(__builtin_amdgcn_readfirstlane(int(uint32_t(a))));
+    const uint32_t hi = uint32_t(__builtin_amdgcn_readfirstlane(int(uint32_t(a >> 32))));
+    return (const __attribute__((address_space(4))) T*)((uint64_t(hi) << 32) | uint64_t(lo));
+}
+
+// a record of read-only data at a wave-uniform address, through scalar loads
+template <class T>
+__device__ __forceinline__ T load_uniform(const T* p) {
+    static_assert(sizeof(T) % 4 == 0, "dword record");
+    const auto q = uniform_const(reinterpret_cast<const uint32_t*>(p));
+    uint32_t w[sizeof(T) / 4];
+#pragma unroll
+    for (int i = 0; i < int(sizeof(T) / 4); ++i) w[i] = q[i];
+    T v;
+    __builtin_memcpy(&v, w, sizeof(T));
+    return v;
+}
+
+// a record of read-only data through global (not FLAT) vector loads, any address
+template <class T>
+__device__ __forceinline__ T load_global_rec(const T* p) {
+    static_assert(sizeof(T) % 4 == 0, "dword record");
+    const __attribute__((address_space(1))) uint32_t* q = (const __attribute__((address_space(1))) uint32_t*)(p);
+    uint32_t w[sizeof(T) / 4];
+#pragma unroll
+    for (int i = 0; i < int(sizeof(T) / 4); ++i) w[i] = q[i];
+    T v;
+    __builtin_memcpy(&v, w, sizeof(T));
+    return v;
+}
+
 // 16 floats of a row table at p, p + 8, p + 16, p + 24 (four aligned float4: a lane's rows
 // 8 c + 0..3, c = 0..3)
 __device__ __forceinline__ void load_rows16(const float* p, float v[16]) {
@@ -239,7 +278,7 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
             __syncthreads();
 
             for (int u = 0; u < A.nunits; ++u) {
-                const FusedUnit U = A.units[u];
+                const FusedUnit U = load_uniform(A.units + u);
                 // ---- 2. Toeplitz GEMM on the matrix cores
                 floatx16_t acc[UPW];
 #pragma unroll
@@ -257,7 +296,7 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
                     // two register sets X and Y: a set is reloaded for d - 32 (d - 48) as soon as
                     // its products are issued, and the load has the other set's products to
                     // arrive (global loads: the wait before a product counts only the newer
-                    // ones).  Loads past the last d (-16) reload d = -16.
+                    // ones).
                     auto fa = [&](int dd, int half) {
                         const int dc = dd > -16 ? dd : -16;
                         return __builtin_bit_cast(V, load_global_u4(F + ((dc + 16) >> 4) * 128 + 64 * half + lane));
@@ -299,7 +338,10 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
                     // phase k: row blocks 0..k-1 of the wave are active (blocks sorted descending)
                     auto phase = [&](auto kc) {
                         constexpr int k = decltype(kc)::value;
-                        const int dlo = k < nuw ? 32 * rbs[k < UPW ? k : 0] + 16 : -16;
+                        // the last phase ends with the pair (0, -16), which reloads nothing (a
+                        // load still in flight past the products would hold up the next stage)
+                        const bool last = !(k < nuw);
+                        const int dlo = last ? 16 : 32 * rbs[k < UPW ? k : 0] + 16;
                         for (; d >= dlo; d -= 32) {
                             // the scheduling barriers keep each reload right after its set's
                             // products instead of sinking it to the end of the pair
@@ -311,6 +353,10 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
                             yhi = fa(d - 48, 0);
                             ylo = fa(d - 48, 1);
                             __builtin_amdgcn_sched_barrier(0);
+                        }
+                        if (last) {
+                            step(0, xhi, xlo, kc);
+                            step(-16, yhi, ylo, kc);
                         }
                     };
                     phase(std::integral_constant<int, 1>());
@@ -351,7 +397,7 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
                     for (int gg = 0; gg < GR; ++gg) {
                         sv[gg] = ss[gg] = sp[gg] = 0.0;
                         fv[gg] = fs[gg] = fp[gg] = 0.f;
-                        c1[gg] = gg < gn ? load_global_f(&A.groups[g0 + gg].c1) : 0.f;
+                        c1[gg] = gg < gn ? uniform_const(A.groups)[g0 + gg].c1 : 0.f;
                     }
                     if (pbw >= 0) {
                         // row terms al P_j + be: group g0 + gg at gg Np + j - 1
@@ -483,7 +529,7 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
                         if constexpr (MC) sp[gg] += __shfl_xor(sp[gg], 32, 64);
                     }
                     // cross-wave sums in chunks of gchunk groups (LDS budget)
-                    const int pg_lo = A.groups[g0].pg_begin;
+                    const int pg_lo = uniform_const(A.groups)[g0].pg_begin;
                     for (int c0 = 0; c0 < gn; c0 += A.gchunk) {
                         const int cn = gn - c0 < A.gchunk ? gn - c0 : A.gchunk;
                         if (h == 0) {
@@ -505,7 +551,7 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
 #pragma clang fp contract(off)
                             const int pl = e & (A.TP - 1), gg = e >> A.log2TP;
                             const int pb = pl >> 5, c = pl & 31;
-                            const FusedGroup G = A.groups[g0 + c0 + gg];
+                            const FusedGroup G = load_global_rec(A.groups + g0 + c0 + gg);
                             const double w10 = double(tile_w1<MODE>(Bf + ((pb * NK) * 2) * 64, 0, c));  // k = 0
                             double a = G.xi, b = G.sxi * w10;   // j = 0: v_0 = xi
                             double cp = 0.0;
@@ -525,7 +571,7 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
                                 // ln S_T = rho Is + sqrt(1 - rho^2) Ip - Iv / 2 (log-Euler, left point)
                                 const double Ip = G.sdt * cp;
                                 for (int pg = G.pg_begin; pg < G.pg_end; ++pg) {
-                                    const double rho = A.pgs[pg].rho;
+                                    const double rho = load_global_rec(A.pgs + pg).rho;
                                     PathPG q;
                                     q.lnS = rho * Is + sqrt(1.0 - rho * rho) * Ip - 0.5 * Iv;
                                     q.sig = exp(q.lnS);
@@ -536,7 +582,7 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
                                 continue;
                             }
                             for (int pg = G.pg_begin; pg < G.pg_end; ++pg) {
-                                const double rho = A.pgs[pg].rho;
+                                const double rho = load_global_rec(A.pgs + pg).rho;
                                 PathPG q;
                                 q.sig = sqrt((1.0 - rho * rho) * Iv);
                                 q.lnS = -0.5 * rho * rho * Iv + rho * Is;
@@ -551,11 +597,11 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
                     //         summing every S-th path; a fixed butterfly leaves the total in lane
                     //         o*S, which owns option o's FP64 sums.  LDS mode (> 256 options):
                     //         one thread per option, sums in LDS.
-                    const int o_lo = A.groups[g0].o_begin, o_hi = A.groups[g0 + gn - 1].o_end;
+                    const int o_lo = uniform_const(A.groups)[g0].o_begin, o_hi = uniform_const(A.groups)[g0 + gn - 1].o_end;
                     const int np = A.M - p0 < A.TP ? int(A.M - p0) : A.TP;
                     if (A.acc_in_lds) {
                         for (int oi = o_lo + tid; oi < o_hi; oi += 256) {
-                            const FusedOption O = A.opts[oi];
+                            const FusedOption O = load_global_rec(A.opts + oi);
                             const int pgl = O.pg - pg_lo;
                             const OptF of = opt_f32(O.lnK, O.K);
                             double s1 = 0.0, s2 = 0.0;
@@ -572,7 +618,7 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
                         const int S = 1 << A.log2slices;
                         const int oi = tid >> A.log2slices, sl = tid & (S - 1);
                         if (oi >= o_lo && oi < o_hi) {
-                            const FusedOption O = A.opts[oi];
+                            const FusedOption O = load_global_rec(A.opts + oi);
                             const int pgl = O.pg - pg_lo;
                             const OptF of = opt_f32(O.lnK, O.K);
                             double s1 = 0.0, s2 = 0.0;
