@@ -19,6 +19,7 @@ a bounded sample of the same trace on the host cores, with all the threads the h
 and with one thread.
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -41,6 +42,34 @@ def workload(paths):
     T = np.repeat(MATURITIES, len(LOG_MONEYNESS))
     K = np.exp(np.tile(LOG_MONEYNESS, len(MATURITIES)))
     return dict(xi=0.04, H=[0.1], eta=[1.5], rho=[-0.7], T=T, K=K, N=512, M=paths)
+
+
+def valu_roofline(rb, prof, avg_ms):
+    """The vector-issue side of the fused kernel's roofline.  The launch's instructions by class
+    (rocprofv3 --pmc counts of one launch, committed in profiles/traffic.json) priced at the issue
+    costs this device shows for each class (rb_probe_issue, measured now): FP32 FMA-class, FP32
+    transcendental, 64-bit integer (priced as the 32 x 32 -> 64-bit multiply-add, the Philox
+    product), FP64.  Matrix instructions run on the matrix cores and are left out.  The sum over
+    the device's SIMDs, as a fraction of the launch, is the share of the launch that one SIMD needs
+    just to issue its vector instructions."""
+    ns = (ctypes.c_double * 4)()
+    simds = ctypes.c_int()
+    assert rb.load().rb_probe_issue(ns, ctypes.byref(simds)) == 0
+    fma, trans, madpair, f64 = list(ns)
+    mad = max(madpair - fma, fma)          # the probe's pair is the multiply-add and a 32-bit add
+    v = prof["valu_insts_per_launch"]
+    tr = prof["trans_f32_insts_per_launch"]
+    i64 = prof.get("int64_insts_per_launch", 0.0)
+    d64 = prof.get("f64_insts_per_launch", 0.0)
+    mf = prof.get("mfma_insts_per_launch", 0.0)
+    plain = v - tr - i64 - d64 - mf
+    busy_ns = (plain * fma + tr * trans + i64 * mad + d64 * f64) / simds.value
+    return {"insts_per_launch": v,
+            "by_class": {"fp32": plain, "trans_f32": tr, "int64": i64, "f64": d64, "mfma": mf},
+            "ns_per_inst": {"fp32": fma, "trans_f32": trans, "int64": mad, "f64": f64},
+            "simds": simds.value, "issue_busy_ms": busy_ns * 1e-6, "issue_busy_frac": busy_ns / (avg_ms * 1e6),
+            "source": "SQ_INSTS_VALU, _TRANS_F32, _INT64, _FMA/ADD/MUL/TRANS_F64, SQ_INSTS_MFMA (rocprofv3 --pmc, %s); "
+                      "issue costs from rb_probe_issue on this device" % prof.get("source")}
 
 
 def load_profile(kernel):
@@ -414,11 +443,8 @@ def serving_roofline(rb, args):
                             "batches": d["batches"],
                             "achieved_over_busy_TFLOPs": d["conv_flops"] / (d["device_busy_ms"] * 1e-3) / 1e12}}
     if prof.get("valu_insts_per_launch"):
-        v, tr = prof["valu_insts_per_launch"], prof["trans_f32_insts_per_launch"]
-        roof["valu"] = {"insts_per_launch": v, "trans_insts_per_launch": tr,
-                        "lane_insts_per_path_step": 64.0 * v * d["kernel_launches"] / work,
-                        "issue_busy_frac": (4.0 * v + 12.0 * tr) / 1024 / (avg_ms * 1e-3 * 2.4e9),
-                        "source": "SQ_INSTS_VALU, SQ_INSTS_VALU_TRANS_F32 (rocprofv3 --pmc), %s" % prof.get("source")}
+        roof["valu"] = valu_roofline(rb, prof, avg_ms)
+        roof["valu"]["lane_insts_per_path_step"] = 64.0 * prof["valu_insts_per_launch"] * d["kernel_launches"] / work
     return roof
 
 
@@ -573,13 +599,7 @@ def single_line(rb, args, sub=False):
                 "kernel": dominant, "avg_launch_ms": avg_ms, "launches_per_step": launches,
                 "flops_per_launch": flops_per_launch}
         if prof_counts.get("valu_insts_per_launch"):
-            # the resource that binds the fused kernel: VALU issue.  A wave64 VALU instruction
-            # holds a 16-lane SIMD for 4 cycles (a transcendental for 16); 1024 SIMDs at 2.4 GHz
-            v, tr = prof_counts["valu_insts_per_launch"], prof_counts["trans_f32_insts_per_launch"]
-            busy = (4.0 * v + 12.0 * tr) / 1024 / (avg_ms * 1e-3 * 2.4e9)
-            roof["valu"] = {"insts_per_launch": v, "trans_insts_per_launch": tr,
-                            "issue_busy_frac": busy,
-                            "source": "SQ_INSTS_VALU, SQ_INSTS_VALU_TRANS_F32 (rocprofv3 --pmc)"}
+            roof["valu"] = valu_roofline(rb, prof_counts, avg_ms)
     out = {
         "metric": "paths*steps/s",
         "value": M * N / (ms * 1e-3),
@@ -686,7 +706,6 @@ def main():
 
 
 def copy_probe(rb, mib):
-    import ctypes
     ms_copy = ctypes.c_double()
     nbytes = mib << 20
     assert rb.load().rb_probe_copy(nbytes, 5, ctypes.byref(ms_copy)) == 0

@@ -383,6 +383,13 @@ int rb_probe_mfma_layout(const float* A, const float* B, int mode, float* out);
 /* Device-to-device copy of `bytes` (multiple of 16) by a plain streaming kernel, reps times;
  * *avg_ms receives the mean kernel time (HIP events).  Used to calibrate HBM traffic counters. */
 int rb_probe_copy(long bytes, int reps, double* avg_ms);
+/* Issue cost of four instruction classes on the device: ns_per_inst[c] = nanoseconds per wave64
+ * instruction per SIMD with every SIMD issuing one class back to back (eight independent chains
+ * per lane, eight waves per SIMD); *simds (if not null) = the device's SIMDs (4 per CU).  c = 0:
+ * FP32 FMA, 1: FP32 exp2 (transcendental), 2: 64-bit integer multiply-add of two 32-bit words
+ * followed by a 32-bit add, 3: FP64 FMA.  No counterpart in the reference: the VALU roofline of
+ * the fused kernels is priced with these. */
+int rb_probe_issue(double* ns_per_inst, int* simds);
 /* The split-precision kernels' per-path payoff for given (ln S', sigma, K) -> out[i], and its
  * Mills-ratio approximation at z[i] -> mills[i] (n values).  form 0: the fused GEMM kernel's
  * (deep out-of-the-money tail through erfcx), 1: the long-path kernel's (through erfc). */

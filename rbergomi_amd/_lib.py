@@ -81,6 +81,7 @@ def load():
     L.rb_probe_mfma_layout.argtypes = [fp, fp, i, fp]
     L.rb_probe_payoff.argtypes = [dp, dp, dp, i, dp, fp, fp, i]
     L.rb_probe_copy.argtypes = [l, i, dp]
+    L.rb_probe_issue.argtypes = [dp, ctypes.POINTER(ctypes.c_int)]
     L.rb_probe_chol.argtypes = [d, i, dp, dp, dp]
     L.rb_probe_sobol.argtypes = [i, l, l, dp]
     L.rb_probe_norm_inv.argtypes = [dp, l, dp]

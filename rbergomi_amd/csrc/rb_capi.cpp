@@ -565,6 +565,37 @@ done:
     return rc;
 }
 
+int rb_probe_issue(double* ns_per_inst, int* simds) {
+    int rc = RB_OK;
+    float* d = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipDeviceProp_t prop;
+    int dev = 0;
+    if (!ns_per_inst) return fail(RB_ERR_ARG, "ns_per_inst is null");
+    PROBE_TRY(hipGetDevice(&dev));
+    PROBE_TRY(hipGetDeviceProperties(&prop, dev));
+    if (simds) *simds = 4 * prop.multiProcessorCount;
+    PROBE_TRY(hipMalloc(&d, size_t(2048) * 256 * sizeof(float)));
+    PROBE_TRY(hipEventCreate(&e0));
+    PROBE_TRY(hipEventCreate(&e1));
+    for (int op = 0; op < 4; ++op) {
+        const int reps = 5;
+        float ms = 0.f;
+        PROBE_TRY(launch_probe_issue(op, d, 1u, nullptr));  // warm-up
+        PROBE_TRY(hipEventRecord(e0, nullptr));
+        for (int r = 0; r < reps; ++r) PROBE_TRY(launch_probe_issue(op, d, 2u + uint32_t(r), nullptr));
+        PROBE_TRY(hipEventRecord(e1, nullptr));
+        PROBE_TRY(hipEventSynchronize(e1));
+        PROBE_TRY(hipEventElapsedTime(&ms, e0, e1));
+        ns_per_inst[op] = double(ms) / reps * 1e6 / probe_issue_insts_per_simd(4 * prop.multiProcessorCount);
+    }
+done:
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (d) (void)hipFree(d);
+    return rc;
+}
+
 int rb_probe_payoff(const double* lnS, const double* sig, const double* K, int n, double* out,
                     const float* z, float* mills, int form) {
     int rc = RB_OK;

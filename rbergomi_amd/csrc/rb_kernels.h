@@ -67,6 +67,8 @@ hipError_t launch_probe_payoff(const double* lnS, const double* sig, const doubl
                                double* out, float* mills_out, const float* z, int form, hipStream_t st);
 
 hipError_t launch_probe_copy(const void* src, void* dst, long bytes, hipStream_t st);
+hipError_t launch_probe_issue(int op, float* out, uint32_t seed, hipStream_t st);
+double probe_issue_insts_per_simd(int simds);
 
 // FFT convolution for long paths (rb_fft.hip)
 size_t volterra_fft_lds_bytes(int log2L);

@@ -91,4 +91,54 @@ hipError_t launch_probe_copy(const void* src, void* dst, long bytes, hipStream_t
     return hipGetLastError();
 }
 
+// Issue-rate probe: eight independent chains per lane of one instruction class, 2048
+// workgroups of 256 threads (eight waves per SIMD), so the SIMDs issue back to back.  op 0: FP32
+// FMA, 1: FP32 exp2 (transcendental), 2: 32 x 32 -> 64-bit integer multiply-add plus a 32-bit add
+// (the Philox round's product), 3: FP64 FMA.
+constexpr int PROBE_ISSUE_IT = 1024;
+
+template <int OP>
+__global__ void __launch_bounds__(256) k_probe_issue(float* __restrict__ out, uint32_t seed) {
+    float x[8];
+    double y[8];
+    uint32_t u[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        x[j] = 0.3f + 0.01f * float(j) + 1e-6f * float(threadIdx.x);
+        y[j] = double(x[j]);
+        u[j] = seed + 77u * uint32_t(j) + threadIdx.x;
+    }
+    for (int i = 0; i < PROBE_ISSUE_IT; ++i) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            if constexpr (OP == 0) x[j] = fmaf(x[j], 0.999f, 0.001f);
+            if constexpr (OP == 1) x[j] = __builtin_amdgcn_exp2f(-x[j]);
+            if constexpr (OP == 2) {
+                const uint64_t q = uint64_t(u[j]) * 0xD2511F53u;
+                u[j] = uint32_t(q >> 32) + uint32_t(q);
+            }
+            if constexpr (OP == 3) y[j] = fma(y[j], 0.999, 0.001);
+        }
+    }
+    float s = 0.f;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) s += x[j] + float(y[j]) + float(u[j] & 1023u);
+    out[blockIdx.x * 256 + threadIdx.x] = s;
+}
+
+hipError_t launch_probe_issue(int op, float* out, uint32_t seed, hipStream_t st) {
+    const dim3 g(2048), b(256);
+    switch (op) {
+        case 0: hipLaunchKernelGGL(k_probe_issue<0>, g, b, 0, st, out, seed); break;
+        case 1: hipLaunchKernelGGL(k_probe_issue<1>, g, b, 0, st, out, seed); break;
+        case 2: hipLaunchKernelGGL(k_probe_issue<2>, g, b, 0, st, out, seed); break;
+        case 3: hipLaunchKernelGGL(k_probe_issue<3>, g, b, 0, st, out, seed); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// wave64 instructions of one class per probe launch and SIMD (loop body only)
+double probe_issue_insts_per_simd(int simds) { return 2048.0 * 4 * PROBE_ISSUE_IT * 8 / simds; }
+
 }  // namespace rb

@@ -224,3 +224,18 @@ def test_flat_variance_limit_mixed(rb, prec, N):
         want = stats.norm.cdf(d1) - K[j] * stats.norm.cdf(d1 - s)
         assert abs(price[j] - want) < 2e-6, (j, price[j], want)
         assert abs(iv[j] - np.sqrt(xi)) < 1e-4
+
+
+def test_issue_rate_probe(rb):
+    """rb_probe_issue: positive per-instruction issue costs, a transcendental dearer than an FP32
+    FMA, and an FP32 FMA within a factor of two of the 64-lane / 32-lane-per-clock floor at the
+    device's peak engine clock (2.4 GHz: 0.83 ns)."""
+    say("issue-rate probe")
+    ns = (ctypes.c_double * 4)()
+    simds = ctypes.c_int()
+    assert rb.load().rb_probe_issue(ns, ctypes.byref(simds)) == 0
+    assert simds.value == 1024, simds.value   # MI355X: 256 CUs
+    fma, exp2, mad, f64 = list(ns)
+    assert all(v > 0 for v in ns), list(ns)
+    assert exp2 > 1.5 * fma and mad > fma and f64 >= 0.9 * fma, list(ns)
+    assert 0.5 < fma < 2.0, fma
