@@ -7,6 +7,9 @@
 
 namespace rb {
 
+// 16-byte units per 16-step chunk of a path block in the LDS W1 tile: the FP16 mode keeps one
+// FP16 value per step (64 lane slots), the BF16 (hi, lo) and FP32 modes two halves (128)
+constexpr int fused_chunk_u4(int mode) { return mode == 0 ? 64 : 128; }
 constexpr int FUSED_MAX_UPW = 8;   // output row blocks (32 rows) per wave
 constexpr int FUSED_GREG = 8;      // most groups integrated per pass over the accumulators
 constexpr int FUSED_MAX_REG_OPTS = 256;  // options whose sums live in registers
@@ -99,7 +102,7 @@ struct FusedReq {
     long M, ntiles, nitems;
     long items_big;              // items of tiles_per_item tiles; the rest have tiles_small
     int tiles_small;
-    int pad_;
+    int chunk_u4;                // fused_chunk_u4 of the launch's operand mode
     PhiloxKey key;
     const FusedUnit* units;
     const FusedGroup* groups;

@@ -32,7 +32,7 @@
 namespace rb {
 
 size_t fused_req_lds_bytes(const FusedReq& a) {
-    return size_t(a.PB) * a.NK * 128 * 16 + size_t(4) * a.gchunk * 32 * (a.mc ? 3 : 2) * 8 +
+    return size_t(a.PB) * a.NK * a.chunk_u4 * 16 + size_t(4) * a.gchunk * 32 * (a.mc ? 3 : 2) * 8 +
            size_t(a.TP) * a.pgmax * sizeof(PathPG) + (a.acc_in_lds ? size_t(a.out_size) * 2 * 8 : 0);
 }
 

@@ -226,6 +226,7 @@ int plan_fused_request(const Request& req, const Layout& lay, int mode, FusedHos
     a.N = N;
     a.Np = Np;
     a.NK = Np / 16;
+    a.chunk_u4 = fused_chunk_u4(mode);
     const int NB = Np / 32;
     // path blocks of 32 per tile: given, else the tuned or default choice for this shape
     a.PB = pb > 0 ? pb : fused_pb_choice(Np, mode, req.method == METHOD_MC ? 1 : 0);
@@ -407,7 +408,7 @@ size_t fused_batch_lds(const std::vector<const FusedHostReq*>& reqs, size_t lds_
     for (const FusedHostReq* q : reqs) {
         const FusedReq& r = q->r;
         ns = r.mc ? 3 : 2;
-        bf = std::max(bf, r.PB * r.NK * 128);
+        bf = std::max(bf, r.PB * r.NK * r.chunk_u4);
         wp = std::max(wp, 4 * r.gchunk * 32 * ns);
         pg = std::max(pg, r.TP * r.pgmax);
         ac = std::max(ac, r.acc_in_lds ? 2 * r.out_size : 0);

@@ -125,9 +125,9 @@ __device__ __forceinline__ void load_rows16(const float* p, float v[16]) {
 }
 
 // The W1 tile in LDS, per path block and 16-step chunk: 64 lane slots of 16 bytes as the MFMA B
-// operand (slot 32 x + col holds steps 16 ks + 8 x + 0..7 of path column col), then 64 more.
-// FP16 mode: W1 rounded to FP16 in the first 64 slots (the second 64 unused).  BF16 mode: hi in
-// the first 64, lo = RN(W1 - hi) in the second.  FP32 mode: steps 8 x + 0..3 in the first 64,
+// operand (slot 32 x + col holds steps 16 ks + 8 x + 0..7 of path column col), then, except in
+// the FP16 mode, 64 more (fused_chunk_u4).  FP16 mode: W1 rounded to FP16.  BF16 mode: hi in the
+// first 64, lo = RN(W1 - hi) in the second.  FP32 mode: steps 8 x + 0..3 in the first 64,
 // 8 x + 4..7 in the second.
 
 // W1 of path column `col`, step 16 ks + kk, from the LDS tile (the value the product used)
@@ -182,13 +182,14 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
     typedef SplitT<MODE> S;
     typedef typename S::V V;
     typedef typename S::V4 V4;
+    constexpr int CH = fused_chunk_u4(MODE);               // 16-byte units per W1 tile chunk
     extern __shared__ uint4 lds4[];
     __shared__ int s_item;
     __shared__ FusedReq s_req;   // the current item's request record
     static_assert(sizeof(FusedReq) % 4 == 0, "dword copy");
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int h = lane >> 5, col = lane & 31;
-    uint4* Bf = lds4;                                                   // [PB][NK][2][64]
+    uint4* Bf = lds4;                                                   // [PB][NK][CH]
     double* wpart = reinterpret_cast<double*>(lds4 + B.bf_u4);          // [4][gchunk][32][2]
     PathPG* pgd = reinterpret_cast<PathPG*>(wpart + B.wpart_d);         // [TP][pgmax]
     double* accopt = reinterpret_cast<double*>(pgd + B.pgd_n);          // [out][2] in LDS mode
@@ -242,7 +243,7 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
                 auto put = [&](int q, const float z[4]) {
                     const int ks = q >> 2, hh = (q & 3) >> 1, j0 = (q & 1);
                     const int ln = 32 * hh + (pl & 31);
-                    uint4* base = Bf + ((pb * NK + ks) * 2) * 64;
+                    uint4* base = Bf + (pb * NK + ks) * CH;
                     if constexpr (MODE == 2) {
                         const V4 v = {z[0], z[1], z[2], z[3]};
                         base[64 * j0 + ln] = __builtin_bit_cast(uint4, v);
@@ -307,7 +308,7 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
 #pragma unroll
                         for (int s = 0; s < k; ++s) {
                             const int ks = (32 * rbs[s] - dd) >> 4;
-                            const uint4* Bp = Bf + ((pbw * NK + ks) * 2) * 64;
+                            const uint4* Bp = Bf + (pbw * NK + ks) * CH;
                             bhi[s] = __builtin_bit_cast(V, Bp[lane]);
                             if constexpr (MODE != 0) blo[s] = __builtin_bit_cast(V, Bp[64 + lane]);
                         }
@@ -452,19 +453,19 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
                                     wv[4 * c + 3] = qb[0];
                                 }
                             } else {
-                                // 16-bit modes, byte offsets from slot col of chunk 2 rb: quad Q is
-                                // the uint2 h of slot 32 (c & 1) + col in chunk 2 rb + c / 2 (BF16
-                                // lo 1024 B further).  Element 0 of quad Q + 1 begins the next uint2
-                                // (h = 0), slot 32 + col (h = 1, c even) or slot col of the next
-                                // chunk (h = 1, c odd); for c = 3 in the last row block that is
-                                // past the tile, and the row (j = Np) has P = +inf, so the read
-                                // takes element 0 of the lane's own slot instead
-                                const char* Wr = reinterpret_cast<const char*>(Bf + ((pbw * NK + 2 * rb) * 2) * 64 + col);
-                                const int nb_even = h ? 512 : 8, nb_odd = h ? 1536 : 8;
+                                // 16-bit modes, byte offsets from slot col of chunk 2 rb (chunks of
+                                // 16 CH bytes): quad Q is the uint2 h of slot 32 (c & 1) + col in
+                                // chunk 2 rb + c / 2 (BF16 lo 1024 B further).  Element 0 of quad
+                                // Q + 1 begins the next uint2 (h = 0), slot 32 + col (h = 1, c
+                                // even) or slot col of the next chunk (h = 1, c odd); for c = 3 in
+                                // the last row block that is past the tile, and the row (j = Np)
+                                // has P = +inf, so the read takes element 0 of the lane's own slot
+                                const char* Wr = reinterpret_cast<const char*>(Bf + (pbw * NK + 2 * rb) * CH + col);
+                                const int nb_even = h ? 512 : 8, nb_odd = h ? 16 * CH - 512 : 8;
                                 const int nb3 = 2 * rb + 2 < NK ? nb_odd : (h ? 0 : 8);
 #pragma unroll
                                 for (int c = 0; c < 4; ++c) {
-                                    const char* slot = Wr + (c >> 1) * 2048 + (c & 1) * 512;
+                                    const char* slot = Wr + (c >> 1) * (16 * CH) + (c & 1) * 512;
                                     const int nb = c == 3 ? nb3 : (c & 1) ? nb_odd : nb_even;
                                     const uint2 ha = *reinterpret_cast<const uint2*>(slot + 8 * h);
                                     const uint32_t hb = *reinterpret_cast<const uint32_t*>(slot + nb);
@@ -552,7 +553,7 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
                             const int pl = e & (A.TP - 1), gg = e >> A.log2TP;
                             const int pb = pl >> 5, c = pl & 31;
                             const FusedGroup G = load_global_rec(A.groups + g0 + c0 + gg);
-                            const double w10 = double(tile_w1<MODE>(Bf + ((pb * NK) * 2) * 64, 0, c));  // k = 0
+                            const double w10 = double(tile_w1<MODE>(Bf + (pb * NK) * CH, 0, c));  // k = 0
                             double a = G.xi, b = G.sxi * w10;   // j = 0: v_0 = xi
                             double cp = 0.0;
                             if constexpr (MC) {

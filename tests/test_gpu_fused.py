@@ -239,3 +239,19 @@ def test_issue_rate_probe(rb):
     assert all(v > 0 for v in ns), list(ns)
     assert exp2 > 1.5 * fma and mad > fma and f64 >= 0.9 * fma, list(ns)
     assert 0.5 < fma < 2.0, fma
+
+
+@pytest.mark.parametrize("method", ["rt", "mc"])
+def test_fused_large_request_tail_items(rb, method):
+    """N = 32, M = 2^22: 32768 tiles of 128 paths in 4096 items of eight, so the request's last
+    eighth of tiles is cut into items of one tile, handed out last.  Every path must be priced
+    once: the FP16 price matches the FP64 path on the same counters (a dropped or repeated item
+    would move it by about an eighth of the price or by its statistical error)."""
+    say("large request, tail items, %s" % method)
+    T, K = [0.1, 0.1, 0.1], [0.95, 1.0, 1.05]
+    meth = rb.METHOD_RT if method == "rt" else rb.METHOD_MC
+    args = (0.04, [0.1], [1.5], [-0.7], T, K, 32, 2 ** 22, [5, 1])
+    p64, _, s64 = rb.price_v2(*args, precision=rb.PREC_FP64, method=meth)
+    p16, _, s16 = rb.price_v2(*args, precision=rb.PREC_SPLIT_FP16, method=meth)
+    assert np.max(np.abs(p16 - p64)) <= 1e-6, (p16, p64)
+    assert np.max(np.abs(s16 - s64)) <= 1e-6
