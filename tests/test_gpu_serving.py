@@ -127,13 +127,14 @@ def test_long_path_requests_in_batches_and_server(rb):
 
 
 def _lds_heavy_pair(method_mc=False):
-    """Two requests that each fit one workgroup's LDS alone but not together: N = 1024 with one
-    maturity (a 131 KB W1 tile) and N = 32 with 8 maturities x 2 rho (64 KB of per-path records
-    at four path blocks per tile)."""
+    """Two requests that each fit one workgroup's LDS (160 KB) alone but not together: N = 1024
+    with one maturity (a 64 KB FP16 W1 tile; 69 KB in all) and N = 32 with 8 maturities x 3 rho
+    (96 KB of per-path records at four path blocks per tile; 123 KB in all); together 181 KB."""
     from rbergomi_amd.serving import Request
     big_tile = Request(0.04, [0.1], [1.5], [-0.7], [1.0, 1.0], [0.9, 1.1], 1024, 3000, [31, 1])
     mats = np.repeat(np.linspace(0.1, 2.0, 8), 2)
-    many_pg = Request(0.05, [0.2, 0.2], [1.2, 1.2], [-0.8, -0.3], mats, np.tile([0.9, 1.1], 8), 32, 5000, [31, 2])
+    many_pg = Request(0.05, [0.2] * 3, [1.2] * 3, [-0.8, -0.55, -0.3], mats, np.tile([0.9, 1.1], 8), 32, 5000,
+                      [31, 2])
     return [big_tile, many_pg]
 
 
@@ -159,14 +160,15 @@ def test_lds_incompatible_requests_batch_and_serve(rb):
 
 
 def test_chunked_and_unchunked_partials_price_alike(rb):
-    # plain MC, N = 512, five maturities: alone, the cross-wave partials are summed in chunks of
-    # three groups (two workgroups per CU fit the LDS); in a batch whose other request brings
-    # 64 KB of per-path records, the launch is above half a CU's LDS anyway and the five groups go
-    # in one chunk.  The sums and their order are the same, so the bits are.
+    # plain MC, N = 1024, five maturities: alone, the cross-wave partials are summed in chunks of
+    # three groups (two workgroups per CU fit the LDS beside the 64 KB FP16 W1 tile); in a batch
+    # whose other request brings 32 KB of per-path records, the launch is above half a CU's LDS
+    # anyway and the five groups go in one chunk.  The sums and their order are the same, so the
+    # bits are.
     from rbergomi_amd.serving import Request, price_batch
     T = np.repeat([0.1, 0.25, 0.5, 1.0, 2.0], 3)
     K = np.tile([0.9, 1.0, 1.1], 5)
-    a = Request(0.04, [0.1], [1.5], [-0.7], T, K, 512, 6000, [44, 5])
+    a = Request(0.04, [0.1], [1.5], [-0.7], T, K, 1024, 6000, [44, 5])
     mats = np.repeat(np.linspace(0.1, 2.0, 8), 2)
     b = Request(0.05, [0.2, 0.2], [1.2, 1.2], [-0.8, -0.3], mats, np.tile([0.9, 1.1], 8), 32, 5000, [44, 6])
     kw = dict(precision=rb.PREC_SPLIT_FP16, method=rb.METHOD_MC)
@@ -194,14 +196,14 @@ def test_plain_mc_server_mix_equals_alone(rb):
 
 
 def test_server_rejects_request_beyond_one_workgroup_lds(rb):
-    # 8 maturities x 8 rho at N = 1024: 64 payoff groups of per-path records (64 KB at 32 paths
-    # per tile) and the 128 KB W1 tile cannot share one workgroup's LDS, so it is refused at
-    # submit with the reason
+    # 8 maturities x 16 rho at N = 1024: 128 payoff groups of per-path records (128 KB at 32
+    # paths per tile) and the 64 KB FP16 W1 tile cannot share one workgroup's 160 KB of LDS, so
+    # it is refused at submit with the reason
     from rbergomi_amd.serving import Request, Server
     mats = np.linspace(0.1, 2.0, 8)
-    H = [0.1] * 8
-    rho = list(np.linspace(-0.9, -0.2, 8))
-    big = Request(0.04, H, [1.5] * 8, rho, mats, np.ones(8), 1024, 1000, [1])
+    H = [0.1] * 16
+    rho = list(np.linspace(-0.9, -0.2, 16))
+    big = Request(0.04, H, [1.5] * 16, rho, mats, np.ones(8), 1024, 1000, [1])
     with Server() as srv:
         with pytest.raises(rb.RbError) as e:
             srv.submit(big)
