@@ -396,13 +396,6 @@ static hipError_t batch_alloc(DevArena* ar, T** p, size_t n, hipStream_t st) {
     return hipSuccess;
 }
 
-template <class T>
-static hipError_t batch_upload(DevArena* ar, T** p, const std::vector<T>& v, hipStream_t st) {
-    hipError_t e = batch_alloc(ar, p, v.size(), st);
-    if (e != hipSuccess || v.empty()) return e;
-    return hipMemcpyAsync(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st);
-}
-
 size_t fused_batch_lds(const std::vector<const FusedHostReq*>& reqs, size_t lds_cu, FusedBatchArgs* a, int* cap) {
     int bf = 0, wp = 0, pg = 0, ac = 0, ns = 2;
     for (const FusedHostReq* q : reqs) {
@@ -466,10 +459,8 @@ std::vector<std::pair<int, int>> FusedBatch::split(const std::vector<const Fused
 FusedBatch::~FusedBatch() { release(); }
 
 void FusedBatch::release() {
-    void* bufs[] = {d_reqs, d_fu, d_fg, d_fpg, d_fo, d_frag, d_ptab, d_item_req, d_item_local,
-                    d_counter, d_partial, d_price, d_stat, d_iv, d_ivopts, d_row_off, d_row_n, d_row_M};
-    for (void* b : bufs)
-        if (b && !arena) (void)hipFreeAsync(b, stream);
+    if (d_block && !arena) (void)hipFreeAsync(d_block, stream);
+    d_block = nullptr;
     d_reqs = nullptr;
     d_fu = nullptr;
     d_fg = nullptr;
@@ -671,24 +662,43 @@ int FusedBatch::build(const std::vector<const FusedHostReq*>& reqs, int mode_, i
         return ST_OK;
     }
     shape_ = shape;
-    // upload
-    RB_TRY(batch_upload(arena, &d_fu, fu, st));
-    RB_TRY(batch_upload(arena, &d_fg, fg, st));
-    RB_TRY(batch_upload(arena, &d_fpg, fpg, st));
-    RB_TRY(batch_upload(arena, &d_fo, fo, st));
-    RB_TRY(batch_upload(arena, &d_frag, frag, st));
-    RB_TRY(batch_upload(arena, &d_ptab, ptab, st));
-    RB_TRY(batch_upload(arena, &d_ivopts, ivo, st));
-    RB_TRY(batch_upload(arena, &d_row_off, row_off, st));
-    RB_TRY(batch_upload(arena, &d_row_n, row_n, st));
-    RB_TRY(batch_upload(arena, &d_row_M, row_M, st));
-    RB_TRY(batch_upload(arena, &d_item_req, item_req, st));
-    RB_TRY(batch_upload(arena, &d_item_local, item_local, st));
-    RB_TRY(batch_alloc(arena, &d_partial, size_t(poff), st));
-    RB_TRY(batch_alloc(arena, &d_counter, 1, st));
-    RB_TRY(batch_alloc(arena, &d_price, size_t(total_out), st));
-    RB_TRY(batch_alloc(arena, &d_stat, size_t(total_out), st));
-    RB_TRY(batch_alloc(arena, &d_iv, size_t(total_out), st));
+    // One device block: the tables, request records and item counter (one upload), then the
+    // partial sums and the results price | iv | stat, adjacent so that one copy brings them back.
+    // Pieces start on 256-byte boundaries.
+    size_t off = 0;
+    auto place = [&](size_t bytes) {
+        const size_t o = off;
+        off += (std::max<size_t>(bytes, 1) + 255) & ~size_t(255);
+        return o;
+    };
+    const size_t o_fu = place(fu.size() * sizeof(FusedUnit)), o_fg = place(fg.size() * sizeof(FusedGroup)),
+                 o_fpg = place(fpg.size() * sizeof(FusedPG)), o_fo = place(fo.size() * sizeof(FusedOption)),
+                 o_frag = place(frag.size() * sizeof(uint4)), o_ptab = place(ptab.size() * sizeof(float)),
+                 o_ivo = place(ivo.size() * sizeof(Option)), o_roff = place(row_off.size() * sizeof(long)),
+                 o_rn = place(row_n.size() * sizeof(long)), o_rm = place(row_M.size() * sizeof(long)),
+                 o_ireq = place(item_req.size() * sizeof(int)), o_iloc = place(item_local.size() * sizeof(int)),
+                 o_reqs = place(R.size() * sizeof(FusedReq)), o_cnt = place(sizeof(int));
+    const size_t up_bytes = off;
+    const size_t o_part = place(size_t(poff) * sizeof(double)), o_out = place(size_t(total_out) * 3 * sizeof(double));
+    RB_TRY(batch_alloc(arena, &d_block, off, st));
+    d_fu = reinterpret_cast<FusedUnit*>(d_block + o_fu);
+    d_fg = reinterpret_cast<FusedGroup*>(d_block + o_fg);
+    d_fpg = reinterpret_cast<FusedPG*>(d_block + o_fpg);
+    d_fo = reinterpret_cast<FusedOption*>(d_block + o_fo);
+    d_frag = reinterpret_cast<uint4*>(d_block + o_frag);
+    d_ptab = reinterpret_cast<float*>(d_block + o_ptab);
+    d_ivopts = reinterpret_cast<Option*>(d_block + o_ivo);
+    d_row_off = reinterpret_cast<long*>(d_block + o_roff);
+    d_row_n = reinterpret_cast<long*>(d_block + o_rn);
+    d_row_M = reinterpret_cast<long*>(d_block + o_rm);
+    d_item_req = reinterpret_cast<int*>(d_block + o_ireq);
+    d_item_local = reinterpret_cast<int*>(d_block + o_iloc);
+    d_reqs = reinterpret_cast<FusedReq*>(d_block + o_reqs);
+    d_counter = reinterpret_cast<int*>(d_block + o_cnt);
+    d_partial = reinterpret_cast<double*>(d_block + o_part);
+    d_price = reinterpret_cast<double*>(d_block + o_out);
+    d_iv = d_price + total_out;
+    d_stat = d_price + 2 * size_t(total_out);
     for (FusedReq& r : R) {
         r.units = d_fu + size_t(r.units);
         r.groups = d_fg + size_t(r.groups);
@@ -698,14 +708,38 @@ int FusedBatch::build(const std::vector<const FusedHostReq*>& reqs, int mode_, i
         r.ptab = d_ptab;
         r.partial = d_partial + size_t(r.partial);
     }
-    RB_TRY(batch_upload(arena, &d_reqs, R, st));
+    unsigned char* h = stage ? stage(up_bytes) : nullptr;
+    const bool staged = h != nullptr;
+    if (!staged) {
+        hblock_.resize(up_bytes);
+        h = hblock_.data();
+    }
+    auto pack = [&](size_t o, const auto& v) {
+        if (!v.empty()) std::memcpy(h + o, v.data(), v.size() * sizeof(v[0]));
+    };
+    pack(o_fu, fu);
+    pack(o_fg, fg);
+    pack(o_fpg, fpg);
+    pack(o_fo, fo);
+    pack(o_frag, frag);
+    pack(o_ptab, ptab);
+    pack(o_ivo, ivo);
+    pack(o_roff, row_off);
+    pack(o_rn, row_n);
+    pack(o_rm, row_M);
+    pack(o_ireq, item_req);
+    pack(o_iloc, item_local);
+    pack(o_reqs, R);
+    std::memset(h + o_cnt, 0, sizeof(int));
+    RB_TRY(hipMemcpyAsync(d_block, h, up_bytes, hipMemcpyHostToDevice, st));
     args.reqs = d_reqs;
     args.item_req = d_item_req;
     args.item_local = d_item_local;
     args.counter = d_counter;
     const int per_cu = int(std::max<size_t>(1, std::min<size_t>(2, lds_cu / lds_total)));
     grid = int(std::min<long>(args.total_items, long(prop.multiProcessorCount) * per_cu));
-    RB_TRY(hipStreamSynchronize(st));
+    // pageable staging is reused by the next build: wait for its copy
+    if (!staged) RB_TRY(hipStreamSynchronize(st));
     return ST_OK;
 }
 

@@ -4,6 +4,7 @@
 
 #include <string>
 #include <array>
+#include <functional>
 #include <vector>
 
 #include "rb_fused.h"
@@ -135,8 +136,14 @@ struct FusedBatch {
     long* d_row_off = nullptr;
     long* d_row_n = nullptr;
     long* d_row_M = nullptr;
+    char* d_block = nullptr;     // the one device allocation every pointer above points into
 
     DevArena* arena = nullptr;   // when set, buffers come from it (see DevArena)
+    // pinned host memory of at least the bytes asked for, or null: when set, build packs the
+    // tables there and returns without waiting for their copy (the caller keeps the memory
+    // untouched until the batch's stream has passed the launch)
+    std::function<unsigned char*(size_t)> stage;
+    std::vector<unsigned char> hblock_;  // pageable staging when no stage is given
 
     ~FusedBatch();
     void release();

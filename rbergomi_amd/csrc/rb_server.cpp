@@ -164,6 +164,9 @@ struct rb_server {
     std::vector<std::unique_ptr<Flight>> flights;
     std::vector<hipStream_t> free_streams;
     std::map<hipStream_t, std::pair<double*, size_t>> pinned;  // reusable result staging per stream
+    // reusable upload staging per stream (pinned: the batch's one table copy runs without the
+    // server thread waiting for it; a stream is reused only after its flight has completed)
+    std::map<hipStream_t, std::pair<unsigned char*, size_t>> upinned;
     size_t inflight_bytes = 0;
     bool stop = false;
     long next_ticket = 1;
@@ -312,6 +315,16 @@ bool rb_server::launch(std::unique_lock<std::mutex>& lk) {
         std::vector<const FusedHostReq*> hr;
         for (Job* j : take) hr.push_back(&j->fh);
         F->batch.reset(new FusedBatch());
+        auto& up = upinned[F->stream];  // only the server thread touches the pool
+        F->batch->stage = [&up](size_t bytes) -> unsigned char* {
+            if (up.second < bytes) {
+                if (up.first) (void)hipHostFree(up.first);
+                up.first = nullptr;
+                up.second = 0;
+                if (hipHostMalloc(reinterpret_cast<void**>(&up.first), 2 * bytes) == hipSuccess) up.second = 2 * bytes;
+            }
+            return up.first;
+        };
         const int mode = fused_mode_of(cfg.precision);
         if (!rc) rc = F->batch->build(hr, mode, device, F->stream, err);
         if (!rc) rc = F->batch->launch(F->stream, err, nullptr, F->kbeg, F->kend);
@@ -337,9 +350,12 @@ bool rb_server::launch(std::unique_lock<std::mutex>& lk) {
     }
     if (!rc) {
         const size_t n = size_t(total) * sizeof(double);
-        if (hipMemcpyAsync(F->pinned, d_price, n, hipMemcpyDeviceToHost, F->stream) ||
-            hipMemcpyAsync(F->pinned + total, d_iv, n, hipMemcpyDeviceToHost, F->stream) ||
-            hipMemcpyAsync(F->pinned + 2 * total, d_stat, n, hipMemcpyDeviceToHost, F->stream) ||
+        // one copy when the results lie price | iv | stat in device memory (fused batches)
+        const bool adjacent = d_iv == d_price + total && d_stat == d_price + 2 * size_t(total);
+        if ((adjacent ? hipMemcpyAsync(F->pinned, d_price, 3 * n, hipMemcpyDeviceToHost, F->stream)
+                      : hipMemcpyAsync(F->pinned, d_price, n, hipMemcpyDeviceToHost, F->stream) ||
+                            hipMemcpyAsync(F->pinned + total, d_iv, n, hipMemcpyDeviceToHost, F->stream) ||
+                            hipMemcpyAsync(F->pinned + 2 * total, d_stat, n, hipMemcpyDeviceToHost, F->stream)) ||
             hipEventCreateWithFlags(&F->done, hipEventDisableTiming) ||
             hipEventRecord(F->done, F->stream)) {
             err = "cannot enqueue the result copies";
@@ -846,6 +862,8 @@ void rb_server_destroy(rb_server* s) {
     if (s->th.joinable()) s->th.join();
     for (hipStream_t st : s->free_streams) (void)hipStreamSynchronize(st);
     for (auto& kv : s->pinned)
+        if (kv.second.first) (void)hipHostFree(kv.second.first);
+    for (auto& kv : s->upinned)
         if (kv.second.first) (void)hipHostFree(kv.second.first);
     for (hipStream_t st : s->free_streams) (void)hipStreamDestroy(st);
     if (s->epoch) (void)hipEventDestroy(s->epoch);
