@@ -695,6 +695,7 @@ int FusedBatch::build(const std::vector<const FusedHostReq*>& reqs, int mode_, i
     d_item_local = reinterpret_cast<int*>(d_block + o_iloc);
     d_reqs = reinterpret_cast<FusedReq*>(d_block + o_reqs);
     d_counter = reinterpret_cast<int*>(d_block + o_cnt);
+    counter_dirty = false;
     d_partial = reinterpret_cast<double*>(d_block + o_part);
     d_price = reinterpret_cast<double*>(d_block + o_out);
     d_iv = d_price + total_out;
@@ -749,18 +750,20 @@ int FusedBatch::launch(hipStream_t st, std::string& err, Plan* timing, hipEvent_
         if (timing) timing->mark(slot, nullptr, nullptr);
     };
     int s;
-    RB_TRY(launch_zero_int(d_counter, 1, st));
+    // the upload and every complete launch leave the item counter at zero (the reduction resets
+    // it); a launch that failed part-way may not have
+    if (counter_dirty) RB_TRY(launch_zero_int(d_counter, 1, st));
+    counter_dirty = true;
     if (kbeg) RB_TRY(hipEventRecord(kbeg, st));
     s = timed(K_FUSED);
     RB_TRY(launch_fused(mode, upw, mc, greg, args, grid, st));
     mark(s);
     if (kend) RB_TRY(hipEventRecord(kend, st));
     s = timed(K_REDUCE);
-    RB_TRY(launch_reduce_rows(d_partial, d_row_off, d_row_n, d_row_M, total_out, d_price, d_stat, st));
+    RB_TRY(launch_reduce_rows_iv(d_partial, d_row_off, d_row_n, d_row_M, d_ivopts, total_out, d_price, d_stat, d_iv,
+                                 d_counter, st));
     mark(s);
-    s = timed(K_IV);
-    RB_TRY(launch_iv_f64(d_price, d_ivopts, total_out, d_iv, st));
-    mark(s);
+    counter_dirty = total_out < 1;
     return ST_OK;
 }
 

@@ -128,6 +128,7 @@ struct FusedBatch {
     int* d_item_req = nullptr;
     int* d_item_local = nullptr;
     int* d_counter = nullptr;
+    bool counter_dirty = true;   // the item counter may be nonzero (zeroed before the next launch)
     double* d_partial = nullptr;
     double* d_price = nullptr;
     double* d_stat = nullptr;
@@ -160,7 +161,8 @@ struct FusedBatch {
     // (soft: also keep launches that fit two workgroups per CU there, fused_batch_accepts)
     static std::vector<std::pair<int, int>> split(const std::vector<const FusedHostReq*>& reqs, size_t lds_block,
                                                   size_t lds_cu, bool soft);
-    // enqueue: zero the item counter, fused kernel, ordered reduction, implied vols; kbeg / kend
+    // enqueue: fused kernel, ordered reduction with implied vols (which zeroes the item counter
+    // for the next launch; the counter is zeroed first only after a failed launch); kbeg / kend
     // (may be null) are recorded on st right before and after the fused kernel
     int launch(hipStream_t st, std::string& err, Plan* timing = nullptr, hipEvent_t kbeg = nullptr,
                hipEvent_t kend = nullptr);

@@ -54,6 +54,10 @@ hipError_t launch_reduce_f64(const double* partial, long nblk, long M, int out_s
 hipError_t launch_reduce_rows(const double* partial, const long* row_off, const long* row_n,
                               const long* row_M, int nrows, double* price, double* stat,
                               hipStream_t st);
+// launch_reduce_rows and launch_iv_f64 in one kernel, which also sets *counter to zero (nrows >= 1)
+hipError_t launch_reduce_rows_iv(const double* partial, const long* row_off, const long* row_n,
+                                 const long* row_M, const Option* opts, int nrows, double* price,
+                                 double* stat, double* iv, int* counter, hipStream_t st);
 hipError_t launch_iv_ref_f64(const double* price, const double* K, const double* T, int n, double* iv,
                              hipStream_t st);
 hipError_t launch_iv_f64(const double* price, const Option* opts, int n, double* iv,

@@ -346,6 +346,42 @@ k_reduce_rows(const double* __restrict__ partial, const long* __restrict__ row_o
     }
 }
 
+// k_reduce_rows and k_iv_f64 in one launch (the same sums, the same vols), and the fused
+// kernel's item counter set back to zero for the batch's next launch
+__global__ void __launch_bounds__(256)
+k_reduce_rows_iv(const double* __restrict__ partial, const long* __restrict__ row_off,
+                 const long* __restrict__ row_n, const long* __restrict__ row_M,
+                 const Option* __restrict__ opts, double* __restrict__ price, double* __restrict__ stat,
+                 double* __restrict__ iv, int* __restrict__ counter) {
+    __shared__ double s1[256], s2[256];
+    const int I = blockIdx.x;
+    const double* base = partial + row_off[I];
+    const long n = row_n[I];
+    double a = 0.0, b = 0.0;
+    for (long k = threadIdx.x; k < n; k += 256) {
+        a += base[2 * k];
+        b += base[2 * k + 1];
+    }
+    s1[threadIdx.x] = a;
+    s2[threadIdx.x] = b;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+            s1[threadIdx.x] += s1[threadIdx.x + off];
+            s2[threadIdx.x] += s2[threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double m = double(row_M[I]);
+        const double pr = s1[0] / m;
+        price[I] = pr;
+        stat[I] = sqrt(s2[0] / m - pr * pr) / sqrt(m);
+        iv[I] = implied_vol_f64(pr, opts[I].K, opts[I].T);
+        if (I == 0) *counter = 0;
+    }
+}
+
 __global__ void k_iv_f64(const double* __restrict__ price, const Option* __restrict__ opts,
                          int n, double* __restrict__ iv) {
     const int I = blockIdx.x * blockDim.x + threadIdx.x;
@@ -485,6 +521,15 @@ hipError_t launch_reduce_rows(const double* partial, const long* row_off, const 
     if (nrows < 1) return hipSuccess;
     hipLaunchKernelGGL(k_reduce_rows, dim3(nrows), dim3(256), 0, st, partial, row_off, row_n, row_M,
                        price, stat);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce_rows_iv(const double* partial, const long* row_off, const long* row_n,
+                                 const long* row_M, const Option* opts, int nrows, double* price,
+                                 double* stat, double* iv, int* counter, hipStream_t st) {
+    if (nrows < 1) return hipSuccess;
+    hipLaunchKernelGGL(k_reduce_rows_iv, dim3(nrows), dim3(256), 0, st, partial, row_off, row_n, row_M,
+                       opts, price, stat, iv, counter);
     return hipGetLastError();
 }
 
