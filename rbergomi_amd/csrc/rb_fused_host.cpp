@@ -711,35 +711,55 @@ int FusedBatch::build(const std::vector<const FusedHostReq*>& reqs, int mode_, i
     }
     unsigned char* h = stage ? stage(up_bytes) : nullptr;
     const bool staged = h != nullptr;
-    if (!staged) {
-        hblock_.resize(up_bytes);
-        h = hblock_.data();
+    if (staged) {
+        // packed into the caller's pinned staging: one copy, not waited for
+        auto pack = [&](size_t o, const auto& v) {
+            if (!v.empty()) std::memcpy(h + o, v.data(), v.size() * sizeof(v[0]));
+        };
+        pack(o_fu, fu);
+        pack(o_fg, fg);
+        pack(o_fpg, fpg);
+        pack(o_fo, fo);
+        pack(o_frag, frag);
+        pack(o_ptab, ptab);
+        pack(o_ivo, ivo);
+        pack(o_roff, row_off);
+        pack(o_rn, row_n);
+        pack(o_rm, row_M);
+        pack(o_ireq, item_req);
+        pack(o_iloc, item_local);
+        pack(o_reqs, R);
+        std::memset(h + o_cnt, 0, sizeof(int));
+        RB_TRY(hipMemcpyAsync(d_block, h, up_bytes, hipMemcpyHostToDevice, st));
+    } else {
+        // straight from the host tables (pageable memory: a packed copy would cost a host copy of
+        // every table); the first launch zeroes the counter
+        auto put = [&](size_t o, const auto& v) {
+            return v.empty() ? hipSuccess
+                             : hipMemcpyAsync(d_block + o, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice, st);
+        };
+        RB_TRY(put(o_fu, fu));
+        RB_TRY(put(o_fg, fg));
+        RB_TRY(put(o_fpg, fpg));
+        RB_TRY(put(o_fo, fo));
+        RB_TRY(put(o_frag, frag));
+        RB_TRY(put(o_ptab, ptab));
+        RB_TRY(put(o_ivo, ivo));
+        RB_TRY(put(o_roff, row_off));
+        RB_TRY(put(o_rn, row_n));
+        RB_TRY(put(o_rm, row_M));
+        RB_TRY(put(o_ireq, item_req));
+        RB_TRY(put(o_iloc, item_local));
+        RB_TRY(put(o_reqs, R));
+        counter_dirty = true;
     }
-    auto pack = [&](size_t o, const auto& v) {
-        if (!v.empty()) std::memcpy(h + o, v.data(), v.size() * sizeof(v[0]));
-    };
-    pack(o_fu, fu);
-    pack(o_fg, fg);
-    pack(o_fpg, fpg);
-    pack(o_fo, fo);
-    pack(o_frag, frag);
-    pack(o_ptab, ptab);
-    pack(o_ivo, ivo);
-    pack(o_roff, row_off);
-    pack(o_rn, row_n);
-    pack(o_rm, row_M);
-    pack(o_ireq, item_req);
-    pack(o_iloc, item_local);
-    pack(o_reqs, R);
-    std::memset(h + o_cnt, 0, sizeof(int));
-    RB_TRY(hipMemcpyAsync(d_block, h, up_bytes, hipMemcpyHostToDevice, st));
     args.reqs = d_reqs;
     args.item_req = d_item_req;
     args.item_local = d_item_local;
     args.counter = d_counter;
     const int per_cu = int(std::max<size_t>(1, std::min<size_t>(2, lds_cu / lds_total)));
     grid = int(std::min<long>(args.total_items, long(prop.multiProcessorCount) * per_cu));
-    // pageable staging is reused by the next build: wait for its copy
+    // pageable sources: the host tables die with this call
     if (!staged) RB_TRY(hipStreamSynchronize(st));
     return ST_OK;
 }

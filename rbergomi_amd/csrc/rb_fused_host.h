@@ -141,10 +141,10 @@ struct FusedBatch {
 
     DevArena* arena = nullptr;   // when set, buffers come from it (see DevArena)
     // pinned host memory of at least the bytes asked for, or null: when set, build packs the
-    // tables there and returns without waiting for their copy (the caller keeps the memory
-    // untouched until the batch's stream has passed the launch)
+    // tables there, copies them in one piece and returns without waiting for the copy (the
+    // caller keeps the memory untouched until the batch's stream has passed the launch);
+    // otherwise each table is copied from its host vector and build waits for the copies
     std::function<unsigned char*(size_t)> stage;
-    std::vector<unsigned char> hblock_;  // pageable staging when no stage is given
 
     ~FusedBatch();
     void release();
