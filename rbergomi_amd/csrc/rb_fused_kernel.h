@@ -639,12 +639,17 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
                             }
                         }
                     }
-                    __syncthreads();
+                    // no barrier here: the W1 tile was last read before the record barrier
+                    // above, so the next tile's Gaussians may overwrite it while other waves
+                    // still price; the records are rewritten, and the partials reread, only
+                    // after the next pass's or tile's barriers, which every wave reaches after
+                    // its payoffs
                 }
             }
         }
         // ---- item partial sums (indexed by the output index I)
         if (A.acc_in_lds) {
+            __syncthreads();   // option o's LDS sums were added by thread (o - o_lo) mod 256
             for (int o = tid; o < A.out_size; o += 256) {
                 const int I = A.opts[o].out;
                 A.partial[(long(I) * A.nitems + item) * 2 + 0] = accopt[2 * o + 0];
