@@ -483,27 +483,30 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
                             }
                             // rows j = 32 rb + row + 1 >= N have the row term +inf (P = +inf,
                             // al > 0), so rr = 0 and they add nothing: no per-row test.  Groups
-                            // outside, rows inside: one uniform branch per group instead of one
-                            // per row and group; each group's sum still runs over r = 0..15 in
-                            // order.  The lane's 16 row terms of a group are four aligned float4
-                            // loads, those of the next group issued before this group's rows (two
-                            // buffers by the parity of the group).  The loads are issued on every
-                            // path (groups past gn reload the last one), so the waits before a
-                            // group's rows count only the loads of the group after it.
+                            // outside, rows inside, and no branch at all: every group slot of the
+                            // window is integrated, so the 16 GR rows are one basic block in which
+                            // the compiler interleaves the groups' dependent sums.  A slot past gn
+                            // has c1 = 0 and the last group's row terms, so its sums are finite,
+                            // and they are never read.  Each group's sum still runs over
+                            // r = 0..15 in order.  (One uniform branch per group, so that dead
+                            // slots cost nothing: config 2 6.55 against 6.43 ms, plain MC 6.85
+                            // against 6.62, serving trace 3.16e11 against 3.19e11 paths*steps/s;
+                            // both forms in one kernel spilled more and lost on the trace.)  The
+                            // lane's 16 row terms of a group are four aligned float4 loads, those
+                            // of the next group issued before this group's rows (two buffers by
+                            // the parity of the group); slots past gn reload the last group's.
                             float qv[2][16];
                             load_rows16(Qr, qv[0]);
 #pragma unroll
                             for (int gg = 0; gg < GR; ++gg) {
                                 if (gg + 1 < GR)
                                     load_rows16(Qr + (gg + 1 < gn ? gg + 1 : gn - 1) * A.Np, qv[(gg + 1) & 1]);
-                                if (gg == 0 || gg < gn) {   // a window has at least one group
 #pragma unroll
-                                    for (int r = 0; r < 16; ++r) {
-                                        const float rr = __builtin_amdgcn_exp2f(fmaf(acc[s][r], c1[gg], -qv[gg & 1][r]));
-                                        fv[gg] = fmaf(rr, rr, fv[gg]);
-                                        fs[gg] = fmaf(rr, wv[r], fs[gg]);
-                                        if constexpr (MC) fp[gg] = fmaf(rr, wq[r], fp[gg]);
-                                    }
+                                for (int r = 0; r < 16; ++r) {
+                                    const float rr = __builtin_amdgcn_exp2f(fmaf(acc[s][r], c1[gg], -qv[gg & 1][r]));
+                                    fv[gg] = fmaf(rr, rr, fv[gg]);
+                                    fs[gg] = fmaf(rr, wv[r], fs[gg]);
+                                    if constexpr (MC) fp[gg] = fmaf(rr, wq[r], fp[gg]);
                                 }
                             }
                             if constexpr (!MC) {
