@@ -494,7 +494,7 @@ def bench_config1(rb, args):
     out = {"metric": "paths*steps/s", "unit": "paths*steps/s", "n_gpus": 1, "higher_is_better": True,
            "value": rows["fp16"]["paths_steps_per_s"], "dtype": "f16+f32", "vs_baseline": None,
            "scaling": "weak", "data": "synthetic (device Philox4x32-10 Gaussians)",
-           "config": {"workload": "config 1: N=64, M=2^16, 21 strikes, one parameter set"}, "rows": rows}
+           "config": {"workload": "one parameter set: N=64, M=2^16, 21 strikes"}, "rows": rows}
     if not args.no_cpu:
         from oracle import pyoracle as orc
         cpu = {}

@@ -489,9 +489,10 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
                             // has c1 = 0 and the last group's row terms, so its sums are finite,
                             // and they are never read.  Each group's sum still runs over
                             // r = 0..15 in order.  (One uniform branch per group, so that dead
-                            // slots cost nothing: config 2 6.55 against 6.43 ms, plain MC 6.85
-                            // against 6.62, serving trace 3.16e11 against 3.19e11 paths*steps/s;
-                            // both forms in one kernel spilled more and lost on the trace.)  The
+                            // slots cost nothing: N = 512, M = 2^22, five maturities 6.55 against
+                            // 6.43 ms, plain MC 6.85 against 6.62, serving trace 3.16e11 against
+                            // 3.19e11 paths*steps/s; both forms in one kernel spilled more and
+                            // lost on the trace.)  The
                             // lane's 16 row terms of a group are four aligned float4 loads, those
                             // of the next group issued before this group's rows (two buffers by
                             // the parity of the group); slots past gn reload the last group's.

@@ -50,19 +50,19 @@ C1_SEED = [20240601, 1]
 
 
 @pytest.fixture(scope="module")
-def config1_oracle(orc):
+def n64_oracle(orc):
     c = C1
     Z = orc.normals_philox(C1_SEED, 0, c["M"], 2, c["N"])
     pay = orc.payoff_rt_samples(c["xi"], c["H"], c["eta"], c["rho"], c["T"], c["K"], c["N"], Z)
     return orc.price_stat(pay)
 
 
-def test_config1_fp64_legacy_abi_matches_oracle(rb, orc, config1_oracle):
+def test_n64_fp64_legacy_abi_matches_oracle(rb, orc, n64_oracle):
     # computeIVRT_tensor itself (the drop-in symbol), num_threads = 8 as a hint
     c = C1
-    say("config 1: computeIVRT_tensor N=64 M=2^16, 21 strikes")
+    say("N=64 M=2^16, 21 strikes: computeIVRT_tensor")
     res = rb.computeIV_RT(c["xi"], c["H"], c["eta"], c["rho"], c["T"], c["K"], c["N"], c["M"], C1_SEED, 8)
-    pr, st = config1_oracle
+    pr, st = n64_oracle
     assert np.max(np.abs(res["price"] - pr)) <= 1e-12
     assert np.max(np.abs(res["stat"] - st)) <= 1e-12
     for I in range(21):
@@ -74,15 +74,15 @@ def test_config1_fp64_legacy_abi_matches_oracle(rb, orc, config1_oracle):
 
 
 @pytest.mark.parametrize("prec", ["fp16", "bf16", "fp32gemm"])
-def test_config1_mixed_plans_match_oracle(rb, orc, config1_oracle, prec):
+def test_n64_mixed_plans_match_oracle(rb, orc, n64_oracle, prec):
     c = C1
-    say("config 1: %s plan vs oracle" % prec)
+    say("N=64 M=2^16: %s plan vs oracle" % prec)
     pl = rb.Plan(c["xi"], c["H"], c["eta"], c["rho"], c["T"], c["K"], c["N"], c["M"], C1_SEED,
                  precision=getattr(rb, CODES[prec]))
     pl.run()
     p, iv, s = pl.results()
     pl.close()
-    pr, st = config1_oracle
+    pr, st = n64_oracle
     err = np.max(np.abs(p - pr))
     say("   max |price - oracle| = %.3e, max |stat - oracle| = %.3e" % (err, np.max(np.abs(s - st))))
     assert err <= 1e-4
@@ -92,8 +92,8 @@ def test_config1_mixed_plans_match_oracle(rb, orc, config1_oracle, prec):
     assert np.all(np.abs(p - pr) <= 3 * st)
 
 
-def test_config1_plain_mc_mixed_matches_oracle(rb, orc):
-    # computeIV_tensor's estimator on config 1's parameters, split FP16 (k_fused, third stream)
+def test_n64_plain_mc_mixed_matches_oracle(rb, orc):
+    # computeIV_tensor's estimator on the N=64, M=2^16 parameters, split FP16 (k_fused, third stream)
     c = C1
     M = 2 ** 14
     Z = orc.normals_philox(C1_SEED, 0, M, 3, c["N"])
@@ -106,7 +106,7 @@ def test_config1_plain_mc_mixed_matches_oracle(rb, orc):
         assert np.max(np.abs(p - pr)) <= tol and np.max(np.abs(s - st)) <= tol
 
 
-# ---------------------------------------------------------------------------------- config 3
+# ------------------------------------------------------------------- 256 parameter sets
 def _requests(rb, ds, M_cap=None):
     from rbergomi_amd.serving import Request
     return [Request(d["xi"], d["H"], d["eta"], d["rho"], d["T"], d["K"], d["N"],
@@ -114,11 +114,11 @@ def _requests(rb, ds, M_cap=None):
 
 
 @pytest.mark.parametrize("prec", ["fp16", "bf16"])
-def test_config3_batched_launch_within_budget_of_fp64(rb, bench, prec):
+def test_256_sets_batched_launch_within_budget_of_fp64(rb, bench, prec):
     from rbergomi_amd.serving import price_batch
     ds = bench.batched_requests()
     assert len(ds) == 256
-    say("config 3: 256 sets, %s batch vs FP64 one by one" % prec)
+    say("256 sets: %s batch vs FP64 one by one" % prec)
     reqs = price_batch(_requests(rb, ds), precision=getattr(rb, CODES[prec]))
     worst = 0.0
     for d, r in zip(ds, reqs):
@@ -130,10 +130,10 @@ def test_config3_batched_launch_within_budget_of_fp64(rb, bench, prec):
     say("   worst |price - fp64| over 256 x 21 prices: %.3e" % worst)
 
 
-CONFIG3_ORACLE_SETS = [0, 1, 2, 5, 6, 10, 15, 32]  # N = 64, 128, 256, 32, 512, 512, 128, 32
+SETS256_ORACLE_SETS = [0, 1, 2, 5, 6, 10, 15, 32]  # N = 64, 128, 256, 32, 512, 512, 128, 32
 
 
-def test_config3_sets_match_oracle(rb, orc, bench):
+def test_256_sets_match_oracle(rb, orc, bench):
     # eight of the 256 sets (every N of the grid among them) directly against the oracle on the
     # same Philox counters, at M = 2^13; the other sets ride in the same batched launch
     from rbergomi_amd.serving import price_batch
@@ -141,13 +141,13 @@ def test_config3_sets_match_oracle(rb, orc, bench):
     Ms = 2 ** 13
     for prec in ("fp16", "bf16", "fp32gemm"):
         reqs = price_batch(_requests(rb, ds, M_cap=Ms), precision=getattr(rb, CODES[prec]))
-        for i in CONFIG3_ORACLE_SETS:
+        for i in SETS256_ORACLE_SETS:
             d = ds[i]
             Z = orc.normals_philox(d["seed"], 0, Ms, 2, d["N"])
             pr, st = orc.price_stat(orc.payoff_rt_samples(d["xi"], d["H"], d["eta"], d["rho"], d["T"],
                                                           d["K"], d["N"], Z))
             e = float(np.max(np.abs(reqs[i].price - pr)))
-            say("config 3 set %d (N=%d) %s: |price - oracle| <= %.3e" % (i, d["N"], prec, e))
+            say("256 sets, set %d (N=%d) %s: |price - oracle| <= %.3e" % (i, d["N"], prec, e))
             assert e <= 1e-4
             assert np.max(np.abs(reqs[i].stat - st)) <= 1e-4
             if prec == "fp16":
@@ -156,9 +156,9 @@ def test_config3_sets_match_oracle(rb, orc, bench):
                 assert np.max(np.abs(p64 - pr)) <= 1e-12 and np.max(np.abs(s64 - st)) <= 1e-12
 
 
-# ---------------------------------------------------------------------------------- config 4
+# ------------------------------------------------------------------------- long paths
 @pytest.mark.parametrize("N,M", [(1024, 4096), (2048, 2048), (4096, 1024)])
-def test_config4_long_paths_match_oracle(rb, orc, N, M):
+def test_long_paths_match_oracle(rb, orc, N, M):
     # split FP16: the Toeplitz GEMM kernel at N = 1024, the single-launch LDS-FFT kernel beyond
     K = list(np.linspace(0.7, 1.3, 31))
     T = [1.0] * 31
@@ -169,7 +169,7 @@ def test_config4_long_paths_match_oracle(rb, orc, N, M):
         for code in (rb.PREC_SPLIT_FP16, rb.PREC_FP32_FFT, rb.PREC_FP64):
             p, _, s = rb.price_v2(0.03, [H], [2.3], [-0.9], T, K, N, M, seed, precision=code)
             e = float(np.max(np.abs(p - pr)))
-            say("config 4 N=%d H=%.2f prec=%d: |price - oracle| <= %.3e" % (N, H, code, e))
+            say("long paths N=%d H=%.2f prec=%d: |price - oracle| <= %.3e" % (N, H, code, e))
             tol = 1e-12 if code == rb.PREC_FP64 else 1e-4
             assert e <= tol and np.max(np.abs(s - st)) <= tol
 
@@ -189,8 +189,8 @@ def test_fft_kernel_plain_mc_matches_oracle(rb, orc, N):
     assert np.max(np.abs(p - pr)) <= 1e-4 and np.max(np.abs(s - st)) <= 1e-4
 
 
-# ---------------------------------------------------------------------------------- config 5
-def test_config5_serving_slice(rb, bench):
+# ---------------------------------------------------------------------- serving trace
+def test_serving_trace_slice(rb, bench):
     """The first 384 requests of the 4096-request serving trace through the server with the
     benchmark's settings: each result is bit-identical to pricing the request alone, within
     1e-4 of the FP64 path, and the device timeline of the batches' kernels (timing mode) shows
@@ -199,7 +199,7 @@ def test_config5_serving_slice(rb, bench):
     ds = bench.serving_requests(384)
     assert [d["seed"] for d in ds] == [d["seed"] for d in bench.serving_requests(4096)[:384]]
     reqs = _requests(rb, ds)
-    say("config 5: 384 serving requests, closed loop")
+    say("serving trace: 384 requests, closed loop")
     with Server(rb.PREC_SPLIT_FP16, max_inflight=bench.SERVE_INFLIGHT, max_batch_requests=bench.SERVE_MAX_BATCH,
                 max_batch_path_steps=bench.SERVE_CAP) as srv:
         srv.set_timing(True)

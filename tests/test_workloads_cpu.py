@@ -6,19 +6,19 @@ import numpy as np
 import bench
 
 
-def test_config3_grid_covers_every_step_count():
-    from test_gpu_configs import CONFIG3_ORACLE_SETS
+def test_256_set_grid_covers_every_step_count():
+    from test_gpu_configs import SETS256_ORACLE_SETS
     ds = bench.batched_requests()
     assert len(ds) == 256
     assert sorted({d["N"] for d in ds}) == [32, 64, 128, 256, 512]
-    assert {ds[i]["N"] for i in CONFIG3_ORACLE_SETS} == {32, 64, 128, 256, 512}
+    assert {ds[i]["N"] for i in SETS256_ORACLE_SETS} == {32, 64, 128, 256, 512}
     for d in ds:
         assert 0.02 <= d["H"][0] <= 0.45 and 0.5 <= d["eta"][0] <= 4.0 and -0.99 <= d["rho"][0] <= -0.1
         assert 0.01 <= d["xi"] <= 0.16 and 0.05 <= d["T"][0] <= 2.0 and 2 ** 14 - 1 <= d["M"] <= 2 ** 20 + 1
         assert len(d["T"]) == 21 and np.allclose(d["K"], np.linspace(0.8, 1.2, 21))
 
 
-def test_config5_requests_span_the_grid():
+def test_serving_requests_span_the_grid():
     ds = bench.serving_requests(4096)
     assert len(ds) == 4096
     assert min(d["N"] for d in ds) == 32 and max(d["N"] for d in ds) == 512
