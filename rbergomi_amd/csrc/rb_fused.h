@@ -97,6 +97,7 @@ struct FusedReq {
     int log2TP;                  // log2(TP)
     int nunits, pgmax, out_size, gchunk, acc_in_lds;
     int log2slices;              // lanes per option in the payoff stage (register mode)
+    int pay2;                    // with one lane per option: options 0 .. pay2 - 1 take two (whole waves)
     int tiles_per_item;          // tiles of items 0 .. items_big - 1
     int mc;                      // 1 = plain Monte Carlo (wpart holds 3 sums per entry), 0 = RT
     long M, ntiles, nitems;

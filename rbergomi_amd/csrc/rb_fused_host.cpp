@@ -331,6 +331,10 @@ int plan_fused_request(const Request& req, const Layout& lay, int mode, FusedHos
     // lanes per option in the payoff stage: the largest power of two S <= 32 with S*options <= 256
     a.log2slices = 0;
     while (!a.acc_in_lds && a.log2slices < 5 && (a.out_size << (a.log2slices + 1)) <= 256) ++a.log2slices;
+    // with one lane per option (129 to 256 options) the lanes left over go to the first options,
+    // two lanes each, in whole waves (32 options a wave), so that no wave prices with most of its
+    // lanes idle: 205 options take 32 lane pairs and 173 single lanes
+    a.pay2 = !a.acc_in_lds && a.log2slices == 0 ? std::min(a.out_size, (256 - a.out_size) / 32 * 32) : 0;
     a.mc = req.method == METHOD_MC ? 1 : 0;
     a.M = req.M;
     a.ntiles = (req.M + a.TP - 1) / a.TP;

@@ -600,8 +600,10 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
                     // ---- 4. payoffs of the options of groups g0 .. g0+gn-1.  Register mode: the
                     //         S = 2^log2slices lanes t = o*S .. o*S+S-1 share option o, each
                     //         summing every S-th path; a fixed butterfly leaves the total in lane
-                    //         o*S, which owns option o's FP64 sums.  LDS mode (> 256 options):
-                    //         one thread per option, sums in LDS.
+                    //         o*S, which owns option o's FP64 sums.  With one lane per option,
+                    //         options 0 .. pay2-1 take lane pairs (threads 0 .. 2 pay2 - 1, whole
+                    //         waves) and option o >= pay2 thread o + pay2 (t = tid - pay2).  LDS
+                    //         mode (> 256 options): one thread per option, sums in LDS.
                     const int o_lo = uniform_const(A.groups)[g0].o_begin, o_hi = uniform_const(A.groups)[g0 + gn - 1].o_end;
                     const int np = A.M - p0 < A.TP ? int(A.M - p0) : A.TP;
                     if (A.acc_in_lds) {
@@ -620,8 +622,11 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
                             accopt[2 * oi + 1] += s2;
                         }
                     } else {
-                        const int S = 1 << A.log2slices;
-                        const int oi = tid >> A.log2slices, sl = tid & (S - 1);
+                        const bool pair = tid < 2 * A.pay2;                   // wave-uniform
+                        const int lg = pair ? 1 : A.log2slices;
+                        const int t = pair ? tid : tid - 2 * A.pay2 + (A.pay2 << A.log2slices);
+                        const int S = 1 << lg;
+                        const int oi = t >> lg, sl = t & (S - 1);
                         if (oi >= o_lo && oi < o_hi) {
                             const FusedOption O = load_global_rec(A.opts + oi);
                             const int pgl = O.pg - pg_lo;
@@ -659,10 +664,15 @@ __global__ void __launch_bounds__(256, UPW >= 8 ? 1 : 2) k_fused(FusedBatchArgs 
                 A.partial[(long(I) * A.nitems + item) * 2 + 0] = accopt[2 * o + 0];
                 A.partial[(long(I) * A.nitems + item) * 2 + 1] = accopt[2 * o + 1];
             }
-        } else if ((tid & ((1 << A.log2slices) - 1)) == 0 && (tid >> A.log2slices) < A.out_size) {
-            const int I = A.opts[tid >> A.log2slices].out;
-            A.partial[(long(I) * A.nitems + item) * 2 + 0] = a1;
-            A.partial[(long(I) * A.nitems + item) * 2 + 1] = a2;
+        } else {
+            const bool pair = tid < 2 * A.pay2;   // the payoff stage's lanes
+            const int lg = pair ? 1 : A.log2slices;
+            const int t = pair ? tid : tid - 2 * A.pay2 + (A.pay2 << A.log2slices);
+            if ((t & ((1 << lg) - 1)) == 0 && (t >> lg) < A.out_size) {
+                const int I = A.opts[t >> lg].out;
+                A.partial[(long(I) * A.nitems + item) * 2 + 0] = a1;
+                A.partial[(long(I) * A.nitems + item) * 2 + 1] = a2;
+            }
         }
         __syncthreads();
     }
